@@ -1,0 +1,276 @@
+#!/usr/bin/env python3
+"""Decoded info Gbit/s of the MI355X LDPC decoder on one GPU.
+
+Workload (the metric's configuration): 8192 DVB-S2 frames in one call,
+75 % normal / 25 % short, rates uniform over the DVB-S2 sets, modulation
+uniform among the rate's MODCODs (QPSK/8PSK/16APSK/32APSK), Es/N0 = QEF +
+0.3 dB, int8 max-log LLRs, max 50 iterations, per-frame early stop.  The
+LLRs are generated on the host, uploaded once, and stay in HBM; a step is
+one decode of the whole batch (kernel launches + synchronize).
+
+Printed: one JSON line with the throughput, the decode kernel's roofline
+(algorithmic bytes per launch / kernel time from HIP events on the
+decoder's stream), and the reference CPU decoder timed on this host's
+cores on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dvb-rx_amd"))
+
+import ldpcgpu  # noqa: E402
+import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz, GB/s
+
+
+def cpu_flags():
+    try:
+        return open("/proc/cpuinfo").read()
+    except OSError:
+        return ""
+
+
+def cpu_baseline(batch, seconds=15.0, max_threads=16):
+    """The reference decoder (libdvbrx ldpc.c built from its sources into
+    oracle/_ref) on this host: one thread per core, each with its own
+    handles, decoding batches of LANES frames of one code (the reference's
+    batch semantics).  Falls back to the oracle (a scalar port) where the
+    reference was not built."""
+    flags = cpu_flags()
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    cand = []
+    if "avx512bw" in flags:
+        cand.append(("v4", "AVX-512BW, 64 lanes"))
+    if "avx2" in flags:
+        cand.append(("v3", "AVX2, 32 lanes"))
+    lib = None
+    for name, desc in cand:
+        p = os.path.join(ref_dir, f"libdrxref_{name}.so")
+        if os.path.exists(p):
+            lib = C.CDLL(p)
+            kind, what = "reference", f"libdvbrx ldpc.c built -O3 -march=x86-64-{name} ({desc})"
+            break
+    if lib is None:
+        return None
+    vp = C.c_void_p
+    lib.drx_ldpc_new.restype = vp
+    lib.drx_ldpc_new.argtypes = [vp, C.c_char_p]
+    lib.drx_ldpc_free.argtypes = [vp]
+    lib.drx_ldpc_decode.argtypes = [vp, vp, vp, C.c_int, C.c_int]
+    lanes = lib.drx_ldpc_lanes()
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    nthreads = max(1, min(max_threads, len(cpus)))
+    # batches of LANES frames of one code, in workload order
+    by_code = {}
+    for f in range(batch.nframes):
+        by_code.setdefault(int(batch.codes[f]), []).append(f)
+    batches = []
+    for code, fl in by_code.items():
+        for i in range(0, len(fl), lanes):
+            batches.append(fl[i:i + lanes])
+    rng = np.random.default_rng(0)
+    rng.shuffle(batches)
+    lock = threading.Lock()
+    state = dict(next=0, frames=0, bits=0, iters=0, nb=0)
+    deadline = [0.0]
+    rate_of = {}
+    for f in range(batch.nframes):
+        s = batch.specs[f]
+        m = ldpcgpu.pls_lookup(s["pls"])
+        rate_of[f] = (m["short_frame"], m["rate"])
+
+    def worker(tid):
+        try:
+            os.sched_setaffinity(0, {cpus[tid % len(cpus)]})
+        except (AttributeError, OSError):
+            pass
+        handles = {}
+        out = np.zeros((lanes, 8100), np.uint8)
+        while True:
+            with lock:
+                i = state["next"]
+                # the sample cycles over the workload until the time is up
+                if time.perf_counter() > deadline[0] or i >= 64 * len(batches):
+                    break
+                state["next"] += 1
+            fl = batches[i % len(batches)]
+            short, rate = rate_of[fl[0]]
+            key = (short, rate)
+            if key not in handles:
+                fec = ldpcgpu.fec_lookup(short, rate)
+                p = fec.as_struct()
+                handles[key] = (lib.drx_ldpc_new(C.byref(p), rate.encode()), p)
+            h = handles[key][0]
+            n = len(fl)
+            ins = (C.c_void_p * n)(*[batch.frame_llr(f).ctypes.data for f in fl])
+            outs = (C.c_void_p * n)(*[out[k].ctypes.data for k in range(n)])
+            it = lib.drx_ldpc_decode(h, ins, outs, n, batch.max_iter)
+            with lock:
+                state["frames"] += n
+                state["bits"] += int(sum(batch.kldpc[f] for f in fl))
+                state["iters"] += it if it >= 0 else batch.max_iter
+                state["nb"] += 1
+        for h, p in handles.values():
+            lib.drx_ldpc_free(h)
+
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
+    t0 = time.perf_counter()
+    deadline[0] = t0 + seconds
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    dt = time.perf_counter() - t0
+    return {
+        "value": state["bits"] / dt / 1e9,
+        "unit": "Gbit/s",
+        "cores": nthreads,
+        "kind": kind,
+        "sample": (f"{state['frames']} frames decoded, cycling over the {batch.nframes} of the workload ({state['nb']} batches of <= {lanes} frames of one "
+                   f"code, shuffled), {dt:.1f} s wall, {nthreads} pinned threads, {what}; batch iterations avg "
+                   f"{state['iters'] / max(1, state['nb']):.1f}"),
+        "frames_per_s": state["frames"] / dt,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=8192)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: frames are independent, so N GPUs would each decode their "
+              "own share of the frame table with no exchange; run it with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.time()
+    batch = workloads.mixed_s2(args.frames)
+    print(f"# generated {batch.nframes} frames ({batch.llr.nbytes / 1e6:.0f} MB LLRs) in {time.time() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    dec = ldpcgpu.Decoder(0)
+    d_llr = dec.malloc(batch.llr.nbytes)
+    d_out = dec.malloc(batch.ref.nbytes)
+    d_it = dec.malloc(4 * batch.nframes)
+    d_ok = dec.malloc(batch.nframes)
+    dec.upload(d_llr, batch.llr)
+
+    for _ in range(args.warmup):
+        dec.decode_batch(batch.frames, d_llr, d_out, d_it, d_ok)
+    dec.synchronize()
+    kernel_ms = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        dec.decode_batch(batch.frames, d_llr, d_out, d_it, d_ok)
+        dec.synchronize()
+        kernel_ms.append(dec.last_kernel_ms())
+    dt = time.perf_counter() - t
+    out = np.zeros_like(batch.ref)
+    it = np.zeros(batch.nframes, np.int32)
+    ok = np.zeros(batch.nframes, np.uint8)
+    dec.download(out, d_out)
+    dec.download(it, d_it)
+    dec.download(ok, d_ok)
+    frame_err = 0
+    for f in range(batch.nframes):
+        frame_err += int(not np.array_equal(batch.frame_ref(f), out[batch.layout[f][4]:batch.layout[f][4] + (batch.kldpc[f] + 7) // 8]))
+    ms_step = dt / args.steps * 1e3
+    value = batch.info_bits / (dt / args.steps) / 1e9
+
+    # roofline of the decode kernel: algorithmic bytes per launch
+    kms = float(np.mean(kernel_ms))
+    hbm_bytes = int(sum(int(batch.n[f]) + (int(batch.kldpc[f]) + 7) // 8 + 8 for f in range(batch.nframes)))
+    edges = batch.edges()
+    iters_total = int(it.sum())
+    # LDS: per iteration read + write of posterior and message per edge (4E),
+    # one read per edge for each syndrome pass (iterations + 1 per frame)
+    per_frame_E = {}
+    lds_bytes = 0
+    for f in range(batch.nframes):
+        c = int(batch.codes[f])
+        if c not in per_frame_E:
+            per_frame_E[c] = ldpcgpu.code_params(c)["links_total"]
+        E = per_frame_E[c]
+        lds_bytes += 4 * E * int(it[f]) + E * (int(it[f]) + 1)
+    hbm_ach = hbm_bytes / (kms * 1e-3) / 1e9
+    lds_ach = lds_bytes / (kms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_mixed_s2_x8192.json")
+    if os.path.exists(tpath) and args.frames == 8192:
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(batch, args.cpu_seconds)
+    dec.close()
+
+    res = {
+        "metric": "decoded info Gbit/s on 1 MI355X (plain min-sum, bit-exact with the reference)",
+        "value": round(value, 3),
+        "unit": "Gbit/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic: own BCH+LDPC encoder, AWGN, int8 max-log LLRs, resident in HBM",
+        "config": {
+            "workload": f"mixed DVB-S2 batch, {batch.nframes} frames (75% normal / 25% short), all S2 rates, "
+                        "QPSK/8PSK/16APSK/32APSK, Es/N0 = QEF + 0.3 dB, max 50 iterations, per-frame early stop",
+            "frames": batch.nframes,
+            "info_bits_per_step": batch.info_bits,
+            "avg_iterations": round(float(it.mean()), 3),
+            "fer": frame_err / batch.nframes,
+            "frames_unconverged": int((ok == 0).sum()),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(hbm_ach, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(hbm_ach / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "kernel": "ldpc_layered_kernel",
+            "kernel_ms": round(kms, 4),
+            "algorithmic_bytes": hbm_bytes,
+        },
+        "roofline_lds": {
+            "achieved": round(lds_ach, 1),
+            "peak": LDS_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(lds_ach / LDS_PEAK_GBS, 5),
+            "algorithmic_bytes": lds_bytes,
+            "edges_per_iteration": edges,
+            "iterations_total": iters_total,
+        },
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        res["gpu_vs_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,202 @@
+/*
+ * LDPC code registry and layer-schedule compiler, see ldpc_codes.h.
+ */
+#include "ldpc_codes.h"
+
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "dvbs2_table_types.h"
+#include "dvbs2_tables.inc"
+
+#define NCODES ((int)(sizeof(LG_CODES) / sizeof(*LG_CODES)))
+
+static lg_code g_codes[sizeof(LG_CODES) / sizeof(*LG_CODES)];
+static pthread_once_t g_once = PTHREAD_ONCE_INIT;
+static int g_build_failed;
+
+/* Layer i holds the table addresses x with x mod q == i; node j of that
+ * layer is check q*j + i, and the bit 360*row + m reaches check
+ * (x + m*q) mod R (ldpc.c:240-249), i.e. node (x div q + m) mod 360. */
+static int build_code(lg_code* c, int id)
+{
+    const struct lg_code_def* t = &LG_CODES[id];
+    memset(c, 0, sizeof(*c));
+    c->id = id;
+    c->name = t->name;
+    c->n = t->n;
+    c->k = t->k;
+    c->r = t->n - t->k;
+    c->q = c->r / LG_M;
+    c->short_frame = t->n == 16200;
+    c->links_total = t->links_total;
+    c->links_max_cn = t->links_max_cn;
+    const int q = c->q;
+
+    int naddr = 0;
+    for (int g = 0; g < t->ngroups; g++)
+        naddr += LG_GROUPS[t->group_off + g].deg * LG_GROUPS[t->group_off + g].rows;
+    c->nent = naddr;
+    c->layer_start = calloc((size_t)q + 1, sizeof(uint16_t));
+    c->ent = malloc(sizeof(uint32_t) * (size_t)naddr);
+    c->level = calloc((size_t)c->r, 1);
+    c->nlev = calloc((size_t)q, 1);
+    int* cnt = calloc((size_t)q, sizeof(int));
+    if (!c->layer_start || !c->ent || !c->level || !c->nlev || !cnt) {
+        free(cnt);
+        return -1;
+    }
+    /* count per layer, then place (stable: table order within a layer) */
+    const uint16_t* a = LG_ADDR + t->addr_off;
+    for (int i = 0; i < naddr; i++)
+        cnt[a[i] % q]++;
+    for (int i = 0; i < q; i++)
+        c->layer_start[i + 1] = (uint16_t)(c->layer_start[i] + cnt[i]);
+    memset(cnt, 0, sizeof(int) * (size_t)q);
+    int row = 0, p = 0;
+    for (int g = 0; g < t->ngroups; g++) {
+        const int deg = LG_GROUPS[t->group_off + g].deg;
+        for (int rr = 0; rr < LG_GROUPS[t->group_off + g].rows; rr++, row++) {
+            for (int e = 0; e < deg; e++, p++) {
+                const int x = a[p], li = x % q;
+                c->ent[c->layer_start[li] + cnt[li]++] = (uint32_t)(row * LG_M) << 16 | (uint32_t)(x / q);
+            }
+        }
+    }
+    free(cnt);
+    if (row * LG_M != c->k)
+        return -1;
+
+    /* levels: owner[b] = last node of the current layer touching bit b */
+    int16_t* owner = malloc(sizeof(int16_t) * (size_t)c->k);
+    if (!owner)
+        return -1;
+    for (int i = 0; i < q; i++) {
+        const int s = c->layer_start[i], d = c->layer_start[i + 1] - s;
+        if (d > c->max_layer_deg)
+            c->max_layer_deg = d;
+        if (d + 2 > LG_MAX_CN_DEG) {
+            free(owner);
+            return -1;
+        }
+        for (int e = 0; e < d; e++) {
+            const int base = (int)(c->ent[s + e] >> 16);
+            for (int m = 0; m < LG_M; m++)
+                owner[base + m] = -1;
+        }
+        int maxl = 0;
+        uint8_t* lv = c->level + (size_t)i * LG_M;
+        for (int j = 0; j < LG_M; j++) {
+            int l = 0;
+            for (int e = 0; e < d; e++) {
+                const int base = (int)(c->ent[s + e] >> 16), sh = (int)(c->ent[s + e] & 0xffff);
+                const int b = base + (j - sh + LG_M) % LG_M;
+                if (owner[b] >= 0 && lv[owner[b]] + 1 > l)
+                    l = lv[owner[b]] + 1;
+                owner[b] = (int16_t)j;
+            }
+            if (l > 255) {
+                free(owner);
+                return -1;
+            }
+            lv[j] = (uint8_t)l;
+            if (l)
+                c->dep_nodes++;
+            if (l > maxl)
+                maxl = l;
+        }
+        c->nlev[i] = (uint8_t)(maxl + 1);
+        c->level_steps += maxl + 1;
+        if (maxl + 1 > c->max_levels)
+            c->max_levels = maxl + 1;
+    }
+    free(owner);
+    /* every edge counted: info edges plus the 2R-1 of the staircase */
+    if ((long)naddr * LG_M + 2L * c->r - 1 != c->links_total)
+        return -1;
+    return 0;
+}
+
+static void build_all(void)
+{
+    for (int i = 0; i < NCODES; i++)
+        if (build_code(&g_codes[i], i))
+            g_build_failed = 1;
+}
+
+int lg_code_count(void) { return NCODES; }
+
+const lg_code* lg_code_get(int id)
+{
+    if (id < 0 || id >= NCODES)
+        return NULL;
+    pthread_once(&g_once, build_all);
+    if (g_build_failed)
+        return NULL;
+    return &g_codes[id];
+}
+
+int lg_code_lookup(int short_frame, const char* rate)
+{
+    if (!rate)
+        return -1;
+    for (size_t i = 0; i < sizeof(LG_RATES) / sizeof(*LG_RATES); i++)
+        if (LG_RATES[i].short_frame == (short_frame != 0) && strcmp(LG_RATES[i].rate, rate) == 0)
+            return LG_RATES[i].code;
+    return -1;
+}
+
+/* "2_3_t2" / "3_5_t2" (the DVB-T2 codes) take the BCH parameters of their
+ * DVB-S2 rate, as libdvbrx/src/dvbt2.c:1963-1967 does. */
+static int same_rate(const char* row, const char* rate)
+{
+    const size_t n = strlen(rate);
+    if (n > 3 && strcmp(rate + n - 3, "_t2") == 0)
+        return strlen(row) == n - 3 && strncmp(row, rate, n - 3) == 0;
+    return strcmp(row, rate) == 0;
+}
+
+int lg_fec_lookup(int short_frame, const char* rate, int* kbch, int* nbch, int* t)
+{
+    if (!rate)
+        return -1;
+    for (size_t i = 0; i < sizeof(LG_FEC) / sizeof(*LG_FEC); i++) {
+        if (LG_FEC[i].short_frame == (short_frame != 0) && same_rate(LG_FEC[i].rate, rate)) {
+            if (kbch)
+                *kbch = LG_FEC[i].kbch;
+            if (nbch)
+                *nbch = LG_FEC[i].nbch;
+            if (t)
+                *t = LG_FEC[i].t;
+            return 0;
+        }
+    }
+    return -1;
+}
+
+int lg_fec_index(int short_frame, const char* rate)
+{
+    int n = 0;
+    if (!rate)
+        return -1;
+    for (size_t i = 0; i < sizeof(LG_FEC) / sizeof(*LG_FEC); i++) {
+        if (LG_FEC[i].short_frame != (short_frame != 0))
+            continue;
+        if (same_rate(LG_FEC[i].rate, rate))
+            return n;
+        n++;
+    }
+    return -1;
+}
+
+const struct lg_modcod_row* lg_modcod_get(int pls)
+{
+    pls &= 0xFE;
+    for (size_t i = 0; i < sizeof(LG_MODCODS) / sizeof(*LG_MODCODS); i++)
+        if (LG_MODCODS[i].pls == pls)
+            return &LG_MODCODS[i];
+    return NULL;
+}
+
+const float* lg_points(void) { return LG_POINTS; }

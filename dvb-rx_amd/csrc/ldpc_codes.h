@@ -1,0 +1,76 @@
+/*
+ * LDPC code registry and layer-schedule compiler (host C).
+ *
+ * Every DVB-S2/S2X/T2 code is quasi-cyclic with circulant size 360.  The
+ * reference expands the address tables into one index list per check node
+ * (libdvbrx/src/ldpc.c:235-255) and visits the R check nodes in "layer
+ * order": check q*j+i is node j of layer i (ldpc.c:251-255).  Here the same
+ * graph is kept in its circulant form: within layer i every node j has the
+ * same info degree, and its e-th information neighbour is
+ *
+ *     bit = base_e + ((j - shift_e) mod 360),
+ *
+ * for one (base_e = 360 * table row, shift_e = address div q) pair per
+ * table address congruent to i mod q.  A layer is thus described by its
+ * degree and a handful of (base, shift) pairs instead of 360 index lists.
+ *
+ * The reference updates the 360 nodes of a layer one after another with
+ * the posteriors updated in place (ldpc.c:474-480), so a node that shares
+ * an information bit with an earlier node of the same layer sees that
+ * node's update.  The schedule compiler assigns each node the level
+ * 1 + max(level of the earlier nodes it shares a bit with); running the
+ * levels of a layer in order, with the nodes of one level in parallel,
+ * reproduces the sequential result exactly.
+ */
+#ifndef LDPC_CODES_H
+#define LDPC_CODES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LG_M 360          /* circulant size */
+#define LG_MAX_CN_DEG 32  /* check node degree bound (tables: <= 30) */
+
+typedef struct lg_code {
+    int id;           /* index in the registry */
+    const char* name; /* table name, e.g. "S2_B4" */
+    int n, k, r, q;   /* codeword bits, info bits, parity bits, layers */
+    int short_frame;
+    int links_total, links_max_cn;
+    int nent;             /* (base, shift) pairs, all layers */
+    int max_layer_deg;    /* largest info degree of a layer */
+    int level_steps;      /* sum over layers of their level count */
+    int max_levels;       /* largest level count of one layer */
+    int dep_nodes;        /* nodes at level > 0 */
+    uint16_t* layer_start; /* q + 1 offsets into ent */
+    uint32_t* ent;         /* base << 16 | shift */
+    uint8_t* level;        /* r: level of node j of layer i at i*360+j */
+    uint8_t* nlev;         /* q: level count of each layer */
+} lg_code;
+
+int lg_code_count(void);
+/* Expanded code, or NULL for an out-of-range id.  Built once, immutable. */
+const lg_code* lg_code_get(int id);
+/* Code used for (frame size, rate name) as libdvbrx's find_table
+ * (ldpc.c:188-197): "1_2", "2_3_t2", "11_45_VLSNR_SF2", ... -1 if none. */
+int lg_code_lookup(int short_frame, const char* rate);
+/* BCH parameters of drx_fec_lookup (libdvbrx/src/bch.c:25-41); the T2
+ * codes "2_3_t2" / "3_5_t2" take those of their DVB-S2 rate. */
+int lg_fec_lookup(int short_frame, const char* rate, int* kbch, int* nbch, int* t);
+/* Position of a rate among the BCH rows of its frame size, -1 if none. */
+int lg_fec_index(int short_frame, const char* rate);
+
+struct lg_modcod_row;
+/* MODCOD row of a PLS code (pilot bit ignored), NULL if unknown. */
+const struct lg_modcod_row* lg_modcod_get(int pls);
+/* Constellation points, (re, im) pairs, indexed by lg_modcod_row.point_off. */
+const float* lg_points(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

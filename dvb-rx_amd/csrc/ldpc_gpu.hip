@@ -1,0 +1,702 @@
+/*
+ * MI355X (gfx950) layered offset min-sum LDPC decoder: kernels and the
+ * C-ABI of include/ldpc_gpu.h.
+ *
+ * Arithmetic (bit-exact with libdvbrx/src/ldpc.c:321-362): for a check node
+ * with edges e, posteriors L_e and stored messages m_e,
+ *   x_e  = sat8(L_e - m_e)            (first iteration: x_e = L_e)
+ *   a_e  = |x_e| in 0..128, min0 <= min1 the two smallest a_e (with
+ *          multiplicity), s = xor of the signs of x_e
+ *   mn0  = max(min(min0, 127) - 1, 0), mn1 likewise       (offset 1)
+ *   o_e  = (s ^ sign x_e) ? -mag : mag, mag = a_e == min0 ? mn1 : mn0
+ *   L_e <- sat8(x_e + o_e),  m_e <- clamp(o_e, -32, 31)
+ * Messages are kept per node as (mn0, mn1, one argmin edge, sign bits),
+ * from which every clamped m_e is recovered exactly (the compressed store
+ * of ldpc.c:376-462 keeps the same information).
+ *
+ * Work layout: one workgroup owns one frame at a time (a persistent grid
+ * takes frames from an atomic ticket).  The frame's N posteriors live in
+ * LDS: information bits 0..K-1, then the R parity bits in layer order
+ * (parity of check q*j+i at K + 360*i + j, ldpc.c:624-626).  Thread j
+ * processes node j of every layer, so the message of node (i, j) is only
+ * ever touched by thread j and lives in global memory (L2 / MALL resident)
+ * at [i*360 + j], coalesced across the wavefronts.  Within a layer the
+ * nodes run level by level (ldpc_codes.h), with a workgroup barrier after
+ * each level; this reproduces the reference's sequential node order.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "ldpc_codes.h"
+#include "ldpc_gpu.h"
+#include "ldpc_node.h"
+
+#define NT 384 /* threads per workgroup: 360 check nodes of a layer */
+
+namespace {
+
+struct DCode {
+    int n, k, r, q;
+    int ent_off; /* into ent: base << 16 | shift */
+    int lay_off; /* into lay: start | deg << 16 | nlev << 24 */
+    int lev_off; /* into lev: one byte per node */
+    int pad;
+};
+
+struct DFrame {
+    int code, max_iter, nbits, pad;
+    unsigned long long llr_off, out_off;
+};
+
+struct KArgs {
+    const DCode* codes;
+    const uint32_t* ent;
+    const uint32_t* lay;
+    const uint8_t* lev;
+    const DFrame* frames;
+    const int* list;   /* frames to run (nullptr: 0..nrun-1) */
+    int nrun;
+    const int8_t* llr;
+    uint8_t* out;
+    int32_t* iters;
+    uint8_t* synd;
+    uint2* msg;        /* messages: per workgroup or per frame */
+    size_t msg_stride; /* uint2 per message block */
+    int msg_per_frame;
+    int8_t* state;     /* group mode: posteriors saved per frame (N_max bytes) */
+    size_t state_stride;
+    int32_t* it_state; /* group mode: iterations done per frame */
+    const int32_t* target; /* resume mode: iteration to reach per frame */
+    int resume;        /* 0: decode from the LLRs with per-frame early stop;
+                          1: continue from the saved state up to target */
+    int save;          /* save posteriors + iteration count at the end */
+    unsigned* ticket;
+};
+
+__device__ __forceinline__ void node_update(int8_t* sm, const uint32_t* ent, int deg, int i, int j, int K,
+                                            int q, uint2* mp, int first)
+{
+    uint32_t w0 = 0, sg = 0;
+    if (!first) {
+        const uint2 w = *mp;
+        w0 = w.x;
+        sg = w.y;
+    }
+    lg_node_update(sm, ent, deg, i, j, K, q, &w0, &sg, first);
+    *mp = make_uint2(w0, sg);
+}
+
+/* all R checks hold (bad(), ldpc.c:286-305) */
+__device__ __forceinline__ int syndrome_ok(const int8_t* sm, const DCode& cd, const uint32_t* ent,
+                                           const uint32_t* lay, int j)
+{
+    int ok = 1;
+    if (j < 360) {
+        for (int i = 0; i < cd.q; i++) {
+            const uint32_t ly = lay[i];
+            const int st = ly & 0xffff, deg = (ly >> 16) & 0xff;
+            ok &= lg_node_check(sm, ent + st, deg, i, j, cd.k, cd.q);
+        }
+    }
+    return __syncthreads_and(ok);
+}
+
+__global__ void __launch_bounds__(NT) reset_ticket(unsigned* t) { *t = 0; }
+
+__global__ void __launch_bounds__(NT) ldpc_layered_kernel(KArgs a)
+{
+    extern __shared__ __align__(16) int8_t sm[];
+    __shared__ int s_run;
+    const int tid = threadIdx.x;
+
+    for (;;) {
+        if (tid == 0)
+            s_run = (int)atomicAdd(a.ticket, 1u);
+        __syncthreads();
+        const int run = s_run;
+        __syncthreads();
+        if (run >= a.nrun)
+            break;
+        const int f = a.list ? a.list[run] : run;
+        const DFrame fr = a.frames[f];
+        const DCode cd = a.codes[fr.code];
+        const uint32_t* ent = a.ent + cd.ent_off;
+        const uint32_t* lay = a.lay + cd.lay_off;
+        const uint8_t* lev = a.lev + cd.lev_off;
+        const int K = cd.k, N = cd.n, R = cd.r, q = cd.q;
+        uint2* msg = a.msg + a.msg_stride * (size_t)(a.msg_per_frame ? f : blockIdx.x);
+
+        int it;
+        int target;
+        if (!a.resume) {
+            /* info bits straight, parity transposed to layer order */
+            const uint2* src = (const uint2*)(a.llr + fr.llr_off);
+            for (int v = tid; v < K / 8; v += NT)
+                ((uint2*)sm)[v] = src[v];
+            const int8_t* par = a.llr + fr.llr_off + K;
+            for (int t = tid; t < R; t += NT) {
+                const int jj = t / q, ii = t - jj * q;
+                sm[K + 360 * ii + jj] = par[t];
+            }
+            it = 0;
+            target = fr.max_iter;
+        } else {
+            const uint2* src = (const uint2*)(a.state + a.state_stride * (size_t)f);
+            for (int v = tid; v < N / 8; v += NT)
+                ((uint2*)sm)[v] = src[v];
+            it = a.it_state[f];
+            target = a.target[f];
+        }
+        __syncthreads();
+
+        const int j = tid;
+        int ok;
+        for (;;) {
+            if (!a.resume) {
+                ok = syndrome_ok(sm, cd, ent, lay, j);
+                if (ok || it >= target)
+                    break;
+            } else if (it >= target) {
+                ok = syndrome_ok(sm, cd, ent, lay, j);
+                break;
+            }
+            const int first = it == 0;
+            for (int i = 0; i < q; i++) {
+                const uint32_t ly = lay[i];
+                const int st = ly & 0xffff, deg = (ly >> 16) & 0xff, nlev = ly >> 24;
+                const int my = j < 360 ? lev[360 * i + j] : 255;
+                for (int l = 0; l < nlev; l++) {
+                    if (my == l)
+                        node_update(sm, ent + st, deg, i, j, K, q, msg + 360 * i + j, first);
+                    __syncthreads();
+                }
+            }
+            it++;
+        }
+
+        /* hard decisions of the first nbits bits, MSB first */
+        const int nbytes = (fr.nbits + 7) / 8;
+        uint8_t* o = a.out + fr.out_off;
+        for (int b = tid; b < nbytes; b += NT) {
+            int v = 0;
+            for (int k = 0; k < 8; k++) {
+                const int bit = 8 * b + k;
+                v = v << 1 | (bit < fr.nbits && sm[bit] < 0);
+            }
+            o[b] = (uint8_t)v;
+        }
+        if (a.save) {
+            uint2* dst = (uint2*)(a.state + a.state_stride * (size_t)f);
+            for (int v = tid; v < N / 8; v += NT)
+                dst[v] = ((const uint2*)sm)[v];
+        }
+        if (tid == 0) {
+            a.iters[f] = it;
+            a.synd[f] = (uint8_t)ok;
+            if (a.save)
+                a.it_state[f] = it;
+        }
+        __syncthreads();
+    }
+}
+
+/* ---- device-resident code tables (one copy per device) ------------------ */
+
+struct DeviceTables {
+    int ready = 0;
+    DCode* codes = nullptr;
+    uint32_t* ent = nullptr;
+    uint32_t* lay = nullptr;
+    uint8_t* lev = nullptr;
+    std::vector<DCode> h_codes;
+};
+
+std::mutex g_tables_lock;
+DeviceTables g_tables[64];
+
+} // namespace
+
+struct ldpc_gpu_ctx {
+    int device;
+    hipStream_t stream;
+    char err[256];
+    DeviceTables* tab;
+    int ncu;
+    /* scratch */
+    unsigned* ticket;
+    DFrame* d_frames;
+    size_t frames_bytes;
+    int* d_list;
+    size_t list_bytes;
+    uint2* msg;
+    size_t msg_bytes;
+    int8_t* state;
+    size_t state_bytes;
+    int32_t* it_state;
+    size_t it_state_bytes;
+    int32_t* target;
+    size_t target_bytes;
+    hipEvent_t ev0, ev1;
+    int timed;
+};
+
+static int set_err(ldpc_gpu_ctx* c, int code, const char* fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof(c->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return set_err((ctx), LDPC_GPU_EDEVICE, "%s: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+static int upload_tables(ldpc_gpu_ctx* c)
+{
+    std::lock_guard<std::mutex> lk(g_tables_lock);
+    if (c->device < 0 || c->device >= 64)
+        return set_err(c, LDPC_GPU_EINVAL, "device %d out of range", c->device);
+    DeviceTables& t = g_tables[c->device];
+    c->tab = &t;
+    if (t.ready)
+        return LDPC_GPU_OK;
+    const int nc = lg_code_count();
+    std::vector<DCode> codes(nc);
+    std::vector<uint32_t> ent, lay;
+    std::vector<uint8_t> lev;
+    for (int id = 0; id < nc; id++) {
+        const lg_code* cd = lg_code_get(id);
+        if (!cd)
+            return set_err(c, LDPC_GPU_EINVAL, "code %d failed to build", id);
+        DCode& d = codes[id];
+        d.n = cd->n;
+        d.k = cd->k;
+        d.r = cd->r;
+        d.q = cd->q;
+        d.ent_off = (int)ent.size();
+        d.lay_off = (int)lay.size();
+        d.lev_off = (int)lev.size();
+        d.pad = 0;
+        ent.insert(ent.end(), cd->ent, cd->ent + cd->nent);
+        for (int i = 0; i < cd->q; i++) {
+            const int st = cd->layer_start[i], deg = cd->layer_start[i + 1] - st;
+            lay.push_back((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24);
+        }
+        lev.insert(lev.end(), cd->level, cd->level + cd->r);
+    }
+    HIPCHK(c, hipMalloc((void**)&t.codes, sizeof(DCode) * codes.size()));
+    HIPCHK(c, hipMalloc((void**)&t.ent, sizeof(uint32_t) * ent.size()));
+    HIPCHK(c, hipMalloc((void**)&t.lay, sizeof(uint32_t) * lay.size()));
+    HIPCHK(c, hipMalloc((void**)&t.lev, lev.size()));
+    HIPCHK(c, hipMemcpy(t.codes, codes.data(), sizeof(DCode) * codes.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.ent, ent.data(), sizeof(uint32_t) * ent.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.lay, lay.data(), sizeof(uint32_t) * lay.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.lev, lev.data(), lev.size(), hipMemcpyHostToDevice));
+    t.h_codes = codes;
+    t.ready = 1;
+    return LDPC_GPU_OK;
+}
+
+extern "C" {
+
+int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
+{
+    if (!out)
+        return LDPC_GPU_EINVAL;
+    *out = nullptr;
+    ldpc_gpu_ctx* c = (ldpc_gpu_ctx*)calloc(1, sizeof(ldpc_gpu_ctx));
+    if (!c)
+        return LDPC_GPU_ENOMEM;
+    c->device = device;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || device < 0 || device >= ndev) {
+        free(c);
+        return LDPC_GPU_EDEVICE;
+    }
+    int rc;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        free(c);
+        return LDPC_GPU_EDEVICE;
+    }
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        c->ncu = 256;
+    if ((rc = upload_tables(c)) != LDPC_GPU_OK || hipMalloc((void**)&c->ticket, 64) != hipSuccess) {
+        ldpc_gpu_destroy(c);
+        return rc ? rc : LDPC_GPU_EDEVICE;
+    }
+    *out = c;
+    return LDPC_GPU_OK;
+}
+
+void ldpc_gpu_destroy(ldpc_gpu_ctx* c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->ticket);
+    (void)hipFree(c->d_frames);
+    (void)hipFree(c->d_list);
+    (void)hipFree(c->msg);
+    (void)hipFree(c->state);
+    (void)hipFree(c->it_state);
+    (void)hipFree(c->target);
+    if (c->ev0)
+        (void)hipEventDestroy(c->ev0);
+    if (c->ev1)
+        (void)hipEventDestroy(c->ev1);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    free(c);
+}
+
+void* ldpc_gpu_stream(ldpc_gpu_ctx* c) { return c ? (void*)c->stream : nullptr; }
+const char* ldpc_gpu_last_error(ldpc_gpu_ctx* c) { return c ? c->err : "no context"; }
+
+int ldpc_gpu_code_count(void) { return lg_code_count(); }
+int ldpc_gpu_code_lookup(int short_frame, const char* rate) { return lg_code_lookup(short_frame, rate); }
+
+int ldpc_gpu_code_params(int code, int* n, int* k, int* q, int* links_total, int* level_steps)
+{
+    const lg_code* c = lg_code_get(code);
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    if (n)
+        *n = c->n;
+    if (k)
+        *k = c->k;
+    if (q)
+        *q = c->q;
+    if (links_total)
+        *links_total = c->links_total;
+    if (level_steps)
+        *level_steps = c->level_steps;
+    return LDPC_GPU_OK;
+}
+
+int ldpc_gpu_fec_lookup(int short_frame, const char* rate, int* nldpc, int* kldpc, int* kbch, int* t)
+{
+    int kb, nb, tt;
+    if (lg_fec_lookup(short_frame, rate, &kb, &nb, &tt))
+        return LDPC_GPU_EINVAL;
+    if (nldpc)
+        *nldpc = short_frame ? 16200 : 64800;
+    if (kldpc)
+        *kldpc = nb;
+    if (kbch)
+        *kbch = kb;
+    if (t)
+        *t = tt;
+    return LDPC_GPU_OK;
+}
+
+void* ldpc_gpu_malloc(ldpc_gpu_ctx* c, size_t bytes)
+{
+    void* p = nullptr;
+    if (!c || hipSetDevice(c->device) != hipSuccess || hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
+        return nullptr;
+    return p;
+}
+
+int ldpc_gpu_free(ldpc_gpu_ctx* c, void* p)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipFree(p));
+    return LDPC_GPU_OK;
+}
+
+int ldpc_gpu_upload(ldpc_gpu_ctx* c, void* d_dst, const void* src, size_t bytes)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LDPC_GPU_OK;
+}
+
+int ldpc_gpu_download(ldpc_gpu_ctx* c, void* dst, const void* d_src, size_t bytes)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LDPC_GPU_OK;
+}
+
+int ldpc_gpu_synchronize(ldpc_gpu_ctx* c)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LDPC_GPU_OK;
+}
+
+float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* c)
+{
+    if (!c || !c->timed)
+        return -1.0f;
+    float ms = -1.0f;
+    if (hipEventSynchronize(c->ev1) != hipSuccess || hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess)
+        return -1.0f;
+    return ms;
+}
+
+} // extern "C"
+
+template <typename T>
+static int grow(ldpc_gpu_ctx* c, T** p, size_t* cap_bytes, size_t need)
+{
+    if (need <= *cap_bytes)
+        return LDPC_GPU_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(*p));
+    *p = nullptr;
+    *cap_bytes = 0;
+    HIPCHK(c, hipMalloc((void**)p, need));
+    *cap_bytes = need;
+    return LDPC_GPU_OK;
+}
+
+/* one launch of the decode kernel over nrun frames (list or identity) */
+static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const int* d_list, int nrun, int maxn, int maxr,
+                  const int8_t* d_llr, uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, int per_frame_msg,
+                  int nframes_total, int resume, int save, int timed)
+{
+    if (nrun <= 0)
+        return LDPC_GPU_OK;
+    const size_t lds = (size_t)maxn;
+    int per_cu = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_layered_kernel, NT, lds));
+    if (per_cu < 1)
+        return set_err(c, LDPC_GPU_EDEVICE, "kernel does not fit (LDS %zu)", lds);
+    const int grid = std::min(nrun, per_cu * c->ncu);
+    const size_t msg_stride = (size_t)maxr; /* uint2 per block */
+    const size_t blocks = per_frame_msg ? (size_t)nframes_total : (size_t)grid;
+    int rc = grow(c, &c->msg, &c->msg_bytes, blocks * msg_stride * sizeof(uint2));
+    if (rc)
+        return rc;
+    KArgs a;
+    memset(&a, 0, sizeof(a));
+    a.codes = c->tab->codes;
+    a.ent = c->tab->ent;
+    a.lay = c->tab->lay;
+    a.lev = c->tab->lev;
+    a.frames = d_frames;
+    a.list = d_list;
+    a.nrun = nrun;
+    a.llr = d_llr;
+    a.out = d_out;
+    a.iters = d_iters;
+    a.synd = d_synd;
+    a.msg = c->msg;
+    a.msg_stride = msg_stride;
+    a.msg_per_frame = per_frame_msg;
+    a.state = c->state;
+    a.state_stride = 64800;
+    a.it_state = c->it_state;
+    a.target = c->target;
+    a.resume = resume;
+    a.save = save;
+    a.ticket = c->ticket;
+    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(1), 0, c->stream, c->ticket);
+    if (timed)
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(ldpc_layered_kernel, dim3(grid), dim3(NT), lds, c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    if (timed) {
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        c->timed = 1;
+    }
+    return LDPC_GPU_OK;
+}
+
+extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
+                                     uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    if (nframes == 0)
+        return LDPC_GPU_OK;
+    if (nframes < 0 || !frames || !d_llr || !d_out || !d_iters || !d_synd)
+        return set_err(c, LDPC_GPU_EINVAL, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    const std::vector<DCode>& hc = c->tab->h_codes;
+    std::vector<DFrame> df(nframes);
+    int maxn = 0, maxr = 0, ngroup = 0;
+    for (int f = 0; f < nframes; f++) {
+        const ldpc_gpu_frame& s = frames[f];
+        if (s.code < 0 || s.code >= (int)hc.size())
+            return set_err(c, LDPC_GPU_EINVAL, "frame %d: unknown code %d", f, s.code);
+        const DCode& cd = hc[s.code];
+        if (s.max_iter < 0 || s.nbits < 0 || s.nbits > cd.k || (s.llr_off & 7))
+            return set_err(c, LDPC_GPU_EINVAL, "frame %d: bad max_iter/nbits/llr_off", f);
+        df[f].code = s.code;
+        df[f].max_iter = s.max_iter;
+        df[f].nbits = s.nbits ? s.nbits : cd.k;
+        df[f].pad = 0;
+        df[f].llr_off = s.llr_off;
+        df[f].out_off = s.out_off;
+        maxn = std::max(maxn, cd.n);
+        maxr = std::max(maxr, cd.r);
+        ngroup += s.group >= 0;
+    }
+    int rc = grow(c, &c->d_frames, &c->frames_bytes, sizeof(DFrame) * nframes);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_frames, df.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice, c->stream));
+    if (!ngroup)
+        return launch(c, c->d_frames, nullptr, nframes, maxn, maxr, d_llr, d_out, d_iters, d_synd, 0, nframes, 0, 0, 1);
+
+    /* Groups (the reference's batch semantics): decode every frame with its
+     * own early stop and keep its state; a group's count T is the largest
+     * of its frames' counts (max_iter if one never converged); the others
+     * continue to T; while some frame fails at T < max_iter, the whole
+     * group goes one more iteration.  Frames are deterministic, so this
+     * equals iterating the group in lock step. */
+    if ((rc = grow(c, &c->state, &c->state_bytes, (size_t)nframes * 64800)) ||
+        (rc = grow(c, &c->it_state, &c->it_state_bytes, sizeof(int32_t) * nframes)) ||
+        (rc = grow(c, &c->target, &c->target_bytes, sizeof(int32_t) * nframes)) ||
+        (rc = grow(c, &c->d_list, &c->list_bytes, sizeof(int) * nframes)))
+        return rc;
+    rc = launch(c, c->d_frames, nullptr, nframes, maxn, maxr, d_llr, d_out, d_iters, d_synd, 1, nframes, 0, 1, 1);
+    if (rc)
+        return rc;
+    std::vector<int32_t> it(nframes), tgt(nframes);
+    std::vector<uint8_t> ok(nframes);
+    HIPCHK(c, hipMemcpyAsync(it.data(), d_iters, sizeof(int32_t) * nframes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ok.data(), d_synd, nframes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    /* group -> (T, all ok, max_iter) */
+    std::vector<int> gid(nframes);
+    std::vector<int> gT, gmax, gall;
+    {
+        std::vector<int> keys;
+        for (int f = 0; f < nframes; f++) {
+            if (frames[f].group < 0) {
+                gid[f] = -1;
+                continue;
+            }
+            auto pos = std::find(keys.begin(), keys.end(), frames[f].group);
+            int g;
+            if (pos == keys.end()) {
+                g = (int)keys.size();
+                keys.push_back(frames[f].group);
+                gT.push_back(0);
+                gmax.push_back(frames[f].max_iter);
+                gall.push_back(1);
+            } else {
+                g = (int)(pos - keys.begin());
+            }
+            gid[f] = g;
+            if (frames[f].max_iter != gmax[g])
+                return set_err(c, LDPC_GPU_EINVAL, "frames of group %d differ in max_iter", frames[f].group);
+            gT[g] = std::max(gT[g], (int)it[f]);
+            gall[g] &= ok[f];
+        }
+    }
+    for (size_t g = 0; g < gT.size(); g++)
+        if (!gall[g])
+            gT[g] = gmax[g];
+    for (;;) {
+        std::vector<int> list;
+        for (int f = 0; f < nframes; f++) {
+            if (gid[f] < 0)
+                continue;
+            tgt[f] = gT[gid[f]];
+            if (it[f] < tgt[f])
+                list.push_back(f);
+        }
+        if (list.empty())
+            break;
+        HIPCHK(c, hipMemcpyAsync(c->target, tgt.data(), sizeof(int32_t) * nframes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->stream));
+        rc = launch(c, c->d_frames, c->d_list, (int)list.size(), maxn, maxr, d_llr, d_out, d_iters, d_synd, 1, nframes,
+                    1, 1, 0);
+        if (rc)
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(it.data(), d_iters, sizeof(int32_t) * nframes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(ok.data(), d_synd, nframes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        /* groups still failing below their cap go one more iteration */
+        std::vector<int> all(gT.size(), 1);
+        for (int f = 0; f < nframes; f++)
+            if (gid[f] >= 0)
+                all[gid[f]] &= ok[f];
+        int more = 0;
+        for (size_t g = 0; g < gT.size(); g++) {
+            if (!all[g] && gT[g] < gmax[g]) {
+                gT[g]++;
+                more = 1;
+            }
+        }
+        if (!more)
+            break;
+    }
+    return LDPC_GPU_OK;
+}
+
+extern "C" int ldpc_gpu_decode_host(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, const int8_t* llr,
+                                    size_t llr_bytes, uint8_t* out, size_t out_bytes, int32_t* iters, uint8_t* synd)
+{
+    if (!c || nframes < 0 || (nframes && (!frames || !llr || !out || !iters || !synd)))
+        return LDPC_GPU_EINVAL;
+    if (nframes == 0)
+        return LDPC_GPU_OK;
+    /* every frame's LLRs and output bytes inside the buffers given */
+    for (int f = 0; f < nframes; f++) {
+        int n = 0, k = 0;
+        if (ldpc_gpu_code_params(frames[f].code, &n, &k, nullptr, nullptr, nullptr))
+            return set_err(c, LDPC_GPU_EINVAL, "frame %d: unknown code %d", f, frames[f].code);
+        const uint64_t nb = ((uint64_t)(frames[f].nbits ? frames[f].nbits : k) + 7) / 8;
+        if (frames[f].llr_off + (uint64_t)n > llr_bytes || frames[f].out_off + nb > out_bytes)
+            return set_err(c, LDPC_GPU_EINVAL, "frame %d: LLRs or output outside the buffers", f);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int8_t* d_llr = nullptr;
+    uint8_t *d_out = nullptr, *d_synd = nullptr;
+    int32_t* d_it = nullptr;
+    int rc = LDPC_GPU_OK;
+    if (hipMalloc((void**)&d_llr, llr_bytes) != hipSuccess || hipMalloc((void**)&d_out, out_bytes ? out_bytes : 1) != hipSuccess ||
+        hipMalloc((void**)&d_it, sizeof(int32_t) * nframes) != hipSuccess || hipMalloc((void**)&d_synd, nframes) != hipSuccess) {
+        rc = set_err(c, LDPC_GPU_ENOMEM, "device allocation failed");
+    }
+    if (!rc && hipMemcpyAsync(d_llr, llr, llr_bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = set_err(c, LDPC_GPU_EDEVICE, "upload failed");
+    if (!rc && out_bytes && hipMemcpyAsync(d_out, out, out_bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = set_err(c, LDPC_GPU_EDEVICE, "upload failed");
+    if (!rc)
+        rc = ldpc_gpu_decode_batch(c, frames, nframes, d_llr, d_out, d_it, d_synd);
+    if (!rc && (hipMemcpyAsync(out, d_out, out_bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipMemcpyAsync(iters, d_it, sizeof(int32_t) * nframes, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipMemcpyAsync(synd, d_synd, nframes, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = set_err(c, LDPC_GPU_EDEVICE, "download failed");
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d_llr);
+    (void)hipFree(d_out);
+    (void)hipFree(d_it);
+    (void)hipFree(d_synd);
+    return rc;
+}

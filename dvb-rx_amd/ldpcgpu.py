@@ -1,0 +1,306 @@
+"""ctypes bindings of the MI355X LDPC decoder (lib/libldpcgpu.so, C-ABI in
+include/ldpc_gpu.h) and of the frame simulator (lib/libdvbs2sim.so,
+include/dvbs2_sim.h).
+
+The decoder runs only through the HIP library: if it is missing, loading
+raises; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+FRAME_DTYPE = np.dtype(
+    [
+        ("code", "<i4"),
+        ("max_iter", "<i4"),
+        ("nbits", "<i4"),
+        ("group", "<i4"),
+        ("llr_off", "<u8"),
+        ("out_off", "<u8"),
+    ],
+    align=True,
+)
+assert FRAME_DTYPE.itemsize == 32
+
+
+class SimSpec(C.Structure):
+    _fields_ = [
+        ("short_frame", C.c_int),
+        ("rate", C.c_char * 24),
+        ("pls", C.c_int),
+        ("esn0_db", C.c_double),
+        ("llr_off", C.c_uint64),
+        ("ref_off", C.c_uint64),
+        ("stream", C.c_uint64),
+    ]
+
+
+def _load(name):
+    path = os.path.join(LIBDIR, name)
+    if not os.path.exists(path):
+        raise OSError(f"{path} is not built: run `make -C dvb-rx_amd` (or __graft_entry__.build())")
+    return C.CDLL(path)
+
+
+_gpu = None
+_sim = None
+
+
+def gpu_lib():
+    """The HIP decoder library, with argument types declared."""
+    global _gpu
+    if _gpu is None:
+        L = _load("libldpcgpu.so")
+        vp, i32, sz = C.c_void_p, C.c_int, C.c_size_t
+        L.ldpc_gpu_create.argtypes = [i32, C.POINTER(vp)]
+        L.ldpc_gpu_destroy.argtypes = [vp]
+        L.ldpc_gpu_stream.argtypes = [vp]
+        L.ldpc_gpu_stream.restype = vp
+        L.ldpc_gpu_last_error.argtypes = [vp]
+        L.ldpc_gpu_last_error.restype = C.c_char_p
+        L.ldpc_gpu_code_count.argtypes = []
+        L.ldpc_gpu_code_lookup.argtypes = [i32, C.c_char_p]
+        ip = C.POINTER(C.c_int)
+        L.ldpc_gpu_code_params.argtypes = [i32, ip, ip, ip, ip, ip]
+        L.ldpc_gpu_fec_lookup.argtypes = [i32, C.c_char_p, ip, ip, ip, ip]
+        L.ldpc_gpu_malloc.argtypes = [vp, sz]
+        L.ldpc_gpu_malloc.restype = vp
+        L.ldpc_gpu_free.argtypes = [vp, vp]
+        L.ldpc_gpu_upload.argtypes = [vp, vp, vp, sz]
+        L.ldpc_gpu_download.argtypes = [vp, vp, vp, sz]
+        L.ldpc_gpu_synchronize.argtypes = [vp]
+        L.ldpc_gpu_decode_batch.argtypes = [vp, vp, i32, vp, vp, vp, vp]
+        L.ldpc_gpu_decode_host.argtypes = [vp, vp, i32, vp, sz, vp, sz, vp, vp]
+        L.ldpc_gpu_last_kernel_ms.argtypes = [vp]
+        L.ldpc_gpu_last_kernel_ms.restype = C.c_float
+        L.drx_ldpc_lanes.argtypes = []
+        L.drx_ldpc_new.argtypes = [vp, C.c_char_p]
+        L.drx_ldpc_new.restype = vp
+        L.drx_ldpc_free.argtypes = [vp]
+        L.drx_ldpc_decode.argtypes = [vp, vp, vp, i32, i32]
+        _gpu = L
+    return _gpu
+
+
+def sim_lib():
+    global _sim
+    if _sim is None:
+        L = _load("libdvbs2sim.so")
+        L.sim_generate.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int]
+        ip = C.POINTER(C.c_int)
+        L.sim_fec_lookup.argtypes = [C.c_int, C.c_char_p, ip, ip, ip, ip]
+        L.sim_pls_lookup.argtypes = [C.c_int, C.c_char_p, C.c_int, ip, ip, C.c_char_p, C.c_int]
+        L.sim_qef_db.argtypes = [C.c_int]
+        L.sim_qef_db.restype = C.c_double
+        L.sim_bch_encode.argtypes = [C.c_int, C.c_char_p, C.c_void_p, C.c_void_p]
+        L.sim_ldpc_encode.argtypes = [C.c_int, C.c_char_p, C.c_void_p, C.c_void_p]
+        _sim = L
+    return _sim
+
+
+# ---- codes -----------------------------------------------------------------
+
+
+class FecParams:
+    def __init__(self, short_frame, nldpc, kldpc, kbch, t):
+        self.short_frame, self.nldpc, self.kldpc, self.kbch, self.t = short_frame, nldpc, kldpc, kbch, t
+
+    def as_struct(self):
+        """drx_fec_params layout (libdvbrx/include/dvbrx/fec.h:13-19)."""
+        return (C.c_int * 5)(self.short_frame, self.nldpc, self.kldpc, self.kbch, self.t)
+
+
+def fec_lookup(short_frame, rate):
+    """drx_fec_lookup: BCH/LDPC sizes of a rate; the T2 codes ("2_3_t2",
+    "3_5_t2") use the DVB-S2 parameters of their rate."""
+    v = [C.c_int() for _ in range(4)]
+    if gpu_lib().ldpc_gpu_fec_lookup(int(short_frame), rate.encode(), *[C.byref(x) for x in v]):
+        raise KeyError(f"unknown rate {rate!r} (short={short_frame})")
+    return FecParams(int(short_frame), *[x.value for x in v])
+
+
+def code_lookup(short_frame, rate):
+    c = gpu_lib().ldpc_gpu_code_lookup(int(short_frame), rate.encode())
+    if c < 0:
+        raise KeyError(f"no LDPC code for rate {rate!r} (short={short_frame})")
+    return c
+
+
+def code_params(code):
+    v = [C.c_int() for _ in range(5)]
+    if gpu_lib().ldpc_gpu_code_params(code, *[C.byref(x) for x in v]):
+        raise KeyError(code)
+    n, k, q, lt, steps = [x.value for x in v]
+    return dict(n=n, k=k, q=q, r=n - k, links_total=lt, level_steps=steps)
+
+
+def pls_lookup(pls):
+    rate = C.create_string_buffer(24)
+    const = C.create_string_buffer(24)
+    sf, bits = C.c_int(), C.c_int()
+    if sim_lib().sim_pls_lookup(pls, rate, 24, C.byref(sf), C.byref(bits), const, 24):
+        raise KeyError(hex(pls))
+    return dict(rate=rate.value.decode(), short_frame=sf.value, bits=bits.value, constellation=const.value.decode())
+
+
+def qef_db(pls):
+    return sim_lib().sim_qef_db(pls)
+
+
+# ---- frame source ------------------------------------------------------------
+
+
+def generate(specs, seed, nthreads=None, with_ref=True):
+    """specs: list of dicts (short_frame, rate) or (pls), esn0_db, stream.
+    Returns (llr int8 buffer, ref uint8 buffer, layout list of
+    (code, n, kldpc, llr_off, ref_off))."""
+    n = len(specs)
+    arr = (SimSpec * n)()
+    layout = []
+    llr_off = ref_off = 0
+    for i, s in enumerate(specs):
+        if s.get("pls", -1) >= 0:
+            m = pls_lookup(s["pls"])
+            short, rate = m["short_frame"], m["rate"]
+        else:
+            short, rate = s["short_frame"], s["rate"]
+        fec = fec_lookup(short, rate)
+        code = code_lookup(short, rate)
+        arr[i].short_frame = short
+        arr[i].rate = rate.encode()
+        arr[i].pls = s.get("pls", -1)
+        arr[i].esn0_db = s["esn0_db"]
+        arr[i].llr_off = llr_off
+        arr[i].ref_off = ref_off
+        arr[i].stream = s.get("stream", i)
+        layout.append((code, fec.nldpc, fec.kldpc, llr_off, ref_off))
+        llr_off += fec.nldpc
+        ref_off += (fec.kldpc + 7) // 8
+    llr = np.zeros(llr_off, np.int8)
+    ref = np.zeros(ref_off, np.uint8) if with_ref else None
+    rc = sim_lib().sim_generate(
+        arr, n, C.c_uint64(seed), llr.ctypes.data, ref.ctypes.data if with_ref else None,
+        nthreads or min(16, os.cpu_count() or 1),
+    )
+    if rc:
+        raise RuntimeError("sim_generate failed")
+    return llr, ref, layout
+
+
+def frames_for(layout, max_iter, group=-1, out_offsets=None):
+    """Frame table for a generated layout (output offsets = reference offsets)."""
+    fr = np.zeros(len(layout), FRAME_DTYPE)
+    for i, (code, n, kldpc, lo, ro) in enumerate(layout):
+        fr[i] = (code, max_iter, kldpc, group if np.isscalar(group) else group[i], lo, ro if out_offsets is None else out_offsets[i])
+    return fr
+
+
+# ---- decoder ----------------------------------------------------------------
+
+
+class Decoder:
+    """A decoder context (device, stream, scratch) of the HIP library."""
+
+    def __init__(self, device=0):
+        self.L = gpu_lib()
+        self.ctx = C.c_void_p()
+        rc = self.L.ldpc_gpu_create(device, C.byref(self.ctx))
+        if rc:
+            raise RuntimeError(f"ldpc_gpu_create({device}) failed: {rc}")
+
+    def close(self):
+        if self.ctx:
+            self.L.ldpc_gpu_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def error(self):
+        return self.L.ldpc_gpu_last_error(self.ctx).decode()
+
+    def _check(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what} failed ({rc}): {self.error()}")
+
+    # device memory
+    def malloc(self, nbytes):
+        p = self.L.ldpc_gpu_malloc(self.ctx, nbytes)
+        if not p:
+            raise MemoryError(nbytes)
+        return p
+
+    def free(self, p):
+        self._check(self.L.ldpc_gpu_free(self.ctx, p), "free")
+
+    def upload(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        self._check(self.L.ldpc_gpu_upload(self.ctx, dptr, arr.ctypes.data, arr.nbytes), "upload")
+
+    def download(self, arr, dptr):
+        self._check(self.L.ldpc_gpu_download(self.ctx, arr.ctypes.data, dptr, arr.nbytes), "download")
+
+    def synchronize(self):
+        self._check(self.L.ldpc_gpu_synchronize(self.ctx), "synchronize")
+
+    def decode_batch(self, frames, d_llr, d_out, d_iters, d_synd):
+        frames = np.ascontiguousarray(frames, FRAME_DTYPE)
+        self._check(
+            self.L.ldpc_gpu_decode_batch(self.ctx, frames.ctypes.data, len(frames), d_llr, d_out, d_iters, d_synd),
+            "decode_batch",
+        )
+
+    def last_kernel_ms(self):
+        return self.L.ldpc_gpu_last_kernel_ms(self.ctx)
+
+    def decode_host(self, frames, llr, out_bytes):
+        """Synchronous decode of host buffers: returns (out, iters, synd)."""
+        frames = np.ascontiguousarray(frames, FRAME_DTYPE)
+        llr = np.ascontiguousarray(llr, np.int8)
+        n = len(frames)
+        out = np.zeros(out_bytes, np.uint8)
+        iters = np.zeros(n, np.int32)
+        synd = np.zeros(n, np.uint8)
+        self._check(
+            self.L.ldpc_gpu_decode_host(
+                self.ctx, frames.ctypes.data, n, llr.ctypes.data, llr.nbytes, out.ctypes.data, out.nbytes,
+                iters.ctypes.data, synd.ctypes.data,
+            ),
+            "decode_host",
+        )
+        return out, iters, synd
+
+
+class DrxLdpc:
+    """The drop-in drx_ldpc_* API of libldpcgpu.so (include/dvbrx/ldpc.h)."""
+
+    def __init__(self, fec, rate):
+        self.L = gpu_lib()
+        self.fec = fec
+        self._p = fec.as_struct()
+        self.h = self.L.drx_ldpc_new(C.byref(self._p), rate.encode())
+        if not self.h:
+            raise RuntimeError(f"drx_ldpc_new({rate}) failed")
+
+    def close(self):
+        if self.h:
+            self.L.drx_ldpc_free(self.h)
+            self.h = None
+
+    def decode(self, llr, max_iter):
+        """llr: (nframes, nldpc) int8.  Returns (iterations or -1, bits)."""
+        llr = np.ascontiguousarray(llr, np.int8)
+        n = llr.shape[0]
+        nb = (self.fec.kldpc + 7) // 8
+        out = np.zeros((n, nb), np.uint8)
+        ins = (C.c_void_p * max(n, 1))(*[llr[f].ctypes.data for f in range(n)])
+        outs = (C.c_void_p * max(n, 1))(*[out[f].ctypes.data for f in range(n)])
+        it = self.L.drx_ldpc_decode(self.h, ins, outs, n, max_iter)
+        return it, out

@@ -1,0 +1,99 @@
+/*
+ * MI355X (gfx950) LDPC decoder for DVB-S2 / S2X / T2 codes: C-ABI.
+ *
+ * Layered offset min-sum on int8 LLRs, bit-exact with libdvbrx's CPU
+ * decoder (libdvbrx/src/ldpc.c), run by hand-written HIP kernels.  The
+ * reference decodes one batch of <= 64 frames of one code per call
+ * (drx_ldpc_decode, libdvbrx/include/dvbrx/ldpc.h:32-36, one frame per SIMD
+ * byte lane); this interface takes any number of frames of mixed codes in
+ * one call, described by a frame table, with per-frame results.  The
+ * reference's four drx_ldpc_* entry points are provided on top of it (see
+ * include/dvbrx/ldpc.h).
+ *
+ * All pointers named d_* are device memory.  Plain C, no HIP types: the
+ * stream is handed out as an opaque pointer (a hipStream_t).
+ */
+#ifndef LDPC_GPU_H
+#define LDPC_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes (negative). */
+#define LDPC_GPU_OK 0
+#define LDPC_GPU_EINVAL (-1)   /* bad argument (unknown code, misaligned offset, ...) */
+#define LDPC_GPU_ENOMEM (-2)   /* device or host allocation failed */
+#define LDPC_GPU_EDEVICE (-3)  /* HIP runtime error; see ldpc_gpu_last_error() */
+
+typedef struct ldpc_gpu_ctx ldpc_gpu_ctx;
+
+/* One frame of a batch.  Frames are independent unless they share a group. */
+typedef struct {
+    int32_t code;     /* ldpc_gpu_code_lookup() */
+    int32_t max_iter; /* iteration cap, >= 0 (the reference's max_iter) */
+    int32_t nbits;    /* decoded bits written (kldpc), 1..K; 0 means K */
+    int32_t group;    /* -1: the frame stops when its own syndrome holds;
+                         >= 0: all frames with this group number iterate until
+                         every one of them holds its syndrome at the same
+                         iteration, as the frames of one drx_ldpc_decode call
+                         (libdvbrx/src/ldpc.c:627-637).  Frames of a group
+                         must share max_iter. */
+    uint64_t llr_off; /* byte offset of the frame's N LLRs in d_llr (multiple of 8) */
+    uint64_t out_off; /* byte offset of its ceil(nbits/8) output bytes in d_out */
+} ldpc_gpu_frame;
+
+/* Context: device, stream, device-resident code tables and scratch.  One
+ * thread at a time per context; distinct contexts are independent. */
+int ldpc_gpu_create(int device, ldpc_gpu_ctx** out);
+void ldpc_gpu_destroy(ldpc_gpu_ctx* ctx);
+void* ldpc_gpu_stream(ldpc_gpu_ctx* ctx); /* hipStream_t */
+const char* ldpc_gpu_last_error(ldpc_gpu_ctx* ctx);
+
+/* Codes.  ldpc_gpu_code_lookup replaces find_table (libdvbrx/src/ldpc.c:188-197):
+ * (frame size, rate name as drx_fec_lookup: "1_2", "2_3_t2", ...) -> code
+ * id, or -1.  ldpc_gpu_fec_lookup is drx_fec_lookup (libdvbrx/src/bch.c:25-41). */
+int ldpc_gpu_code_count(void);
+int ldpc_gpu_code_lookup(int short_frame, const char* rate);
+int ldpc_gpu_code_params(int code, int* n, int* k, int* q, int* links_total, int* level_steps);
+int ldpc_gpu_fec_lookup(int short_frame, const char* rate, int* nldpc, int* kldpc, int* kbch, int* t);
+
+/* Device memory helpers (synchronous copies on the context's stream). */
+void* ldpc_gpu_malloc(ldpc_gpu_ctx* ctx, size_t bytes);
+int ldpc_gpu_free(ldpc_gpu_ctx* ctx, void* d_ptr);
+int ldpc_gpu_upload(ldpc_gpu_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+int ldpc_gpu_download(ldpc_gpu_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+int ldpc_gpu_synchronize(ldpc_gpu_ctx* ctx);
+
+/*
+ * Decode a batch (replaces drx_ldpc_decode, libdvbrx/src/ldpc.c:614-638).
+ *
+ * d_llr:   int8 channel LLRs, codeword order, positive for a 0 bit.
+ * d_out:   hard decisions of the first nbits bits, packed MSB first.
+ * d_iters: per frame, iterations run (for a group: the group's count).
+ * d_synd:  per frame, 1 if every parity check holds at the end.
+ *
+ * Without groups the call only enqueues work on the context's stream
+ * (results valid after ldpc_gpu_synchronize); with groups it returns
+ * after the batch is decoded.  Returns LDPC_GPU_OK or an error code.
+ */
+int ldpc_gpu_decode_batch(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nframes,
+                          const int8_t* d_llr, uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd);
+
+/* Same on host buffers: copies in, decodes, copies out (synchronous). */
+int ldpc_gpu_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nframes,
+                         const int8_t* llr, size_t llr_bytes, uint8_t* out, size_t out_bytes,
+                         int32_t* iters, uint8_t* synd);
+
+/* Timing of the last ldpc_gpu_decode_batch's decode kernel on the stream it
+ * ran on (HIP events around the launch), in milliseconds; < 0 if none. */
+float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

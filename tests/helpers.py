@@ -1,0 +1,179 @@
+"""Test-side wrappers: the CPU oracle (oracle/liboracle.so), the reference
+decoder compiled from its own sources (oracle/_ref/, present where it was
+built), the CPU model of the kernel schedule (tests/model/libkmodel.so), and
+the product's bindings (dvb-rx_amd/ldpcgpu.py)."""
+import ctypes as C
+import glob
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "dvb-rx_amd"))
+
+import ldpcgpu  # noqa: E402
+
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+KMODEL_SO = os.path.join(ROOT, "tests", "model", "libkmodel.so")
+
+ALL_RATES = {
+    0: ["1_4", "1_3", "2_5", "1_2", "3_5", "2_3", "3_4", "4_5", "5_6", "8_9", "9_10", "2_9_VLSNR", "13_45",
+        "9_20", "90_180", "96_180", "11_20", "100_180", "104_180", "26_45", "18_30", "28_45", "23_36",
+        "116_180", "20_30", "124_180", "25_36", "128_180", "13_18", "132_180", "22_30", "135_180",
+        "140_180", "7_9", "154_180", "2_3_t2"],
+    1: ["1_4", "1_3", "2_5", "1_2", "3_5", "2_3", "3_4", "4_5", "5_6", "8_9", "11_45", "4_15", "14_45",
+        "7_15", "8_15", "26_45", "32_45", "1_5_VLSNR_SF2", "11_45_VLSNR_SF2", "1_5_VLSNR", "4_15_VLSNR",
+        "1_3_VLSNR", "3_5_t2"],
+}
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        L = C.CDLL(ORACLE_SO)
+        vp = C.c_void_p
+        L.oracle_ldpc_new.restype = vp
+        L.oracle_ldpc_new.argtypes = [C.c_int, C.c_char_p, C.c_int]
+        L.oracle_ldpc_free.argtypes = [vp]
+        L.oracle_ldpc_decode.argtypes = [vp, vp, vp, C.c_int, C.c_int]
+        L.oracle_ldpc_decode_frames.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp]
+        L.oracle_ldpc_check.argtypes = [vp, vp]
+        L.oracle_check_node.argtypes = [C.c_int, vp, vp, C.c_int, vp, vp]
+        _oracle = L
+    return _oracle
+
+
+class Oracle:
+    def __init__(self, short_frame, rate, kldpc):
+        self.L = oracle()
+        self.h = self.L.oracle_ldpc_new(short_frame, rate.encode(), kldpc)
+        assert self.h, rate
+        self.kb = (kldpc + 7) // 8
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.oracle_ldpc_free(self.h)
+
+    def frames(self, llr, max_iter):
+        """Per-frame decode: (bits (n, kb), iters, ok)."""
+        llr = np.ascontiguousarray(llr, np.int8)
+        n = llr.shape[0]
+        out = np.zeros((n, self.kb), np.uint8)
+        it = np.zeros(n, np.int32)
+        ok = np.zeros(n, np.uint8)
+        self.L.oracle_ldpc_decode_frames(self.h, llr.ctypes.data, out.ctypes.data, n, max_iter, it.ctypes.data, ok.ctypes.data)
+        return out, it, ok
+
+    def group(self, llr, max_iter):
+        llr = np.ascontiguousarray(llr, np.int8)
+        n = llr.shape[0]
+        out = np.zeros((n, self.kb), np.uint8)
+        r = self.L.oracle_ldpc_decode(self.h, llr.ctypes.data, out.ctypes.data, n, max_iter)
+        return r, out
+
+    def check(self, llr):
+        llr = np.ascontiguousarray(llr, np.int8)
+        return self.L.oracle_ldpc_check(self.h, llr.ctypes.data)
+
+
+def check_node(deg, L, m, first):
+    Lb = np.ascontiguousarray(L, np.int8)
+    mb = np.ascontiguousarray(m, np.int8)
+    Lo = np.zeros(deg, np.int8)
+    mo = np.zeros(deg, np.int8)
+    oracle().oracle_check_node(deg, Lb.ctypes.data, mb.ctypes.data, int(first), Lo.ctypes.data, mo.ctypes.data)
+    return Lo, mo
+
+
+def ref_builds():
+    """Reference decoders built from the reference's own sources by
+    oracle/Makefile (name -> CDLL); empty where they were not built.  A build
+    whose SIMD level the host lacks is skipped."""
+    flags = ""
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        pass
+    out = {}
+    for path in sorted(glob.glob(os.path.join(REF_DIR, "libdrxref_*.so"))):
+        name = os.path.basename(path)[len("libdrxref_"):-3]
+        if name == "v4" and "avx512bw" not in flags:
+            continue
+        if name == "v3" and "avx2" not in flags:
+            continue
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.drx_ldpc_new.restype = vp
+        L.drx_ldpc_new.argtypes = [vp, C.c_char_p]
+        L.drx_ldpc_free.argtypes = [vp]
+        L.drx_ldpc_decode.argtypes = [vp, vp, vp, C.c_int, C.c_int]
+        L.drx_ldpc_lanes.restype = C.c_int
+        L.drx_bch_new.restype = vp
+        L.drx_bch_new.argtypes = [vp]
+        L.drx_bch_free.argtypes = [vp]
+        L.drx_bch_decode.argtypes = [vp, vp]
+        out[name] = L
+    return out
+
+
+def ref_decode(L, fec, rate, llr, max_iter):
+    """drx_ldpc_decode of the reference on up to its lane count of frames."""
+    p = fec.as_struct()
+    d = L.drx_ldpc_new(C.byref(p), rate.encode())
+    assert d, rate
+    try:
+        llr = np.ascontiguousarray(llr, np.int8)
+        n = llr.shape[0]
+        kb = (fec.kldpc + 7) // 8
+        out = np.zeros((n, kb), np.uint8)
+        ins = (C.c_void_p * n)(*[llr[f].ctypes.data for f in range(n)])
+        outs = (C.c_void_p * n)(*[out[f].ctypes.data for f in range(n)])
+        it = L.drx_ldpc_decode(d, ins, outs, n, max_iter)
+        return it, out
+    finally:
+        L.drx_ldpc_free(d)
+
+
+_km = None
+
+
+def kmodel():
+    global _km
+    if _km is None:
+        L = C.CDLL(KMODEL_SO)
+        L.kmodel_decode.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.kmodel_check_levels.argtypes = [C.c_int]
+        _km = L
+    return _km
+
+
+def kmodel_decode(code, llr, max_iter, nbits):
+    llr = np.ascontiguousarray(llr, np.int8)
+    out = np.zeros((nbits + 7) // 8, np.uint8)
+    ok = C.c_int()
+    it = kmodel().kmodel_decode(code, llr.ctypes.data, max_iter, nbits, out.ctypes.data, C.byref(ok))
+    return out, it, ok.value
+
+
+def gen(short_frame, rate, esn0_db, nframes, seed, pls=-1, stream0=0):
+    """nframes frames of one rate: (fec, llr (n, N), ref bits (n, kb))."""
+    spec = [dict(short_frame=short_frame, rate=rate, pls=pls, esn0_db=esn0_db, stream=stream0 + f) for f in range(nframes)]
+    llr, ref, layout = ldpcgpu.generate(spec, seed)
+    if pls < 0:
+        fec = ldpcgpu.fec_lookup(short_frame, rate)
+    else:
+        m = ldpcgpu.pls_lookup(pls)
+        fec = ldpcgpu.fec_lookup(m["short_frame"], m["rate"])
+    n = layout[0][1]
+    kb = (fec.kldpc + 7) // 8
+    return fec, llr.reshape(nframes, n), ref.reshape(nframes, kb)
+
+
+def sha256(a):
+    import hashlib
+
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
