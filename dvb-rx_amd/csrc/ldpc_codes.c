@@ -111,6 +111,45 @@ static int build_code(lg_code* c, int id)
         if (maxl + 1 > c->max_levels)
             c->max_levels = maxl + 1;
     }
+    /* dependency masks: edge e of node j is dependent if an earlier node of
+     * the layer touches the same bit, successor-shared if a later one does */
+    c->meta = calloc((size_t)c->r, sizeof(uint64_t));
+    c->msg_off = malloc(sizeof(int) * (size_t)q);
+    if (!c->meta || !c->msg_off) {
+        free(owner);
+        return -1;
+    }
+    int rows = 0;
+    for (int i = 0; i < q; i++) {
+        const int s = c->layer_start[i], d = c->layer_start[i + 1] - s;
+        c->msg_off[i] = rows;
+        rows += (d + 2 + 3) / 4;
+        uint64_t* mt = c->meta + (size_t)i * LG_M;
+        for (int j = 0; j < LG_M; j++)
+            mt[j] = c->level[(size_t)i * LG_M + j];
+        if (c->nlev[i] == 1)
+            continue;
+        for (int e = 0; e < d; e++) {
+            const int base = (int)(c->ent[s + e] >> 16);
+            for (int m = 0; m < LG_M; m++)
+                owner[base + m] = -1;
+        }
+        /* owner[b] = last node so far touching b, with its edge index in
+         * the high bits */
+        for (int j = 0; j < LG_M; j++) {
+            for (int e = 0; e < d; e++) {
+                const int base = (int)(c->ent[s + e] >> 16), sh = (int)(c->ent[s + e] & 0xffff);
+                const int b = base + (j - sh + LG_M) % LG_M;
+                if (owner[b] >= 0) {
+                    const int pj = owner[b] & 0x1ff, pe = owner[b] >> 9;
+                    mt[j] |= (uint64_t)1 << (8 + e);
+                    mt[pj] |= (uint64_t)1 << (36 + pe);
+                }
+                owner[b] = (int16_t)(j | e << 9);
+            }
+        }
+    }
+    c->msg_words = rows * LG_M;
     free(owner);
     /* every edge counted: info edges plus the 2R-1 of the staircase */
     if ((long)naddr * LG_M + 2L * c->r - 1 != c->links_total)

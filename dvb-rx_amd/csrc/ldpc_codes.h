@@ -49,7 +49,18 @@ typedef struct lg_code {
     uint32_t* ent;         /* base << 16 | shift */
     uint8_t* level;        /* r: level of node j of layer i at i*360+j */
     uint8_t* nlev;         /* q: level count of each layer */
+    /* r: per node, bits 0-7 its level, bits 8-35 the information edges it
+     * shares with an earlier node of its layer (read after that node's
+     * update: "dependent"), bits 36-63 those it shares with a later node
+     * (whose final value that later node writes) */
+    uint64_t* meta;
+    int msg_words;         /* 32-bit message words per frame: 360 * sum of ceil((deg_i+2)/4) */
+    int* msg_off;          /* q: first message word row of each layer (rows of 360 words) */
 } lg_code;
+
+#define LG_META_LEVEL(m) ((int)((m) & 0xff))
+#define LG_META_DEP(m) ((uint32_t)(((m) >> 8) & 0xfffffffu))
+#define LG_META_SUCC(m) ((uint32_t)(((m) >> 36) & 0xfffffffu))
 
 int lg_code_count(void);
 /* Expanded code, or NULL for an out-of-range id.  Built once, immutable. */

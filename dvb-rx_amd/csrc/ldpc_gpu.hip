@@ -39,14 +39,20 @@
 #include "ldpc_node.h"
 
 #define NT 384 /* threads per workgroup: 360 check nodes of a layer */
+#define NCLASS 2 /* degree classes: one kernel instance each (registers grow with the degree) */
+static const int CLASS_DI[NCLASS] = { 12, 28 }; /* largest layer degree of each class */
 
 namespace {
 
 struct DCode {
     int n, k, r, q;
-    int ent_off; /* into ent: base << 16 | shift */
-    int lay_off; /* into lay: start | deg << 16 | nlev << 24 */
-    int lev_off; /* into lev: one byte per node */
+    int ent_off;   /* into ent: base << 16 | shift */
+    int lay_off;   /* into lay (see lay_deg / lay_nlev / lay_start) */
+    int meta_off;  /* into meta: level and dependency masks per node */
+    int nent;      /* (base, shift) pairs */
+    int dmax;      /* largest information degree of a layer */
+    int dmin;      /* smallest */
+    int msg_words; /* 32-bit message words per frame */
     int pad;
 };
 
@@ -58,8 +64,8 @@ struct DFrame {
 struct KArgs {
     const DCode* codes;
     const uint32_t* ent;
-    const uint32_t* lay;
-    const uint8_t* lev;
+    const uint2* lay;
+    const uint64_t* meta;
     const DFrame* frames;
     const int* list;   /* frames to run (nullptr: 0..nrun-1) */
     int nrun;
@@ -67,8 +73,8 @@ struct KArgs {
     uint8_t* out;
     int32_t* iters;
     uint8_t* synd;
-    uint2* msg;        /* messages: per workgroup or per frame */
-    size_t msg_stride; /* uint2 per message block */
+    uint32_t* msg;     /* messages: per workgroup or per frame */
+    size_t msg_stride; /* words per message block */
     int msg_per_frame;
     int8_t* state;     /* group mode: posteriors saved per frame (N_max bytes) */
     size_t state_stride;
@@ -80,40 +86,189 @@ struct KArgs {
     unsigned* ticket;
 };
 
-__device__ __forceinline__ void node_update(int8_t* sm, const uint32_t* ent, int deg, int i, int j, int K,
-                                            int q, uint2* mp, int first)
+/* largest table of one code staged in LDS: (base, shift) pairs and layers */
+#define MAX_ENT 1024
+#define MAX_LAYERS 192
+#define SYN_CHUNK 4  /* layers checked between two early-exit votes */
+#define WMAX 8       /* message words of a node: (28 + 2 + 3) / 4 */
+#define LOW_MAX_DEG 12 /* class 0: layers of information degree 1..12 */
+#define HIGH_MIN_DEG 9 /* class 1: 9..28 */
+
+/* layer table entry: x = first (base, shift) pair | degree << 16 | levels << 24,
+ * y = first message row (rows of 360 words) */
+__device__ __forceinline__ int lay_deg(uint2 ly) { return (ly.x >> 16) & 0xff; }
+__device__ __forceinline__ int lay_nlev(uint2 ly) { return ly.x >> 24; }
+__device__ __forceinline__ int lay_start(uint2 ly) { return ly.x & 0xffff; }
+
+#define LOW_DEGS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+#define HIGH_DEGS(X)                                                                                        \
+    X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) \
+        X(27) X(28)
+
+/* Syndrome (bad(), ldpc.c:286-305) over chunks of layers, stopping at the
+ * first chunk where some check fails: an unconverged frame usually fails
+ * in the first chunk. */
+template <int CLS>
+__device__ __forceinline__ int syndrome_ok(const int8_t* sm, const uint32_t* s_ent, const uint2* s_lay, int j, int K,
+                                           int q)
 {
-    uint32_t w0 = 0, sg = 0;
-    if (!first) {
-        const uint2 w = *mp;
-        w0 = w.x;
-        sg = w.y;
+    for (int i0 = 0; i0 < q; i0 += SYN_CHUNK) {
+        int ok = 1;
+        if (j < 360) {
+            const int i1 = min(i0 + SYN_CHUNK, q);
+            for (int i = i0; i < i1; i++) {
+                const uint2 ly = s_lay[i];
+                const uint32_t* e = s_ent + lay_start(ly);
+#define X(D)                                                   \
+    case D:                                                    \
+        ok &= lg_node_check_t<D>(sm, e, D, i, j, K, q);        \
+        break;
+                if constexpr (CLS == 0) {
+                    switch (lay_deg(ly)) {
+                        LOW_DEGS(X)
+                    default:
+                        ok = 0;
+                    }
+                } else {
+                    switch (lay_deg(ly)) {
+                        HIGH_DEGS(X)
+                    default:
+                        ok = 0;
+                    }
+                }
+#undef X
+            }
+        }
+        if (!__syncthreads_and(ok))
+            return 0;
     }
-    lg_node_update(sm, ent, deg, i, j, K, q, &w0, &sg, first);
-    *mp = make_uint2(w0, sg);
+    return 1;
 }
 
-/* all R checks hold (bad(), ldpc.c:286-305) */
-__device__ __forceinline__ int syndrome_ok(const int8_t* sm, const DCode& cd, const uint32_t* ent,
-                                           const uint32_t* lay, int j)
+/* One layer by the workgroup: phases A, S (per level), C of ldpc_node.h.
+ * mw: the node's message words (zero in the first iteration). */
+template <int DEG>
+__device__ __forceinline__ void layer_t(int8_t* sm, const uint32_t* ent, int i, int j, int K, int q, int nlev,
+                                        const uint32_t* mw, uint64_t meta, uint32_t* mrow, int first)
 {
-    int ok = 1;
-    if (j < 360) {
-        for (int i = 0; i < cd.q; i++) {
-            const uint32_t ly = lay[i];
-            const int st = ly & 0xffff, deg = (ly >> 16) & 0xff;
-            ok &= lg_node_check(sm, ent + st, deg, i, j, cd.k, cd.q);
+    const bool act = j < 360;
+    const uint32_t dep = LG_META_DEP(meta), succ = LG_META_SUCC(meta);
+    const int lvl = LG_META_LEVEL(meta);
+    LgNode<DEG> n;
+    if (act) {
+        lg_phase_a<DEG>(n, sm, ent, i, j, K, q, dep, mw, first);
+        if (nlev > 1 && lvl == 0)
+            lg_write_succ<DEG>(n, sm, succ);
+    }
+    for (int l = 1; l < nlev; l++) {
+        __syncthreads();
+        if (act && lvl == l) {
+            lg_step<DEG>(n, sm, dep, mw, first);
+            lg_write_succ<DEG>(n, sm, succ);
         }
     }
-    return __syncthreads_and(ok);
+    if (act) {
+        uint32_t o[LG_MSG_WORDS(DEG)];
+        lg_phase_c<DEG>(n, sm, succ, o);
+#pragma unroll
+        for (int w = 0; w < LG_MSG_WORDS(DEG); w++)
+            mrow[360 * w + j] = o[w];
+    }
 }
 
-__global__ void __launch_bounds__(NT) reset_ticket(unsigned* t) { *t = 0; }
+/*
+ * One decode (from the LLRs or resumed) of one frame by the workgroup.
+ * Returns the syndrome flag; *itp holds the iterations run.  The messages
+ * and dependency masks of layer i+1 are fetched while layer i runs.
+ */
+template <int CLS>
+__device__ __forceinline__ int decode_frame(int8_t* sm, const uint32_t* s_ent, const uint2* s_lay,
+                                            const uint64_t* meta, uint32_t* msg, int K, int q, int* itp, int target,
+                                            int early_stop)
+{
+    const int j = threadIdx.x;
+    const bool act = j < 360;
+    int it = *itp, ok;
+    for (;;) {
+        if (early_stop || it >= target) {
+            ok = syndrome_ok<CLS>(sm, s_ent, s_lay, j, K, q);
+            if (ok || it >= target)
+                break;
+        }
+        const int first = it == 0;
+        uint32_t nx[WMAX];
+        uint64_t nmeta = 0;
+#pragma unroll
+        for (int w = 0; w < WMAX; w++)
+            nx[w] = 0;
+        {
+            const uint2 ly = s_lay[0];
+            const int W = LG_MSG_WORDS(lay_deg(ly));
+            if (act && !first) {
+#pragma unroll
+                for (int w = 0; w < WMAX; w++)
+                    if (w < W)
+                        nx[w] = msg[360 * (ly.y + w) + j];
+            }
+            if (act && lay_nlev(ly) > 1)
+                nmeta = meta[j];
+        }
+        for (int i = 0; i < q; i++) {
+            uint32_t mw[WMAX];
+#pragma unroll
+            for (int w = 0; w < WMAX; w++)
+                mw[w] = nx[w];
+            const uint64_t mt = nmeta;
+            const uint2 ly = s_lay[i];
+            if (i + 1 < q) {
+                const uint2 ln = s_lay[i + 1];
+                const int W = LG_MSG_WORDS(lay_deg(ln));
+                if (act && !first) {
+#pragma unroll
+                    for (int w = 0; w < WMAX; w++)
+                        if (w < W)
+                            nx[w] = msg[360 * (ln.y + w) + j];
+                }
+                nmeta = act && lay_nlev(ln) > 1 ? meta[360 * (i + 1) + j] : 0;
+            }
+            const uint32_t* e = s_ent + lay_start(ly);
+            uint32_t* mrow = msg + 360 * ly.y;
+            const int nlev = lay_nlev(ly);
+#define X(D)                                                            \
+    case D:                                                             \
+        layer_t<D>(sm, e, i, j, K, q, nlev, mw, mt, mrow, first);       \
+        break;
+            if constexpr (CLS == 0) {
+                switch (lay_deg(ly)) {
+                    LOW_DEGS(X)
+                }
+            } else {
+                switch (lay_deg(ly)) {
+                    HIGH_DEGS(X)
+                }
+            }
+#undef X
+            __syncthreads();
+        }
+        it++;
+    }
+    *itp = it;
+    return ok;
+}
 
+__global__ void __launch_bounds__(64) reset_ticket(unsigned* t)
+{
+    if (threadIdx.x < 4)
+        t[threadIdx.x] = 0;
+}
+
+template <int CLS>
 __global__ void __launch_bounds__(NT) ldpc_layered_kernel(KArgs a)
 {
     extern __shared__ __align__(16) int8_t sm[];
     __shared__ int s_run;
+    __shared__ uint32_t s_ent[MAX_ENT];
+    __shared__ uint2 s_lay[MAX_LAYERS];
     const int tid = threadIdx.x;
 
     for (;;) {
@@ -128,10 +283,10 @@ __global__ void __launch_bounds__(NT) ldpc_layered_kernel(KArgs a)
         const DFrame fr = a.frames[f];
         const DCode cd = a.codes[fr.code];
         const uint32_t* ent = a.ent + cd.ent_off;
-        const uint32_t* lay = a.lay + cd.lay_off;
-        const uint8_t* lev = a.lev + cd.lev_off;
+        const uint2* lay = a.lay + cd.lay_off;
+        const uint64_t* meta = a.meta + cd.meta_off;
         const int K = cd.k, N = cd.n, R = cd.r, q = cd.q;
-        uint2* msg = a.msg + a.msg_stride * (size_t)(a.msg_per_frame ? f : blockIdx.x);
+        uint32_t* msg = a.msg + a.msg_stride * (size_t)(a.msg_per_frame ? f : blockIdx.x);
 
         int it;
         int target;
@@ -154,43 +309,27 @@ __global__ void __launch_bounds__(NT) ldpc_layered_kernel(KArgs a)
             it = a.it_state[f];
             target = a.target[f];
         }
+        /* the code's layer table and (base, shift) pairs, read by every
+         * node of the frame */
+        for (int t = tid; t < cd.nent; t += NT)
+            s_ent[t] = ent[t];
+        for (int t = tid; t < q; t += NT)
+            s_lay[t] = lay[t];
         __syncthreads();
 
-        const int j = tid;
-        int ok;
-        for (;;) {
-            if (!a.resume) {
-                ok = syndrome_ok(sm, cd, ent, lay, j);
-                if (ok || it >= target)
-                    break;
-            } else if (it >= target) {
-                ok = syndrome_ok(sm, cd, ent, lay, j);
-                break;
-            }
-            const int first = it == 0;
-            for (int i = 0; i < q; i++) {
-                const uint32_t ly = lay[i];
-                const int st = ly & 0xffff, deg = (ly >> 16) & 0xff, nlev = ly >> 24;
-                const int my = j < 360 ? lev[360 * i + j] : 255;
-                for (int l = 0; l < nlev; l++) {
-                    if (my == l)
-                        node_update(sm, ent + st, deg, i, j, K, q, msg + 360 * i + j, first);
-                    __syncthreads();
-                }
-            }
-            it++;
-        }
+        const int ok = decode_frame<CLS>(sm, s_ent, s_lay, meta, msg, K, q, &it, target, !a.resume);
 
-        /* hard decisions of the first nbits bits, MSB first */
+        /* hard decisions of the first nbits bits, MSB first, 4 bytes a thread */
         const int nbytes = (fr.nbits + 7) / 8;
         uint8_t* o = a.out + fr.out_off;
-        for (int b = tid; b < nbytes; b += NT) {
-            int v = 0;
-            for (int k = 0; k < 8; k++) {
-                const int bit = 8 * b + k;
-                v = v << 1 | (bit < fr.nbits && sm[bit] < 0);
+        for (int b4 = 4 * tid; b4 < nbytes; b4 += 4 * NT) {
+            uint32_t v = 0;
+            for (int k = 0; k < 32; k++) {
+                const int bit = 8 * b4 + k;
+                v = v << 1 | (uint32_t)(bit < fr.nbits && sm[bit] < 0);
             }
-            o[b] = (uint8_t)v;
+            for (int k = 0; k < 4 && b4 + k < nbytes; k++)
+                o[b4 + k] = (uint8_t)(v >> (24 - 8 * k));
         }
         if (a.save) {
             uint2* dst = (uint2*)(a.state + a.state_stride * (size_t)f);
@@ -213,8 +352,8 @@ struct DeviceTables {
     int ready = 0;
     DCode* codes = nullptr;
     uint32_t* ent = nullptr;
-    uint32_t* lay = nullptr;
-    uint8_t* lev = nullptr;
+    uint2* lay = nullptr;
+    uint64_t* meta = nullptr;
     std::vector<DCode> h_codes;
 };
 
@@ -235,7 +374,7 @@ struct ldpc_gpu_ctx {
     size_t frames_bytes;
     int* d_list;
     size_t list_bytes;
-    uint2* msg;
+    uint32_t* msg;
     size_t msg_bytes;
     int8_t* state;
     size_t state_bytes;
@@ -244,6 +383,8 @@ struct ldpc_gpu_ctx {
     int32_t* target;
     size_t target_bytes;
     hipEvent_t ev0, ev1;
+    hipEvent_t ev_cls[2 * NCLASS];
+    int cls_frames[NCLASS];
     int timed;
 };
 
@@ -276,8 +417,9 @@ static int upload_tables(ldpc_gpu_ctx* c)
         return LDPC_GPU_OK;
     const int nc = lg_code_count();
     std::vector<DCode> codes(nc);
-    std::vector<uint32_t> ent, lay;
-    std::vector<uint8_t> lev;
+    std::vector<uint32_t> ent;
+    std::vector<uint2> lay;
+    std::vector<uint64_t> meta;
     for (int id = 0; id < nc; id++) {
         const lg_code* cd = lg_code_get(id);
         if (!cd)
@@ -289,23 +431,34 @@ static int upload_tables(ldpc_gpu_ctx* c)
         d.q = cd->q;
         d.ent_off = (int)ent.size();
         d.lay_off = (int)lay.size();
-        d.lev_off = (int)lev.size();
+        d.meta_off = (int)meta.size();
+        d.nent = cd->nent;
+        d.dmax = cd->max_layer_deg;
+        d.dmin = 255;
+        d.msg_words = cd->msg_words;
         d.pad = 0;
         ent.insert(ent.end(), cd->ent, cd->ent + cd->nent);
         for (int i = 0; i < cd->q; i++) {
             const int st = cd->layer_start[i], deg = cd->layer_start[i + 1] - st;
-            lay.push_back((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24);
+            d.dmin = std::min(d.dmin, deg);
+            lay.push_back(make_uint2((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24,
+                                     (uint32_t)cd->msg_off[i]));
         }
-        lev.insert(lev.end(), cd->level, cd->level + cd->r);
+        meta.insert(meta.end(), cd->meta, cd->meta + cd->r);
+        /* every layer degree must be one the code's class kernel handles */
+        const int cls = d.dmax <= LOW_MAX_DEG ? 0 : 1;
+        if (cd->nent > MAX_ENT || cd->q > MAX_LAYERS || d.dmax > 28 || (cls == 1 && d.dmin < HIGH_MIN_DEG) ||
+            d.dmin < 1)
+            return set_err(c, LDPC_GPU_EINVAL, "code %d exceeds the kernel's table bounds", id);
     }
     HIPCHK(c, hipMalloc((void**)&t.codes, sizeof(DCode) * codes.size()));
     HIPCHK(c, hipMalloc((void**)&t.ent, sizeof(uint32_t) * ent.size()));
-    HIPCHK(c, hipMalloc((void**)&t.lay, sizeof(uint32_t) * lay.size()));
-    HIPCHK(c, hipMalloc((void**)&t.lev, lev.size()));
+    HIPCHK(c, hipMalloc((void**)&t.lay, sizeof(uint2) * lay.size()));
+    HIPCHK(c, hipMalloc((void**)&t.meta, sizeof(uint64_t) * meta.size()));
     HIPCHK(c, hipMemcpy(t.codes, codes.data(), sizeof(DCode) * codes.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.ent, ent.data(), sizeof(uint32_t) * ent.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(t.lay, lay.data(), sizeof(uint32_t) * lay.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(t.lev, lev.data(), lev.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.lay, lay.data(), sizeof(uint2) * lay.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.meta, meta.data(), sizeof(uint64_t) * meta.size(), hipMemcpyHostToDevice));
     t.h_codes = codes;
     t.ready = 1;
     return LDPC_GPU_OK;
@@ -330,7 +483,13 @@ int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
     }
     int rc;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        [&] {
+            for (int k = 0; k < 2 * NCLASS; k++)
+                if (hipEventCreate(&c->ev_cls[k]) != hipSuccess)
+                    return true;
+            return false;
+        }()) {
         free(c);
         return LDPC_GPU_EDEVICE;
     }
@@ -362,6 +521,9 @@ void ldpc_gpu_destroy(ldpc_gpu_ctx* c)
         (void)hipEventDestroy(c->ev0);
     if (c->ev1)
         (void)hipEventDestroy(c->ev1);
+    for (int k = 0; k < 2 * NCLASS; k++)
+        if (c->ev_cls[k])
+            (void)hipEventDestroy(c->ev_cls[k]);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     free(c);
@@ -389,6 +551,12 @@ int ldpc_gpu_code_params(int code, int* n, int* k, int* q, int* links_total, int
     if (level_steps)
         *level_steps = c->level_steps;
     return LDPC_GPU_OK;
+}
+
+int ldpc_gpu_code_max_degree(int code)
+{
+    const lg_code* c = lg_code_get(code);
+    return c ? c->max_layer_deg : LDPC_GPU_EINVAL;
 }
 
 int ldpc_gpu_fec_lookup(int short_frame, const char* rate, int* nldpc, int* kldpc, int* kbch, int* t)
@@ -453,6 +621,29 @@ int ldpc_gpu_synchronize(ldpc_gpu_ctx* c)
     return LDPC_GPU_OK;
 }
 
+int ldpc_gpu_last_launches(ldpc_gpu_ctx* c, int* di, int* nframes, float* ms, int max)
+{
+    if (!c || !c->timed)
+        return 0;
+    int n = 0;
+    for (int k = 0; k < NCLASS && n < max; k++) {
+        if (!c->cls_frames[k])
+            continue;
+        float t = -1.0f;
+        if (hipEventSynchronize(c->ev_cls[2 * k + 1]) != hipSuccess ||
+            hipEventElapsedTime(&t, c->ev_cls[2 * k], c->ev_cls[2 * k + 1]) != hipSuccess)
+            t = -1.0f;
+        if (di)
+            di[n] = CLASS_DI[k];
+        if (nframes)
+            nframes[n] = c->cls_frames[k];
+        if (ms)
+            ms[n] = t;
+        n++;
+    }
+    return n;
+}
+
 float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* c)
 {
     if (!c || !c->timed)
@@ -479,33 +670,86 @@ static int grow(ldpc_gpu_ctx* c, T** p, size_t* cap_bytes, size_t need)
     return LDPC_GPU_OK;
 }
 
-/* one launch of the decode kernel over nrun frames (list or identity) */
-static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const int* d_list, int nrun, int maxn, int maxr,
-                  const int8_t* d_llr, uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, int per_frame_msg,
-                  int nframes_total, int resume, int save, int timed)
+
+static int class_of(const DCode& cd) { return cd.dmax <= LOW_MAX_DEG ? 0 : 1; }
+
+template <int DI>
+static int class_grid(ldpc_gpu_ctx* c, size_t lds, int nrun, int* grid)
 {
-    if (nrun <= 0)
-        return LDPC_GPU_OK;
-    const size_t lds = (size_t)maxn;
     int per_cu = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_layered_kernel, NT, lds));
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_layered_kernel<DI>, NT, lds));
     if (per_cu < 1)
         return set_err(c, LDPC_GPU_EDEVICE, "kernel does not fit (LDS %zu)", lds);
-    const int grid = std::min(nrun, per_cu * c->ncu);
-    const size_t msg_stride = (size_t)maxr; /* uint2 per block */
-    const size_t blocks = per_frame_msg ? (size_t)nframes_total : (size_t)grid;
-    int rc = grow(c, &c->msg, &c->msg_bytes, blocks * msg_stride * sizeof(uint2));
+    *grid = std::min(nrun, per_cu * c->ncu);
+    return LDPC_GPU_OK;
+}
+
+template <int DI>
+static int launch_class(ldpc_gpu_ctx* c, const KArgs& a, size_t lds, int grid, int cls)
+{
+    if (!a.msg_per_frame && (size_t)grid * a.msg_stride * sizeof(uint32_t) > c->msg_bytes)
+        return set_err(c, LDPC_GPU_EINVAL, "message scratch too small");
+    HIPCHK(c, hipEventRecord(c->ev_cls[2 * cls], c->stream));
+    hipLaunchKernelGGL(ldpc_layered_kernel<DI>, dim3(grid), dim3(NT), lds, c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev_cls[2 * cls + 1], c->stream));
+    return LDPC_GPU_OK;
+}
+
+/* Decode the frames of `run` (indices into the frame table): frames are
+ * split by degree class, each class is one persistent launch over its
+ * frames, normal frames before short ones. */
+static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFrame>& hf, const std::vector<int>& run,
+                  int maxw, const int8_t* d_llr, uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, int per_frame_msg,
+                  int nframes_total, int resume, int save, int timed)
+{
+    if (run.empty())
+        return LDPC_GPU_OK;
+    const std::vector<DCode>& hc = c->tab->h_codes;
+    std::vector<int> lists[NCLASS];
+    int maxn[NCLASS] = { 0 };
+    for (int f : run) {
+        const DCode& cd = hc[hf[f].code];
+        const int k = class_of(cd);
+        lists[k].push_back(f);
+        maxn[k] = std::max(maxn[k], cd.n);
+    }
+    std::vector<int> all;
+    int off[NCLASS];
+    for (int k = 0; k < NCLASS; k++) {
+        std::stable_sort(lists[k].begin(), lists[k].end(),
+                         [&](int x, int y) { return hc[hf[x].code].n > hc[hf[y].code].n; });
+        off[k] = (int)all.size();
+        all.insert(all.end(), lists[k].begin(), lists[k].end());
+    }
+    int rc = grow(c, &c->d_list, &c->list_bytes, sizeof(int) * all.size());
     if (rc)
         return rc;
+    /* message blocks: one per workgroup of the largest grid, or per frame */
+    int grid[NCLASS] = { 0 }, maxgrid = 0;
+    for (int k = 0; k < NCLASS; k++) {
+        if (lists[k].empty())
+            continue;
+        const size_t lds = (size_t)maxn[k];
+        const int nr = (int)lists[k].size();
+        rc = k == 0 ? class_grid<0>(c, lds, nr, &grid[k]) : class_grid<1>(c, lds, nr, &grid[k]);
+        if (rc)
+            return rc;
+        maxgrid = std::max(maxgrid, grid[k]);
+    }
+    const size_t msg_stride = (size_t)maxw;
+    const size_t blocks = per_frame_msg ? (size_t)nframes_total : (size_t)maxgrid;
+    if ((rc = grow(c, &c->msg, &c->msg_bytes, blocks * msg_stride * sizeof(uint32_t))))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_list, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(64), 0, c->stream, c->ticket);
     KArgs a;
     memset(&a, 0, sizeof(a));
     a.codes = c->tab->codes;
     a.ent = c->tab->ent;
     a.lay = c->tab->lay;
-    a.lev = c->tab->lev;
+    a.meta = c->tab->meta;
     a.frames = d_frames;
-    a.list = d_list;
-    a.nrun = nrun;
     a.llr = d_llr;
     a.out = d_out;
     a.iters = d_iters;
@@ -519,12 +763,20 @@ static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const int* d_list, in
     a.target = c->target;
     a.resume = resume;
     a.save = save;
-    a.ticket = c->ticket;
-    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(1), 0, c->stream, c->ticket);
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(ldpc_layered_kernel, dim3(grid), dim3(NT), lds, c->stream, a);
-    HIPCHK(c, hipGetLastError());
+    for (int k = 0; k < NCLASS; k++) {
+        c->cls_frames[k] = (int)lists[k].size();
+        if (lists[k].empty())
+            continue;
+        a.list = c->d_list + off[k];
+        a.nrun = (int)lists[k].size();
+        a.ticket = c->ticket + k;
+        const size_t lds = (size_t)maxn[k];
+        rc = k == 0 ? launch_class<0>(c, a, lds, grid[k], k) : launch_class<1>(c, a, lds, grid[k], k);
+        if (rc)
+            return rc;
+    }
     if (timed) {
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         c->timed = 1;
@@ -544,7 +796,7 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
     HIPCHK(c, hipSetDevice(c->device));
     const std::vector<DCode>& hc = c->tab->h_codes;
     std::vector<DFrame> df(nframes);
-    int maxn = 0, maxr = 0, ngroup = 0;
+    int maxw = 0, ngroup = 0;
     for (int f = 0; f < nframes; f++) {
         const ldpc_gpu_frame& s = frames[f];
         if (s.code < 0 || s.code >= (int)hc.size())
@@ -558,16 +810,18 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
         df[f].pad = 0;
         df[f].llr_off = s.llr_off;
         df[f].out_off = s.out_off;
-        maxn = std::max(maxn, cd.n);
-        maxr = std::max(maxr, cd.r);
+        maxw = std::max(maxw, cd.msg_words);
         ngroup += s.group >= 0;
     }
     int rc = grow(c, &c->d_frames, &c->frames_bytes, sizeof(DFrame) * nframes);
     if (rc)
         return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_frames, df.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice, c->stream));
+    std::vector<int> every(nframes);
+    for (int f = 0; f < nframes; f++)
+        every[f] = f;
     if (!ngroup)
-        return launch(c, c->d_frames, nullptr, nframes, maxn, maxr, d_llr, d_out, d_iters, d_synd, 0, nframes, 0, 0, 1);
+        return launch(c, c->d_frames, df, every, maxw, d_llr, d_out, d_iters, d_synd, 0, nframes, 0, 0, 1);
 
     /* Groups (the reference's batch semantics): decode every frame with its
      * own early stop and keep its state; a group's count T is the largest
@@ -577,10 +831,9 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
      * equals iterating the group in lock step. */
     if ((rc = grow(c, &c->state, &c->state_bytes, (size_t)nframes * 64800)) ||
         (rc = grow(c, &c->it_state, &c->it_state_bytes, sizeof(int32_t) * nframes)) ||
-        (rc = grow(c, &c->target, &c->target_bytes, sizeof(int32_t) * nframes)) ||
-        (rc = grow(c, &c->d_list, &c->list_bytes, sizeof(int) * nframes)))
+        (rc = grow(c, &c->target, &c->target_bytes, sizeof(int32_t) * nframes)))
         return rc;
-    rc = launch(c, c->d_frames, nullptr, nframes, maxn, maxr, d_llr, d_out, d_iters, d_synd, 1, nframes, 0, 1, 1);
+    rc = launch(c, c->d_frames, df, every, maxw, d_llr, d_out, d_iters, d_synd, 1, nframes, 0, 1, 1);
     if (rc)
         return rc;
     std::vector<int32_t> it(nframes), tgt(nframes);
@@ -631,9 +884,7 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
         if (list.empty())
             break;
         HIPCHK(c, hipMemcpyAsync(c->target, tgt.data(), sizeof(int32_t) * nframes, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->stream));
-        rc = launch(c, c->d_frames, c->d_list, (int)list.size(), maxn, maxr, d_llr, d_out, d_iters, d_synd, 1, nframes,
-                    1, 1, 0);
+        rc = launch(c, c->d_frames, df, list, maxw, d_llr, d_out, d_iters, d_synd, 1, nframes, 1, 1, 0);
         if (rc)
             return rc;
         HIPCHK(c, hipMemcpyAsync(it.data(), d_iters, sizeof(int32_t) * nframes, hipMemcpyDeviceToHost, c->stream));

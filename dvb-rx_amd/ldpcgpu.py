@@ -77,6 +77,8 @@ def gpu_lib():
         L.ldpc_gpu_decode_host.argtypes = [vp, vp, i32, vp, sz, vp, sz, vp, vp]
         L.ldpc_gpu_last_kernel_ms.argtypes = [vp]
         L.ldpc_gpu_last_kernel_ms.restype = C.c_float
+        L.ldpc_gpu_last_launches.argtypes = [vp, ip, ip, C.POINTER(C.c_float), i32]
+        L.ldpc_gpu_code_max_degree.argtypes = [i32]
         L.drx_ldpc_lanes.argtypes = []
         L.drx_ldpc_new.argtypes = [vp, C.c_char_p]
         L.drx_ldpc_new.restype = vp
@@ -135,7 +137,9 @@ def code_params(code):
     if gpu_lib().ldpc_gpu_code_params(code, *[C.byref(x) for x in v]):
         raise KeyError(code)
     n, k, q, lt, steps = [x.value for x in v]
-    return dict(n=n, k=k, q=q, r=n - k, links_total=lt, level_steps=steps)
+    dmax = gpu_lib().ldpc_gpu_code_max_degree(code)
+    return dict(n=n, k=k, q=q, r=n - k, links_total=lt, level_steps=steps, max_degree=dmax,
+                degree_class=12 if dmax <= 12 else 28)
 
 
 def pls_lookup(pls):
@@ -259,6 +263,14 @@ class Decoder:
 
     def last_kernel_ms(self):
         return self.L.ldpc_gpu_last_kernel_ms(self.ctx)
+
+    def last_launches(self):
+        """[(degree class bound, frames, ms)] of the last decode's launches."""
+        di = (C.c_int * 8)()
+        nf = (C.c_int * 8)()
+        ms = (C.c_float * 8)()
+        n = self.L.ldpc_gpu_last_launches(self.ctx, di, nf, ms, 8)
+        return [(di[k], nf[k], ms[k]) for k in range(n)]
 
     def decode_host(self, frames, llr, out_bytes):
         """Synchronous decode of host buffers: returns (out, iters, synd)."""

@@ -59,6 +59,8 @@ const char* ldpc_gpu_last_error(ldpc_gpu_ctx* ctx);
 int ldpc_gpu_code_count(void);
 int ldpc_gpu_code_lookup(int short_frame, const char* rate);
 int ldpc_gpu_code_params(int code, int* n, int* k, int* q, int* links_total, int* level_steps);
+/* Largest information degree of a layer of the code (check degree - 2). */
+int ldpc_gpu_code_max_degree(int code);
 int ldpc_gpu_fec_lookup(int short_frame, const char* rate, int* nldpc, int* kldpc, int* kbch, int* t);
 
 /* Device memory helpers (synchronous copies on the context's stream). */
@@ -91,6 +93,12 @@ int ldpc_gpu_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nf
 /* Timing of the last ldpc_gpu_decode_batch's decode kernel on the stream it
  * ran on (HIP events around the launch), in milliseconds; < 0 if none. */
 float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* ctx);
+
+/* Per-launch detail of the last ldpc_gpu_decode_batch: frames are decoded
+ * by one persistent kernel launch per degree class (largest information
+ * degree of a layer <= 8, 16, 28).  Fills up to max entries with the class
+ * bound, the frames of the launch and its time in ms; returns the count. */
+int ldpc_gpu_last_launches(ldpc_gpu_ctx* ctx, int* di, int* nframes, float* ms, int max);
 
 #ifdef __cplusplus
 }

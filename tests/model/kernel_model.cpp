@@ -1,0 +1,195 @@
+/*
+ * CPU model of the HIP decoder's schedule (test infrastructure, never part
+ * of the product): the same code registry (circulant layers, dependency
+ * levels and masks), the same per-node arithmetic (dvb-rx_amd/csrc/ldpc_node.h)
+ * and the same posterior / message layout as the decode kernel, run on one
+ * core.  Within each phase and level the nodes are processed in decreasing
+ * j: the kernel runs them concurrently, so the result must not depend on
+ * their order; a wrong level or mask would change the result against the
+ * oracle, which processes them in increasing j.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ldpc_codes.h"
+#include "ldpc_node.h"
+
+extern "C" {
+int kmodel_decode(int code, const int8_t* llr, int max_iter, int nbits, uint8_t* out, int* ok);
+int kmodel_check_levels(int code);
+}
+
+template <int DEG>
+static void model_layer(const lg_code* c, int i, int8_t* post, uint32_t* msg, int first)
+{
+    const int K = c->k, q = c->q;
+    const uint32_t* ent = c->ent + c->layer_start[i];
+    const uint64_t* meta = c->meta + (size_t)i * 360;
+    static LgNode<DEG> nodes[360];
+    uint32_t mw[360][LG_MSG_WORDS(DEG)];
+    const int W = LG_MSG_WORDS(DEG);
+    uint32_t* m = msg + (size_t)c->msg_off[i] * 360;
+    for (int j = 0; j < 360; j++)
+        for (int w = 0; w < W; w++)
+            mw[j][w] = first ? 0 : m[w * 360 + j];
+    for (int j = 359; j >= 0; j--) {
+        lg_phase_a<DEG>(nodes[j], post, ent, i, j, K, q, LG_META_DEP(meta[j]), mw[j], first);
+        if (LG_META_LEVEL(meta[j]) == 0)
+            lg_write_succ<DEG>(nodes[j], post, LG_META_SUCC(meta[j]));
+    }
+    for (int l = 1; l < c->nlev[i]; l++)
+        for (int j = 359; j >= 0; j--)
+            if (LG_META_LEVEL(meta[j]) == l) {
+                lg_step<DEG>(nodes[j], post, LG_META_DEP(meta[j]), mw[j], first);
+                lg_write_succ<DEG>(nodes[j], post, LG_META_SUCC(meta[j]));
+            }
+    for (int j = 359; j >= 0; j--) {
+        uint32_t o[LG_MSG_WORDS(DEG)];
+        lg_phase_c<DEG>(nodes[j], post, LG_META_SUCC(meta[j]), o);
+        for (int w = 0; w < W; w++)
+            m[w * 360 + j] = o[w];
+    }
+}
+
+template <int DEG>
+static int model_check(const lg_code* c, int i, const int8_t* post)
+{
+    int good = 1;
+    for (int j = 0; j < 360; j++)
+        good &= lg_node_check_t<DEG>(post, c->ent + c->layer_start[i], DEG, i, j, c->k, c->q);
+    return good;
+}
+
+#define DEG_CASES(X)                                                                                      \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) \
+        X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28)
+
+static int layer_any(const lg_code* c, int i, int8_t* post, uint32_t* msg, int first)
+{
+    switch (c->layer_start[i + 1] - c->layer_start[i]) {
+#define X(D)                                 \
+    case D:                                  \
+        model_layer<D>(c, i, post, msg, first); \
+        return 0;
+        DEG_CASES(X)
+#undef X
+    }
+    return -1;
+}
+
+static int check_any(const lg_code* c, int i, const int8_t* post)
+{
+    switch (c->layer_start[i + 1] - c->layer_start[i]) {
+#define X(D) \
+    case D:  \
+        return model_check<D>(c, i, post);
+        DEG_CASES(X)
+#undef X
+    }
+    return 0;
+}
+
+/* Decode one frame with per-frame early stop.  Returns iterations run;
+ * *ok = syndrome holds.  out: ceil(nbits/8) bytes, MSB first. */
+int kmodel_decode(int code, const int8_t* llr, int max_iter, int nbits, uint8_t* out, int* ok)
+{
+    const lg_code* c = lg_code_get(code);
+    if (!c)
+        return -1;
+    const int K = c->k, R = c->r, q = c->q;
+    int8_t* post = (int8_t*)malloc((size_t)c->n);
+    uint32_t* msg = (uint32_t*)calloc((size_t)c->msg_words, sizeof(uint32_t));
+    if (!post || !msg) {
+        free(post);
+        free(msg);
+        return -1;
+    }
+    memcpy(post, llr, (size_t)K);
+    for (int t = 0; t < R; t++)
+        post[K + 360 * (t % q) + t / q] = llr[K + t];
+    int it = 0, good;
+    for (;;) {
+        good = 1;
+        for (int i = 0; i < q && good; i++)
+            good &= check_any(c, i, post);
+        if (good || it >= max_iter)
+            break;
+        for (int i = 0; i < q; i++)
+            if (layer_any(c, i, post, msg, it == 0)) {
+                free(post);
+                free(msg);
+                return -1;
+            }
+        it++;
+    }
+    if (nbits <= 0 || nbits > K)
+        nbits = K;
+    memset(out, 0, (size_t)(nbits + 7) / 8);
+    for (int v = 0; v < nbits; v++)
+        if (post[v] < 0)
+            out[v / 8] |= (uint8_t)(0x80 >> (v % 8));
+    *ok = good;
+    free(post);
+    free(msg);
+    return it;
+}
+
+/* The schedule's own invariants: no two nodes of one level of a layer
+ * share a variable, and the masks name exactly the shared edges.  Returns
+ * the number of violations. */
+int kmodel_check_levels(int code)
+{
+    const lg_code* c = lg_code_get(code);
+    if (!c)
+        return -1;
+    int bad = 0;
+    int* seen = (int*)malloc(sizeof(int) * (size_t)c->n);
+    int* count = (int*)malloc(sizeof(int) * (size_t)c->n);
+    if (!seen || !count) {
+        free(seen);
+        free(count);
+        return -1;
+    }
+    for (int i = 0; i < c->q; i++) {
+        const int st = c->layer_start[i], deg = c->layer_start[i + 1] - st;
+        for (int l = 0; l < c->nlev[i]; l++) {
+            for (int v = 0; v < c->n; v++)
+                seen[v] = -1;
+            for (int j = 0; j < 360; j++) {
+                if (c->level[360 * i + j] != l)
+                    continue;
+                for (int e = 0; e < lg_node_degree(deg, i, j); e++) {
+                    const int ad = lg_edge_addr(c->ent + st, deg, e, i, j, c->k, c->q);
+                    if (seen[ad] >= 0)
+                        bad++;
+                    seen[ad] = j;
+                }
+            }
+        }
+        /* masks: an info edge is dependent iff some earlier node of the
+         * layer touches its bit, successor-shared iff a later one does */
+        for (int v = 0; v < c->n; v++)
+            count[v] = 0;
+        for (int j = 0; j < 360; j++)
+            for (int e = 0; e < deg; e++)
+                count[lg_edge_addr(c->ent + st, deg, e, i, j, c->k, c->q)]++;
+        for (int v = 0; v < c->n; v++)
+            seen[v] = 0;
+        for (int j = 0; j < 360; j++) {
+            const uint64_t m = c->meta[(size_t)360 * i + j];
+            if (LG_META_LEVEL(m) != c->level[360 * i + j])
+                bad++;
+            for (int e = 0; e < deg; e++) {
+                const int ad = lg_edge_addr(c->ent + st, deg, e, i, j, c->k, c->q);
+                const int before = seen[ad], after = count[ad] - seen[ad] - 1;
+                if (((LG_META_DEP(m) >> e) & 1) != (before > 0) || ((LG_META_SUCC(m) >> e) & 1) != (after > 0))
+                    bad++;
+                seen[ad]++;
+            }
+        }
+    }
+    free(seen);
+    free(count);
+    return bad;
+}
