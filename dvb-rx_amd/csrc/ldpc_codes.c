@@ -115,15 +115,18 @@ static int build_code(lg_code* c, int id)
      * the layer touches the same bit, successor-shared if a later one does */
     c->meta = calloc((size_t)c->r, sizeof(uint64_t));
     c->msg_off = malloc(sizeof(int) * (size_t)q);
-    if (!c->meta || !c->msg_off) {
+    c->pk_msg_off = malloc(sizeof(int) * (size_t)q);
+    if (!c->meta || !c->msg_off || !c->pk_msg_off) {
         free(owner);
         return -1;
     }
-    int rows = 0;
+    int rows = 0, pk_rows = 0;
     for (int i = 0; i < q; i++) {
         const int s = c->layer_start[i], d = c->layer_start[i + 1] - s;
         c->msg_off[i] = rows;
         rows += (d + 2 + 3) / 4;
+        c->pk_msg_off[i] = pk_rows;
+        pk_rows += (d + 2 + 1) / 2;
         uint64_t* mt = c->meta + (size_t)i * LG_M;
         for (int j = 0; j < LG_M; j++)
             mt[j] = c->level[(size_t)i * LG_M + j];
@@ -150,6 +153,7 @@ static int build_code(lg_code* c, int id)
         }
     }
     c->msg_words = rows * LG_M;
+    c->pk_msg_words = pk_rows * LG_M;
     free(owner);
     /* every edge counted: info edges plus the 2R-1 of the staircase */
     if ((long)naddr * LG_M + 2L * c->r - 1 != c->links_total)

@@ -56,6 +56,8 @@ typedef struct lg_code {
     uint64_t* meta;
     int msg_words;         /* 32-bit message words per frame: 360 * sum of ceil((deg_i+2)/4) */
     int* msg_off;          /* q: first message word row of each layer (rows of 360 words) */
+    int pk_msg_words;      /* the same for two frames, two edges a word: 360 * sum of ceil((deg_i+2)/2) */
+    int* pk_msg_off;       /* q: first row of each layer in that layout */
 } lg_code;
 
 #define LG_META_LEVEL(m) ((int)((m) & 0xff))

@@ -37,9 +37,11 @@
 #include "ldpc_codes.h"
 #include "ldpc_gpu.h"
 #include "ldpc_node.h"
+#include "ldpc_pk.h"
 
 #define NT 384 /* threads per workgroup: 360 check nodes of a layer */
 #define NCLASS 2 /* degree classes: one kernel instance each (registers grow with the degree) */
+#define NLAUNCH 4 /* launches of one decode: degree class x frame size */
 static const int CLASS_DI[NCLASS] = { 12, 28 }; /* largest layer degree of each class */
 
 namespace {
@@ -53,7 +55,7 @@ struct DCode {
     int dmax;      /* largest information degree of a layer */
     int dmin;      /* smallest */
     int msg_words; /* 32-bit message words per frame */
-    int pad;
+    int pk_msg_words; /* the same for the two-frame layout */
 };
 
 struct DFrame {
@@ -256,10 +258,11 @@ __device__ __forceinline__ int decode_frame(int8_t* sm, const uint32_t* s_ent, c
     return ok;
 }
 
-__global__ void __launch_bounds__(64) reset_ticket(unsigned* t)
+#define NCOUNTERS 256 /* work counters: tickets and per-segment claims */
+
+__global__ void __launch_bounds__(256) reset_ticket(unsigned* t)
 {
-    if (threadIdx.x < 4)
-        t[threadIdx.x] = 0;
+    t[threadIdx.x] = 0;
 }
 
 template <int CLS>
@@ -346,6 +349,325 @@ __global__ void __launch_bounds__(NT) ldpc_layered_kernel(KArgs a)
     }
 }
 
+/* ---- two frames per workgroup: packed 16-bit arithmetic (ldpc_pk.h) ----- */
+
+#define PK_WMAX 15 /* message words of a node: (28 + 2 + 1) / 2 */
+
+struct PkArgs {
+    const DCode* codes;
+    const uint32_t* ent;
+    const uint2* lay;
+    const uint32_t* pkoff; /* per layer: first message row in the two-frame layout */
+    const uint64_t* meta;
+    const DFrame* frames;
+    const int* list;       /* frame indices, grouped by code into segments */
+    const int2* seg;       /* per segment: first list entry, length */
+    int nseg;
+    unsigned* cnt;         /* per segment: frames claimed so far */
+    const int8_t* llr;
+    uint8_t* out;
+    int32_t* iters;
+    uint8_t* synd;
+    uint32_t* msg;         /* message block per workgroup */
+    size_t msg_stride;
+};
+
+/* AND of a 2-bit value over the workgroup, one barrier: three vote words
+ * in turn, the next one reset before the barrier of the current vote */
+__device__ __forceinline__ uint32_t wg_and2(uint32_t v, unsigned* s_vote, int& n)
+{
+    const int slot = n % 3;
+    n++;
+    const uint64_t f0 = __ballot(!(v & 1)), f1 = __ballot(!(v & 2));
+    if ((threadIdx.x & 63) == 0)
+        atomicAnd(&s_vote[slot], (f0 ? 0u : 1u) | (f1 ? 0u : 2u));
+    if (threadIdx.x == 0)
+        s_vote[(slot + 1) % 3] = 3;
+    __syncthreads();
+    return s_vote[slot];
+}
+
+/* syndrome of both slots, chunks of layers with an early exit once every
+ * slot that is still decoding has failed a check; bit k: slot k holds */
+template <int CLS>
+__device__ __forceinline__ uint32_t pk_syndrome(const uint16_t* sm, const uint32_t* ent, const uint2* lay, int j,
+                                                int K, int q, uint32_t live, unsigned* s_vote, int& nvote)
+{
+    uint32_t good = 3;
+    for (int i0 = 0; i0 < q; i0 += SYN_CHUNK) {
+        uint32_t ok = 3;
+        if (j < 360) {
+            const int i1 = min(i0 + SYN_CHUNK, q);
+            for (int i = i0; i < i1; i++) {
+                const uint2 ly = lay[i];
+                const uint32_t* e = ent + lay_start(ly);
+#define X(D)                                             \
+    case D:                                              \
+        ok &= pk_check<D>(sm, e, i, j, K, q);            \
+        break;
+                if constexpr (CLS == 0) {
+                    switch (lay_deg(ly)) {
+                        LOW_DEGS(X)
+                    default:
+                        ok = 0;
+                    }
+                } else {
+                    switch (lay_deg(ly)) {
+                        HIGH_DEGS(X)
+                    default:
+                        ok = 0;
+                    }
+                }
+#undef X
+            }
+        }
+        good &= wg_and2(ok, s_vote, nvote);
+        if (!(good & live))
+            break;
+    }
+    return good;
+}
+
+template <int DEG, bool TAIL>
+__device__ __forceinline__ void pk_layer_t(uint16_t* sm, const uint32_t* ent, int i, int j, int K, int q, int nlev,
+                                           const uint32_t* mw, uint64_t meta, uint32_t* mrow, uint32_t se, uint32_t so)
+{
+    const bool act = j < 360;
+    const uint32_t dep = TAIL ? LG_META_DEP(meta) : 0, succ = TAIL ? LG_META_SUCC(meta) : 0;
+    const int lvl = LG_META_LEVEL(meta);
+    PkNode<DEG> n;
+    if (act) {
+        pk_phase_a<DEG, TAIL>(n, sm, ent, i, j, K, q, dep, mw, se, so);
+        if (TAIL && lvl == 0)
+            pk_write_succ<DEG>(n, sm, succ);
+    }
+    if (TAIL) {
+        for (int l = 1; l < nlev; l++) {
+            __syncthreads();
+            if (act && lvl == l) {
+                pk_step<DEG>(n, sm, dep, mw, se, so);
+                pk_write_succ<DEG>(n, sm, succ);
+            }
+        }
+    }
+    if (act) {
+        uint32_t o[LG_PK_WORDS(DEG)];
+        pk_phase_c<DEG, TAIL>(n, sm, succ, o);
+#pragma unroll
+        for (int w = 0; w < LG_PK_WORDS(DEG); w++)
+            mrow[360 * w + j] = o[w];
+    }
+}
+
+/* one iteration (all layers) on both slots; the code's tables are read
+ * through uniform pointers (scalar loads) */
+template <int CLS>
+__device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, const uint2* lay,
+                                             const uint32_t* pkoff, const uint64_t* meta, uint32_t* msg, int K,
+                                             int q, int first0, int first1)
+{
+    const int j = threadIdx.x;
+    const bool act = j < 360;
+    const uint32_t se = pk_msg_sel(0, first0, first1), so = pk_msg_sel(1, first0, first1);
+    const bool load = !(first0 && first1);
+    constexpr int WM = CLS == 0 ? LG_PK_WORDS(LOW_MAX_DEG) : PK_WMAX;
+    /* messages and masks of layer i+1 are fetched while layer i runs */
+    uint32_t nx[WM];
+    uint64_t nm = 0;
+#pragma unroll
+    for (int w = 0; w < WM; w++)
+        nx[w] = 0;
+    auto fetch = [&](int l) {
+        if (l < q && act) {
+            const uint2 ln = lay[l];
+            const int W = LG_PK_WORDS(lay_deg(ln));
+            if (load) {
+                const uint32_t* row = msg + 360 * pkoff[l] + j;
+#pragma unroll
+                for (int w = 0; w < WM; w++)
+                    if (w < W)
+                        nx[w] = row[360 * w];
+            }
+            nm = lay_nlev(ln) > 1 ? meta[360 * l + j] : 0;
+        }
+    };
+    fetch(0);
+    for (int i = 0; i < q; i++) {
+        uint32_t mw[WM];
+#pragma unroll
+        for (int w = 0; w < WM; w++)
+            mw[w] = nx[w];
+        const uint64_t mt = nm;
+        fetch(i + 1);
+        const uint2 ly = lay[i];
+        const uint32_t* e = ent + lay_start(ly);
+        uint32_t* mrow = msg + 360 * pkoff[i];
+        const int nlev = lay_nlev(ly);
+#define X(D)                                                                        \
+    case D:                                                                         \
+        if (nlev > 1)                                                               \
+            pk_layer_t<D, true>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so);     \
+        else                                                                        \
+            pk_layer_t<D, false>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so);    \
+        break;
+        if constexpr (CLS == 0) {
+            switch (lay_deg(ly)) {
+                LOW_DEGS(X)
+            }
+        } else {
+            switch (lay_deg(ly)) {
+                HIGH_DEGS(X)
+            }
+        }
+#undef X
+        __syncthreads();
+    }
+}
+
+/*
+ * Persistent two-slot decoder.  Frames of the launch are grouped by code
+ * into segments; a workgroup works on one segment at a time, and whenever
+ * a slot's frame is done (syndrome holds or max_iter reached) the slot is
+ * refilled with the next frame of the same segment while the other slot
+ * continues.  When the segment is exhausted and both slots are empty, the
+ * workgroup moves to another segment with frames left, or exits.
+ */
+template <int CLS>
+__global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
+{
+    extern __shared__ __align__(16) uint16_t sm16[];
+    __shared__ unsigned s_vote[3];
+    __shared__ int s_claim[2], s_seg, s_newseg;
+    const int tid = threadIdx.x;
+    if (tid < 3)
+        s_vote[tid] = 3;
+    int nvote = 0;
+    int seg = -1;                 /* current segment (uniform) */
+    int frame[2] = { -1, -1 };    /* frame in each slot, -1: empty */
+    int it[2] = { 0, 0 }, maxit[2] = { 0, 0 };
+    uint32_t* msg = a.msg + a.msg_stride * (size_t)blockIdx.x;
+    DCode cd;
+    const uint64_t* meta = nullptr;
+
+    for (;;) {
+        /* ---- claim frames for the empty slots ---- */
+        if (tid == 0) {
+            int cl[2] = { -1, -1 }, sg = seg, newseg = 0;
+            for (int s = 0; s < 2; s++) {
+                if (frame[s] >= 0 || sg < 0)
+                    continue;
+                const unsigned t = atomicAdd(&a.cnt[sg], 1u);
+                if ((int)t < a.seg[sg].y)
+                    cl[s] = a.list[a.seg[sg].x + t];
+            }
+            if (frame[0] < 0 && frame[1] < 0 && cl[0] < 0 && cl[1] < 0) {
+                /* both slots empty, segment done: take another one */
+                const int start = sg < 0 ? (int)(blockIdx.x % (unsigned)a.nseg) : sg + 1;
+                for (int k = 0; k < a.nseg && cl[0] < 0; k++) {
+                    const int g = (start + k) % a.nseg;
+                    if (__hip_atomic_load(&a.cnt[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (unsigned)a.seg[g].y)
+                        continue;
+                    for (int s = 0; s < 2; s++) {
+                        const unsigned t = atomicAdd(&a.cnt[g], 1u);
+                        if ((int)t < a.seg[g].y) {
+                            cl[s] = a.list[a.seg[g].x + t];
+                            sg = g;
+                            newseg = 1;
+                        }
+                    }
+                }
+            }
+            s_claim[0] = cl[0];
+            s_claim[1] = cl[1];
+            s_seg = sg;
+            s_newseg = newseg;
+        }
+        __syncthreads();
+        const int cl0 = __builtin_amdgcn_readfirstlane(s_claim[0]);
+        const int cl1 = __builtin_amdgcn_readfirstlane(s_claim[1]);
+        const int newseg = __builtin_amdgcn_readfirstlane(s_newseg);
+        seg = __builtin_amdgcn_readfirstlane(s_seg);
+        __syncthreads(); /* s_claim is rewritten only after this */
+        if (cl0 >= 0)
+            frame[0] = cl0;
+        if (cl1 >= 0)
+            frame[1] = cl1;
+        if (frame[0] < 0 && frame[1] < 0)
+            break;
+        const int fany = frame[0] >= 0 ? frame[0] : frame[1];
+        if (newseg) {
+            cd = a.codes[a.frames[fany].code];
+            meta = a.meta + cd.meta_off;
+        }
+        const uint32_t* ent = a.ent + cd.ent_off;
+        const uint2* lay = a.lay + cd.lay_off;
+        const uint32_t* pkoff = a.pkoff + cd.lay_off;
+        const int K = cd.k, R = cd.r, q = cd.q;
+        /* ---- load the new frames into their slot bytes ---- */
+        for (int s = 0; s < 2; s++) {
+            const int f = s ? cl1 : cl0;
+            if (f < 0)
+                continue;
+            const DFrame fr = a.frames[f];
+            it[s] = 0;
+            maxit[s] = fr.max_iter;
+            const uint32_t* src = (const uint32_t*)(a.llr + fr.llr_off);
+            uint8_t* sb = (uint8_t*)sm16 + s;
+            for (int v4 = tid; v4 < K / 4; v4 += NT) {
+                const uint32_t w = src[v4];
+                for (int b = 0; b < 4; b++)
+                    sb[2 * (4 * v4 + b)] = (uint8_t)(w >> (8 * b));
+            }
+            const uint8_t* par = (const uint8_t*)(a.llr + fr.llr_off + K);
+            for (int t = tid; t < R; t += NT) {
+                const int jj = t / q, ii = t - jj * q;
+                sb[2 * (K + 360 * ii + jj)] = par[t];
+            }
+        }
+        __syncthreads();
+
+        /* ---- iterate until a slot is done ---- */
+        for (;;) {
+            const uint32_t live = (frame[0] >= 0 ? 1u : 0u) | (frame[1] >= 0 ? 2u : 0u);
+            const uint32_t good = pk_syndrome<CLS>(sm16, ent, lay, tid, K, q, live, s_vote, nvote);
+            uint32_t done = 0;
+            for (int s = 0; s < 2; s++)
+                if (frame[s] >= 0 && (((good >> s) & 1) || it[s] >= maxit[s]))
+                    done |= 1u << s;
+            if (done) {
+                for (int s = 0; s < 2; s++) {
+                    if (!((done >> s) & 1))
+                        continue;
+                    const DFrame fr = a.frames[frame[s]];
+                    const int nbytes = (fr.nbits + 7) / 8;
+                    uint8_t* o = a.out + fr.out_off;
+                    const uint8_t* sb = (const uint8_t*)sm16 + s;
+                    for (int b4 = 4 * tid; b4 < nbytes; b4 += 4 * NT) {
+                        uint32_t v = 0;
+                        for (int k = 0; k < 32; k++) {
+                            const int bit = 8 * b4 + k;
+                            v = v << 1 | (uint32_t)(bit < fr.nbits && (int8_t)sb[2 * bit] < 0);
+                        }
+                        for (int k = 0; k < 4 && b4 + k < nbytes; k++)
+                            o[b4 + k] = (uint8_t)(v >> (24 - 8 * k));
+                    }
+                    if (tid == 0) {
+                        a.iters[frame[s]] = it[s];
+                        a.synd[frame[s]] = (uint8_t)((good >> s) & 1);
+                    }
+                    frame[s] = -1;
+                }
+                __syncthreads(); /* slot bytes read before a refill overwrites them */
+                break;
+            }
+            pk_iteration<CLS>(sm16, ent, lay, pkoff, meta, msg, K, q, frame[0] < 0 || it[0] == 0,
+                              frame[1] < 0 || it[1] == 0);
+            for (int s = 0; s < 2; s++)
+                it[s] += frame[s] >= 0;
+        }
+    }
+}
+
 /* ---- device-resident code tables (one copy per device) ------------------ */
 
 struct DeviceTables {
@@ -353,6 +675,7 @@ struct DeviceTables {
     DCode* codes = nullptr;
     uint32_t* ent = nullptr;
     uint2* lay = nullptr;
+    uint32_t* pkoff = nullptr;
     uint64_t* meta = nullptr;
     std::vector<DCode> h_codes;
 };
@@ -365,6 +688,7 @@ DeviceTables g_tables[64];
 struct ldpc_gpu_ctx {
     int device;
     hipStream_t stream;
+    hipStream_t aux[NLAUNCH]; /* forked streams of concurrent launches */
     char err[256];
     DeviceTables* tab;
     int ncu;
@@ -382,10 +706,14 @@ struct ldpc_gpu_ctx {
     size_t it_state_bytes;
     int32_t* target;
     size_t target_bytes;
+    int2* d_seg;
+    size_t seg_bytes;
     hipEvent_t ev0, ev1;
-    hipEvent_t ev_cls[2 * NCLASS];
-    int cls_frames[NCLASS];
+    hipEvent_t ev_cls[2 * NLAUNCH];
+    int cls_frames[NLAUNCH]; /* frames of each launch of the last decode */
+    int cls_di[NLAUNCH];     /* its degree class bound */
     int timed;
+    int force_scalar;        /* per-frame batches through the one-frame kernel */
 };
 
 static int set_err(ldpc_gpu_ctx* c, int code, const char* fmt, ...)
@@ -419,6 +747,7 @@ static int upload_tables(ldpc_gpu_ctx* c)
     std::vector<DCode> codes(nc);
     std::vector<uint32_t> ent;
     std::vector<uint2> lay;
+    std::vector<uint32_t> pkoff;
     std::vector<uint64_t> meta;
     for (int id = 0; id < nc; id++) {
         const lg_code* cd = lg_code_get(id);
@@ -436,13 +765,14 @@ static int upload_tables(ldpc_gpu_ctx* c)
         d.dmax = cd->max_layer_deg;
         d.dmin = 255;
         d.msg_words = cd->msg_words;
-        d.pad = 0;
+        d.pk_msg_words = cd->pk_msg_words;
         ent.insert(ent.end(), cd->ent, cd->ent + cd->nent);
         for (int i = 0; i < cd->q; i++) {
             const int st = cd->layer_start[i], deg = cd->layer_start[i + 1] - st;
             d.dmin = std::min(d.dmin, deg);
             lay.push_back(make_uint2((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24,
                                      (uint32_t)cd->msg_off[i]));
+            pkoff.push_back((uint32_t)cd->pk_msg_off[i]);
         }
         meta.insert(meta.end(), cd->meta, cd->meta + cd->r);
         /* every layer degree must be one the code's class kernel handles */
@@ -455,6 +785,8 @@ static int upload_tables(ldpc_gpu_ctx* c)
     HIPCHK(c, hipMalloc((void**)&t.ent, sizeof(uint32_t) * ent.size()));
     HIPCHK(c, hipMalloc((void**)&t.lay, sizeof(uint2) * lay.size()));
     HIPCHK(c, hipMalloc((void**)&t.meta, sizeof(uint64_t) * meta.size()));
+    HIPCHK(c, hipMalloc((void**)&t.pkoff, sizeof(uint32_t) * pkoff.size()));
+    HIPCHK(c, hipMemcpy(t.pkoff, pkoff.data(), sizeof(uint32_t) * pkoff.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.codes, codes.data(), sizeof(DCode) * codes.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.ent, ent.data(), sizeof(uint32_t) * ent.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.lay, lay.data(), sizeof(uint2) * lay.size(), hipMemcpyHostToDevice));
@@ -483,9 +815,15 @@ int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
     }
     int rc;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        [&] {
+            for (int k = 0; k < NLAUNCH; k++)
+                if (hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking) != hipSuccess)
+                    return true;
+            return false;
+        }() ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         [&] {
-            for (int k = 0; k < 2 * NCLASS; k++)
+            for (int k = 0; k < 2 * NLAUNCH; k++)
                 if (hipEventCreate(&c->ev_cls[k]) != hipSuccess)
                     return true;
             return false;
@@ -495,7 +833,7 @@ int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
     }
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         c->ncu = 256;
-    if ((rc = upload_tables(c)) != LDPC_GPU_OK || hipMalloc((void**)&c->ticket, 64) != hipSuccess) {
+    if ((rc = upload_tables(c)) != LDPC_GPU_OK || hipMalloc((void**)&c->ticket, NCOUNTERS * sizeof(unsigned)) != hipSuccess) {
         ldpc_gpu_destroy(c);
         return rc ? rc : LDPC_GPU_EDEVICE;
     }
@@ -521,9 +859,15 @@ void ldpc_gpu_destroy(ldpc_gpu_ctx* c)
         (void)hipEventDestroy(c->ev0);
     if (c->ev1)
         (void)hipEventDestroy(c->ev1);
-    for (int k = 0; k < 2 * NCLASS; k++)
+    for (int k = 0; k < 2 * NLAUNCH; k++)
         if (c->ev_cls[k])
             (void)hipEventDestroy(c->ev_cls[k]);
+    (void)hipFree(c->d_seg);
+    for (int k = 0; k < NLAUNCH; k++)
+        if (c->aux[k]) {
+            (void)hipStreamSynchronize(c->aux[k]);
+            (void)hipStreamDestroy(c->aux[k]);
+        }
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     free(c);
@@ -621,12 +965,20 @@ int ldpc_gpu_synchronize(ldpc_gpu_ctx* c)
     return LDPC_GPU_OK;
 }
 
+int ldpc_gpu_set_kernel(ldpc_gpu_ctx* c, int kernel)
+{
+    if (!c || (kernel != LDPC_GPU_KERNEL_AUTO && kernel != LDPC_GPU_KERNEL_SINGLE))
+        return LDPC_GPU_EINVAL;
+    c->force_scalar = kernel == LDPC_GPU_KERNEL_SINGLE;
+    return LDPC_GPU_OK;
+}
+
 int ldpc_gpu_last_launches(ldpc_gpu_ctx* c, int* di, int* nframes, float* ms, int max)
 {
     if (!c || !c->timed)
         return 0;
     int n = 0;
-    for (int k = 0; k < NCLASS && n < max; k++) {
+    for (int k = 0; k < NLAUNCH && n < max; k++) {
         if (!c->cls_frames[k])
             continue;
         float t = -1.0f;
@@ -634,7 +986,7 @@ int ldpc_gpu_last_launches(ldpc_gpu_ctx* c, int* di, int* nframes, float* ms, in
             hipEventElapsedTime(&t, c->ev_cls[2 * k], c->ev_cls[2 * k + 1]) != hipSuccess)
             t = -1.0f;
         if (di)
-            di[n] = CLASS_DI[k];
+            di[n] = c->cls_di[k];
         if (nframes)
             nframes[n] = c->cls_frames[k];
         if (ms)
@@ -742,7 +1094,7 @@ static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFr
     if ((rc = grow(c, &c->msg, &c->msg_bytes, blocks * msg_stride * sizeof(uint32_t))))
         return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_list, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(64), 0, c->stream, c->ticket);
+    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(NCOUNTERS), 0, c->stream, c->ticket);
     KArgs a;
     memset(&a, 0, sizeof(a));
     a.codes = c->tab->codes;
@@ -765,8 +1117,11 @@ static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFr
     a.save = save;
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int k = 0; k < NLAUNCH; k++)
+        c->cls_frames[k] = 0;
     for (int k = 0; k < NCLASS; k++) {
         c->cls_frames[k] = (int)lists[k].size();
+        c->cls_di[k] = CLASS_DI[k];
         if (lists[k].empty())
             continue;
         a.list = c->d_list + off[k];
@@ -781,6 +1136,120 @@ static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFr
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         c->timed = 1;
     }
+    return LDPC_GPU_OK;
+}
+
+template <int CLS>
+static int pk_grid(ldpc_gpu_ctx* c, size_t lds, int nframes, int* grid)
+{
+    int per_cu = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_pk_kernel<CLS>, NT, lds));
+    if (per_cu < 1)
+        return set_err(c, LDPC_GPU_EDEVICE, "two-slot kernel does not fit (LDS %zu)", lds);
+    *grid = std::min((nframes + 1) / 2, per_cu * c->ncu);
+    return LDPC_GPU_OK;
+}
+
+/* Frames with their own early stop: two per workgroup (ldpc_pk_kernel).
+ * One launch per (degree class, frame size), each over segments of frames
+ * of one code. */
+static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFrame>& hf, const int8_t* d_llr,
+                     uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd)
+{
+    const std::vector<DCode>& hc = c->tab->h_codes;
+    std::vector<int> gcodes[NLAUNCH];
+    std::vector<std::vector<int>> gseg[NLAUNCH];
+    int maxn[NLAUNCH] = { 0 }, maxw = 0;
+    for (int f = 0; f < (int)hf.size(); f++) {
+        const DCode& cd = hc[hf[f].code];
+        const int k = class_of(cd) * 2 + (cd.n < 64800);
+        auto it = std::find(gcodes[k].begin(), gcodes[k].end(), hf[f].code);
+        size_t g = it - gcodes[k].begin();
+        if (it == gcodes[k].end()) {
+            gcodes[k].push_back(hf[f].code);
+            gseg[k].emplace_back();
+        }
+        gseg[k][g].push_back(f);
+        maxn[k] = std::max(maxn[k], cd.n);
+        maxw = std::max(maxw, cd.pk_msg_words);
+    }
+    std::vector<int> list;
+    std::vector<int2> segs;
+    int seg0[NLAUNCH], nseg[NLAUNCH], nfr[NLAUNCH], grid[NLAUNCH] = { 0 }, maxgrid = 0;
+    for (int k = 0; k < NLAUNCH; k++) {
+        seg0[k] = (int)segs.size();
+        nseg[k] = (int)gseg[k].size();
+        nfr[k] = 0;
+        for (auto& sg : gseg[k]) {
+            segs.push_back(make_int2((int)list.size(), (int)sg.size()));
+            list.insert(list.end(), sg.begin(), sg.end());
+            nfr[k] += (int)sg.size();
+        }
+        if (!nfr[k])
+            continue;
+        const size_t lds = 2 * (size_t)maxn[k];
+        const int rc = k < 2 ? pk_grid<0>(c, lds, nfr[k], &grid[k]) : pk_grid<1>(c, lds, nfr[k], &grid[k]);
+        if (rc)
+            return rc;
+        maxgrid = std::max(maxgrid, grid[k]);
+    }
+    if (segs.size() > NCOUNTERS)
+        return set_err(c, LDPC_GPU_EINVAL, "too many codes in one batch");
+    int rc;
+    size_t blocks = 0;
+    for (int k = 0; k < NLAUNCH; k++)
+        blocks += (size_t)grid[k];
+    (void)maxgrid;
+    if ((rc = grow(c, &c->d_list, &c->list_bytes, sizeof(int) * list.size())) ||
+        (rc = grow(c, &c->d_seg, &c->seg_bytes, sizeof(int2) * segs.size())) ||
+        (rc = grow(c, &c->msg, &c->msg_bytes, blocks * maxw * sizeof(uint32_t))))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_seg, segs.data(), sizeof(int2) * segs.size(), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(NCOUNTERS), 0, c->stream, c->ticket);
+    PkArgs a;
+    memset(&a, 0, sizeof(a));
+    a.codes = c->tab->codes;
+    a.ent = c->tab->ent;
+    a.lay = c->tab->lay;
+    a.pkoff = c->tab->pkoff;
+    a.meta = c->tab->meta;
+    a.frames = d_frames;
+    a.list = c->d_list;
+    a.llr = d_llr;
+    a.out = d_out;
+    a.iters = d_iters;
+    a.synd = d_synd;
+    a.msg = c->msg;
+    a.msg_stride = (size_t)maxw;
+    /* the launches run concurrently on forked streams, so that the last
+     * frames of one overlap the others' work; each has its own messages */
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    size_t moff = 0;
+    for (int k = 0; k < NLAUNCH; k++) {
+        c->cls_frames[k] = nfr[k];
+        c->cls_di[k] = CLASS_DI[k / 2];
+        if (!nfr[k])
+            continue;
+        a.seg = c->d_seg + seg0[k];
+        a.nseg = nseg[k];
+        a.cnt = c->ticket + seg0[k];
+        a.msg = c->msg + moff;
+        moff += (size_t)grid[k] * maxw;
+        const size_t lds = 2 * (size_t)maxn[k];
+        hipStream_t st = c->aux[k];
+        HIPCHK(c, hipStreamWaitEvent(st, c->ev0, 0));
+        HIPCHK(c, hipEventRecord(c->ev_cls[2 * k], st));
+        if (k < 2)
+            hipLaunchKernelGGL(ldpc_pk_kernel<0>, dim3(grid[k]), dim3(NT), lds, st, a);
+        else
+            hipLaunchKernelGGL(ldpc_pk_kernel<1>, dim3(grid[k]), dim3(NT), lds, st, a);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev_cls[2 * k + 1], st));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cls[2 * k + 1], 0));
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = 1;
     return LDPC_GPU_OK;
 }
 
@@ -820,6 +1289,8 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
     std::vector<int> every(nframes);
     for (int f = 0; f < nframes; f++)
         every[f] = f;
+    if (!ngroup && !c->force_scalar)
+        return launch_pk(c, c->d_frames, df, d_llr, d_out, d_iters, d_synd);
     if (!ngroup)
         return launch(c, c->d_frames, df, every, maxw, d_llr, d_out, d_iters, d_synd, 0, nframes, 0, 0, 1);
 

@@ -79,6 +79,7 @@ def gpu_lib():
         L.ldpc_gpu_last_kernel_ms.restype = C.c_float
         L.ldpc_gpu_last_launches.argtypes = [vp, ip, ip, C.POINTER(C.c_float), i32]
         L.ldpc_gpu_code_max_degree.argtypes = [i32]
+        L.ldpc_gpu_set_kernel.argtypes = [vp, i32]
         L.drx_ldpc_lanes.argtypes = []
         L.drx_ldpc_new.argtypes = [vp, C.c_char_p]
         L.drx_ldpc_new.restype = vp
@@ -209,12 +210,15 @@ def frames_for(layout, max_iter, group=-1, out_offsets=None):
 class Decoder:
     """A decoder context (device, stream, scratch) of the HIP library."""
 
-    def __init__(self, device=0):
+    KERNELS = {"auto": 0, "single": 1}
+
+    def __init__(self, device=0, kernel="auto"):
         self.L = gpu_lib()
         self.ctx = C.c_void_p()
         rc = self.L.ldpc_gpu_create(device, C.byref(self.ctx))
         if rc:
             raise RuntimeError(f"ldpc_gpu_create({device}) failed: {rc}")
+        self._check(self.L.ldpc_gpu_set_kernel(self.ctx, self.KERNELS[kernel]), "set_kernel")
 
     def close(self):
         if self.ctx:

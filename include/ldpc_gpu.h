@@ -90,6 +90,14 @@ int ldpc_gpu_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nf
                          const int8_t* llr, size_t llr_bytes, uint8_t* out, size_t out_bytes,
                          int32_t* iters, uint8_t* synd);
 
+/* Kernel choice for frames with their own early stop (group -1):
+ * LDPC_GPU_KERNEL_AUTO runs two frames of one code per workgroup in packed
+ * 16-bit arithmetic; LDPC_GPU_KERNEL_SINGLE one frame per workgroup (the
+ * kernel that also serves groups).  Both give identical results. */
+#define LDPC_GPU_KERNEL_AUTO 0
+#define LDPC_GPU_KERNEL_SINGLE 1
+int ldpc_gpu_set_kernel(ldpc_gpu_ctx* ctx, int kernel);
+
 /* Timing of the last ldpc_gpu_decode_batch's decode kernel on the stream it
  * ran on (HIP events around the launch), in milliseconds; < 0 if none. */
 float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* ctx);

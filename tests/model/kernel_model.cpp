@@ -14,6 +14,7 @@
 
 #include "ldpc_codes.h"
 #include "ldpc_node.h"
+#include "ldpc_pk.h"
 
 extern "C" {
 int kmodel_decode(int code, const int8_t* llr, int max_iter, int nbits, uint8_t* out, int* ok);
@@ -133,6 +134,145 @@ int kmodel_decode(int code, const int8_t* llr, int max_iter, int nbits, uint8_t*
     free(post);
     free(msg);
     return it;
+}
+
+/* ---- the two-frame packed path (dvb-rx_amd/csrc/ldpc_pk.h) --------------- */
+
+template <int DEG>
+static void pk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t sel_even,
+                           uint32_t sel_odd)
+{
+    const int K = c->k, q = c->q;
+    const uint32_t* ent = c->ent + c->layer_start[i];
+    const uint64_t* meta = c->meta + (size_t)i * 360;
+    static PkNode<DEG> nodes[360];
+    const int W = LG_PK_WORDS(DEG);
+    static uint32_t mw[360][LG_PK_WORDS(28)];
+    uint32_t* m = msg + (size_t)c->pk_msg_off[i] * 360;
+    for (int j = 0; j < 360; j++)
+        for (int w = 0; w < W; w++)
+            mw[j][w] = m[w * 360 + j];
+    for (int j = 359; j >= 0; j--) {
+        if (c->nlev[i] > 1)
+            pk_phase_a<DEG, true>(nodes[j], post, ent, i, j, K, q, LG_META_DEP(meta[j]), mw[j], sel_even, sel_odd);
+        else
+            pk_phase_a<DEG, false>(nodes[j], post, ent, i, j, K, q, 0, mw[j], sel_even, sel_odd);
+        if (LG_META_LEVEL(meta[j]) == 0)
+            pk_write_succ<DEG>(nodes[j], post, LG_META_SUCC(meta[j]));
+    }
+    for (int l = 1; l < c->nlev[i]; l++)
+        for (int j = 359; j >= 0; j--)
+            if (LG_META_LEVEL(meta[j]) == l) {
+                pk_step<DEG>(nodes[j], post, LG_META_DEP(meta[j]), mw[j], sel_even, sel_odd);
+                pk_write_succ<DEG>(nodes[j], post, LG_META_SUCC(meta[j]));
+            }
+    for (int j = 359; j >= 0; j--) {
+        uint32_t o[LG_PK_WORDS(DEG)];
+        if (c->nlev[i] > 1)
+            pk_phase_c<DEG, true>(nodes[j], post, LG_META_SUCC(meta[j]), o);
+        else
+            pk_phase_c<DEG, false>(nodes[j], post, 0, o);
+        for (int w = 0; w < W; w++)
+            m[w * 360 + j] = o[w];
+    }
+}
+
+static int pk_layer_any(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t se, uint32_t so)
+{
+    switch (c->layer_start[i + 1] - c->layer_start[i]) {
+#define X(D)                                          \
+    case D:                                           \
+        pk_model_layer<D>(c, i, post, msg, se, so);   \
+        return 0;
+        DEG_CASES(X)
+#undef X
+    }
+    return -1;
+}
+
+static uint32_t pk_check_any(const lg_code* c, int i, int j, const uint16_t* post)
+{
+    const uint32_t* e = c->ent + c->layer_start[i];
+    switch (c->layer_start[i + 1] - c->layer_start[i]) {
+#define X(D) \
+    case D:  \
+        return pk_check<D>(post, e, i, j, c->k, c->q);
+        DEG_CASES(X)
+#undef X
+    }
+    return 0;
+}
+
+/* Two frames of one code in the two slots; frame B enters slot 1 `delay`
+ * iterations after frame A started in slot 0 (as when a slot is refilled
+ * while the other one runs).  Each slot stops on its own syndrome. */
+extern "C" int kmodel_pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay, int max_iter,
+                              int nbits, uint8_t* out_a, uint8_t* out_b, int* it_out, int* ok_out)
+{
+    const lg_code* c = lg_code_get(code);
+    if (!c)
+        return -1;
+    const int K = c->k, R = c->r, q = c->q, N = c->n;
+    uint16_t* post = (uint16_t*)calloc((size_t)N, sizeof(uint16_t));
+    uint32_t* msg = (uint32_t*)calloc((size_t)c->pk_msg_words, sizeof(uint32_t));
+    if (!post || !msg) {
+        free(post);
+        free(msg);
+        return -1;
+    }
+    const int8_t* src[2] = { llr_a, llr_b };
+    uint8_t* dst[2] = { out_a, out_b };
+    int it[2] = { 0, 0 }, active[2] = { 0, 0 }, started[2] = { 0, 0 };
+    if (nbits <= 0 || nbits > K)
+        nbits = K;
+    for (int t = 0;; t++) {
+        for (int s = 0; s < 2; s++) {
+            if (started[s] || t < (s ? delay : 0))
+                continue;
+            /* load the frame's LLRs into the slot's bytes */
+            for (int v = 0; v < N; v++) {
+                const int pos = v < K ? v : K + 360 * ((v - K) % q) + (v - K) / q;
+                post[pos] = (uint16_t)((post[pos] & (s ? 0x00ff : 0xff00)) | (uint16_t)(uint8_t)src[s][v] << (8 * s));
+            }
+            started[s] = active[s] = 1;
+            it[s] = 0;
+        }
+        uint32_t good = 3;
+        for (int i = 0; i < q; i++)
+            for (int j = 0; j < 360; j++)
+                good &= pk_check_any(c, i, j, post);
+        for (int s = 0; s < 2; s++) {
+            if (!active[s])
+                continue;
+            const int ok = (good >> s) & 1;
+            if (ok || it[s] >= max_iter) {
+                memset(dst[s], 0, (size_t)(nbits + 7) / 8);
+                for (int v = 0; v < nbits; v++)
+                    if ((int8_t)(post[v] >> (8 * s)) < 0)
+                        dst[s][v / 8] |= (uint8_t)(0x80 >> (v % 8));
+                it_out[s] = it[s];
+                ok_out[s] = ok;
+                active[s] = 0;
+            }
+        }
+        if (!active[0] && !active[1] && started[0] && started[1])
+            break;
+        const uint32_t se = pk_msg_sel(0, !active[0] || it[0] == 0, !active[1] || it[1] == 0);
+        const uint32_t so = pk_msg_sel(1, !active[0] || it[0] == 0, !active[1] || it[1] == 0);
+        if (active[0] || active[1])
+            for (int i = 0; i < q; i++)
+                if (pk_layer_any(c, i, post, msg, se, so)) {
+                    free(post);
+                    free(msg);
+                    return -1;
+                }
+        for (int s = 0; s < 2; s++)
+            it[s] += active[s];
+    }
+    (void)R;
+    free(post);
+    free(msg);
+    return 0;
 }
 
 /* The schedule's own invariants: no two nodes of one level of a layer

@@ -9,9 +9,11 @@ from helpers import ALL_RATES, Oracle, gen, ldpcgpu
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def dec():
-    d = ldpcgpu.Decoder(0)
+@pytest.fixture(scope="module", params=["auto", "single"])
+def dec(request):
+    """Both kernels: two frames per workgroup (packed 16-bit arithmetic) and
+    one frame per workgroup."""
+    d = ldpcgpu.Decoder(0, kernel=request.param)
     yield d
     d.close()
 
@@ -96,6 +98,35 @@ def test_mixed_batch_one_launch(dec):
         o_bits, o_it, o_ok = Oracle(short, rate, kldpc).frames(llr[lo:lo + n][None], 40)
         assert it[f] == o_it[0] and ok[f] == o_ok[0], (f, rate)
         assert (out[ro:ro + kb] == o_bits[0]).all(), (f, rate)
+
+
+def test_many_frames_refill(dec):
+    """More frames than workgroups of a few codes, iteration counts spread
+    wide: slots are refilled while their partner runs, workgroups move
+    between code segments; every frame must still equal its own decode."""
+    rng = np.random.default_rng(17)
+    codes = [(0, "1_2"), (0, "3_4"), (1, "3_5"), (1, "1_4"), (0, "9_10"), (1, "8_9")]
+    specs = []
+    for f in range(700):
+        short, rate = codes[rng.integers(len(codes))]
+        specs.append(dict(short_frame=short, rate=rate, esn0_db=esn0_for(short, rate) + rng.uniform(-0.2, 0.8),
+                          stream=10_000 + f))
+    llr, ref, layout = ldpcgpu.generate(specs, 123)
+    fr = ldpcgpu.frames_for(layout, 30)
+    out, it, ok = dec.decode_host(fr, llr, len(ref))
+    by = {}
+    for f, (code, n, kldpc, lo, ro) in enumerate(layout):
+        by.setdefault((int(n == 16200), specs[f]["rate"], kldpc), []).append(f)
+    for (short, rate, kldpc), fl in by.items():
+        n = 16200 if short else 64800
+        kb = (kldpc + 7) // 8
+        lls = np.stack([llr[layout[f][3]:layout[f][3] + n] for f in fl])
+        o_bits, o_it, o_ok = Oracle(short, rate, kldpc).frames(lls, 30)
+        for k, f in enumerate(fl):
+            ro = layout[f][4]
+            assert it[f] == o_it[k] and ok[f] == o_ok[k], (rate, f, it[f], o_it[k])
+            assert (out[ro:ro + kb] == o_bits[k]).all(), (rate, f)
+    assert len(set(it.tolist())) > 5  # the counts really spread
 
 
 def test_groups_and_singles_in_one_call(dec):
