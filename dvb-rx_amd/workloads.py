@@ -72,7 +72,8 @@ def single_code(nframes=64, pls=0x10, esn0_db=1.5, seed=2024, max_iter=50):
     return Batch(f"pls{pls:#04x}_x{nframes}_esn0_{esn0_db}dB", specs, seed, max_iter)
 
 
-def mixed_s2(nframes=8192, seed=7, max_iter=50, margin_db=0.3, short_share=0.25):
+def mixed_s2_specs(nframes=8192, seed=7, margin_db=0.3, short_share=0.25):
+    """Frame specs of the mixed DVB-S2 workload (no generation)."""
     rng = np.random.default_rng(seed)
     mc = s2_modcods()
     specs = []
@@ -83,4 +84,9 @@ def mixed_s2(nframes=8192, seed=7, max_iter=50, margin_db=0.3, short_share=0.25)
         cands = mc[(short, rate)]
         pls = int(cands[rng.integers(len(cands))])
         specs.append(dict(pls=pls, esn0_db=ldpcgpu.qef_db(pls) + margin_db, stream=f))
+    return specs
+
+
+def mixed_s2(nframes=8192, seed=7, max_iter=50, margin_db=0.3, short_share=0.25):
+    specs = mixed_s2_specs(nframes, seed, margin_db, short_share)
     return Batch(f"mixed_s2_x{nframes}", specs, seed, max_iter)

@@ -104,8 +104,9 @@ float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* ctx);
 
 /* Per-launch detail of the last ldpc_gpu_decode_batch: frames are decoded
  * by one persistent kernel launch per degree class (largest information
- * degree of a layer <= 8, 16, 28).  Fills up to max entries with the class
- * bound, the frames of the launch and its time in ms; returns the count. */
+ * degree of a layer <= 12 or <= 28) and frame size.  Fills up to max
+ * entries with the class bound, the frames of the launch and its time in
+ * ms; returns the count. */
 int ldpc_gpu_last_launches(ldpc_gpu_ctx* ctx, int* di, int* nframes, float* ms, int max);
 
 #ifdef __cplusplus

@@ -28,6 +28,20 @@ ALL_RATES = {
         "1_3_VLSNR", "3_5_t2"],
 }
 
+ESN0 = {"1_4": -2.0, "1_3": -1.1, "2_5": -0.2, "1_2": 1.0, "3_5": 2.2, "2_3": 3.0, "3_4": 4.0, "4_5": 4.6,
+        "5_6": 5.2, "8_9": 6.2, "9_10": 6.4}
+
+
+def esn0_for(short, rate):
+    """Es/N0 (dB) of the unit channel near a code's waterfall."""
+    if rate in ESN0:
+        return ESN0[rate] + (0.3 if short else 0.0)
+    num, den = [int(x) for x in rate.replace("_VLSNR", "").replace("_SF2", "").replace("_t2", "").split("_")]
+    r = num / den
+    # rough BPSK-per-component threshold of the unit channel
+    return float(10 * np.log10(2 ** (2 * r) - 1) + 0.4)
+
+
 _oracle = None
 
 

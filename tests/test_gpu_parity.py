@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import helpers
-from helpers import ALL_RATES, Oracle, gen, ldpcgpu
+from helpers import ALL_RATES, Oracle, esn0_for, gen, ldpcgpu
 
 pytestmark = pytest.mark.gpu
 
@@ -54,19 +54,6 @@ def test_config1_group(dec):
 
 # Es/N0 near each code's threshold, so that frames take many iterations,
 # some fail, and the dependent tails of the classic codes are exercised.
-ESN0 = {"1_4": -2.0, "1_3": -1.1, "2_5": -0.2, "1_2": 1.0, "3_5": 2.2, "2_3": 3.0, "3_4": 4.0, "4_5": 4.6,
-        "5_6": 5.2, "8_9": 6.2, "9_10": 6.4}
-
-
-def esn0_for(short, rate):
-    if rate in ESN0:
-        return ESN0[rate] + (0.3 if short else 0.0)
-    num, den = [int(x) for x in rate.replace("_VLSNR", "").replace("_SF2", "").replace("_t2", "").split("_")]
-    r = num / den
-    # rough BPSK-per-component threshold of the unit channel
-    return 10 * np.log10(2 ** (2 * r) - 1) + 0.4
-
-
 @pytest.mark.parametrize("short", [0, 1])
 def test_all_codes_per_frame(dec, short):
     """Every rate of both frame sizes, 3 noisy frames each, max 25."""

@@ -1,0 +1,78 @@
+"""CPU models of the two kernel schedules against the oracle.
+
+tests/model/libkmodel.so compiles the device code's node routines
+(dvb-rx_amd/csrc/ldpc_node.h, ldpc_pk.h) for the host and drives them in
+the kernels' order:
+
+* kmodel_decode: the three-phase layer schedule of the one-frame kernel
+  (dependency levels inside a layer, int8 messages);
+* kmodel_pk_pair: the packed two-frame kernel (two frames in the high and
+  low bytes of 16-bit lanes), the second frame entering its slot a few
+  iterations after the first, as when a slot is refilled.
+
+Both must reproduce the oracle's per-frame decode bit for bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import ALL_RATES, Oracle, esn0_for, gen, kmodel, kmodel_decode, ldpcgpu
+
+CODES = [(s, r) for s in (0, 1) for r in ALL_RATES[s]]
+
+
+@pytest.mark.parametrize("short,rate", CODES)
+def test_layer_schedule_model(short, rate):
+    fec, llr, _ = gen(short, rate, esn0_for(short, rate), 1, 900 + ALL_RATES[short].index(rate), stream0=short)
+    code = ldpcgpu.code_lookup(short, rate)
+    mi = 8
+    bits, it, ok = Oracle(short, rate, fec.kldpc).frames(llr, mi)
+    out, mit, mok = kmodel_decode(code, llr[0], mi, fec.kldpc)
+    assert (mit, mok) == (int(it[0]), int(ok[0]))
+    assert (out == bits[0]).all()
+
+
+def pk_pair(code, a, b, delay, max_iter, nbits):
+    L = kmodel()
+    L.kmodel_pk_pair.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p]
+    a = np.ascontiguousarray(a, np.int8)
+    b = np.ascontiguousarray(b, np.int8)
+    kb = (nbits + 7) // 8
+    oa, ob = np.zeros(kb, np.uint8), np.zeros(kb, np.uint8)
+    it = np.zeros(2, np.int32)
+    ok = np.zeros(2, np.int32)
+    rc = L.kmodel_pk_pair(code, a.ctypes.data, b.ctypes.data, delay, max_iter, nbits, oa.ctypes.data,
+                          ob.ctypes.data, it.ctypes.data, ok.ctypes.data)
+    assert rc == 0
+    return oa, ob, it, ok
+
+
+PK_CASES = [(0, "1_2", 0), (0, "1_2", 3), (0, "3_5", 5), (0, "9_10", 2), (1, "1_4", 4), (1, "8_9", 1),
+            (0, "2_9_VLSNR", 2), (1, "11_45", 3), (0, "135_180", 1), (0, "2_3_t2", 2), (1, "3_5_t2", 0)]
+
+
+@pytest.mark.parametrize("short,rate,delay", PK_CASES)
+def test_packed_pair_model(short, rate, delay):
+    fec, llr, _ = gen(short, rate, esn0_for(short, rate), 2, 40 + delay, stream0=7)
+    code = ldpcgpu.code_lookup(short, rate)
+    mi = 10
+    bits, it, ok = Oracle(short, rate, fec.kldpc).frames(llr, mi)
+    oa, ob, pit, pok = pk_pair(code, llr[0], llr[1], delay, mi, fec.kldpc)
+    assert list(pit) == list(it) and list(pok) == list(ok)
+    assert (oa == bits[0]).all() and (ob == bits[1]).all()
+
+
+def test_packed_pair_extremes():
+    """Saturated and zero LLRs in the two bytes of one lane: +127 next to
+    -128 and 0 (the unpacking must not carry between the slots)."""
+    fec = ldpcgpu.fec_lookup(1, "1_2")
+    code = ldpcgpu.code_lookup(1, "1_2")
+    rng = np.random.default_rng(2)
+    a = rng.choice([-128, -127, 127, 0, 1, -1], fec.nldpc).astype(np.int8)
+    b = rng.choice([-128, 127, 0], fec.nldpc).astype(np.int8)
+    bits, it, ok = Oracle(1, "1_2", fec.kldpc).frames(np.stack([a, b]), 4)
+    oa, ob, pit, pok = pk_pair(code, a, b, 0, 4, fec.kldpc)
+    assert list(pit) == list(it) and list(pok) == list(ok)
+    assert (oa == bits[0]).all() and (ob == bits[1]).all()
