@@ -10,19 +10,27 @@
  *   mn0  = max(min(min0, 127) - 1, 0), mn1 likewise       (offset 1)
  *   o_e  = (s ^ sign x_e) ? -mag : mag, mag = a_e == min0 ? mn1 : mn0
  *   L_e <- sat8(x_e + o_e),  m_e <- clamp(o_e, -32, 31)
- * Messages are kept per node as (mn0, mn1, one argmin edge, sign bits),
- * from which every clamped m_e is recovered exactly (the compressed store
- * of ldpc.c:376-462 keeps the same information).
+ * Messages are stored clamped to [-32, 31] as the reference stores them
+ * (one int8 per edge; the compressed store of ldpc.c:376-462 holds the
+ * same values).
  *
- * Work layout: one workgroup owns one frame at a time (a persistent grid
- * takes frames from an atomic ticket).  The frame's N posteriors live in
- * LDS: information bits 0..K-1, then the R parity bits in layer order
- * (parity of check q*j+i at K + 360*i + j, ldpc.c:624-626).  Thread j
- * processes node j of every layer, so the message of node (i, j) is only
- * ever touched by thread j and lives in global memory (L2 / MALL resident)
- * at [i*360 + j], coalesced across the wavefronts.  Within a layer the
- * nodes run level by level (ldpc_codes.h), with a workgroup barrier after
- * each level; this reproduces the reference's sequential node order.
+ * Two kernels, both persistent grids over a frame table:
+ *
+ * ldpc_layered_kernel: one frame per workgroup.  The frame's N posteriors
+ * live in LDS: information bits 0..K-1, then the R parity bits in layer
+ * order (parity of check q*j+i at K + 360*i + j, ldpc.c:624-626).  Thread
+ * j processes node j of every layer; the messages of node (i, j) are only
+ * touched by thread j and live in global memory at rows of 360 words
+ * (4 edges per word), coalesced across the wavefronts.  Within a layer the
+ * nodes sharing a variable run in dependency levels (ldpc_codes.h) so the
+ * result equals the reference's sequential node order.  Serves batches
+ * with a shared early stop (groups) and resumable passes.
+ *
+ * ldpc_pk_kernel: two frames per workgroup in the two bytes of 16-bit
+ * lanes (ldpc_pk.h: value y held as y*256, so 16-bit saturating packed
+ * arithmetic is the reference's int8 saturation), each slot stopping on
+ * its own syndrome and refilled from the frames of the same code while
+ * the other slot runs.  Frames with their own early stop take this path.
  */
 #include <hip/hip_runtime.h>
 
