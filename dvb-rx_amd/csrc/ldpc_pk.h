@@ -174,6 +174,29 @@ LG_PK_HD PkOut<DEG> pk_mags(const PkNode<DEG>& n)
     return PkOut<DEG>{ mn1, mn0 - mn1 };
 }
 
+/* Learned magnitude of one minimum, per lane: clamp(((min(m, 127) * S) >>
+ * 4) - B, 0, 127) in units of 256, S (0..255) and B (-128..127) in the
+ * lanes of ps / pb.  S = 16, B = 1 gives pk_mags' values. */
+LG_PK_HD lg_pk pk_nms_mag(lg_pk m, lg_pk ps, lg_pk pb)
+{
+    const lg_pku v = pku(pk_minu(m, pk_dup(0x7F00))) >> (unsigned short)8;
+    const lg_pk p = pks((lg_pku)(v * pku(ps)) >> (unsigned short)4);
+    return pk_min(pk_max(p - pb, pk_dup(0)), pk_dup(127)) << (short)8;
+}
+
+template <int DEG, bool NMS>
+LG_PK_HD PkOut<DEG> pk_mags_p(const PkNode<DEG>& n, lg_pk ps, lg_pk pb)
+{
+    if constexpr (NMS) {
+        const lg_pk mn0 = pk_nms_mag(n.min0, ps, pb), mn1 = pk_nms_mag(n.min1, ps, pb);
+        return PkOut<DEG>{ mn1, mn0 - mn1 };
+    } else {
+        (void)ps;
+        (void)pb;
+        return pk_mags(n);
+    }
+}
+
 /* o_e: |o| = mn1 if |x_e| is the smallest input, else mn0; sign = product
  * of the other signs.  a >= min0 always, so a - min0 is 0 exactly for the
  * smallest input.  Where a 0xff low byte makes |x_e| differ from min0 by
@@ -203,10 +226,11 @@ LG_PK_HD lg_pk pk_out(const PkNode<DEG>& n, const PkOut<DEG>& m, int e)
 
 LG_PK_HD uint16_t pk_post_bytes(lg_pk y) { return (uint16_t)lg_perm(0, pk_bits(y), PK_PACK_POST); }
 
-template <int DEG>
-LG_PK_HD void pk_write_succ(const PkNode<DEG>& n, uint16_t* post, uint32_t succ)
+template <int DEG, bool NMS = false>
+LG_PK_HD void pk_write_succ(const PkNode<DEG>& n, uint16_t* post, uint32_t succ, lg_pk ps = lg_pk{ 16, 16 },
+                            lg_pk pb = lg_pk{ 1, 1 })
 {
-    const PkOut<DEG> m = pk_mags(n);
+    const PkOut<DEG> m = pk_mags_p<DEG, NMS>(n, ps, pb);
 #pragma unroll
     for (int e = 0; e < DEG; e++)
         if ((succ >> e) & 1)
@@ -215,10 +239,11 @@ LG_PK_HD void pk_write_succ(const PkNode<DEG>& n, uint16_t* post, uint32_t succ)
 
 /* final posteriors (bits this node touches last) and the new messages,
  * clamped to [-32, 31] (ldpc.c:343) */
-template <int DEG, bool TAIL>
-LG_PK_HD void pk_phase_c(const PkNode<DEG>& n, uint16_t* post, uint32_t succ, uint32_t* mw_out)
+template <int DEG, bool TAIL, bool NMS = false>
+LG_PK_HD void pk_phase_c(const PkNode<DEG>& n, uint16_t* post, uint32_t succ, uint32_t* mw_out,
+                         lg_pk ps = lg_pk{ 16, 16 }, lg_pk pb = lg_pk{ 1, 1 })
 {
-    const PkOut<DEG> m = pk_mags(n);
+    const PkOut<DEG> m = pk_mags_p<DEG, NMS>(n, ps, pb);
     lg_pk mo[DEG + 2];
 #pragma unroll
     for (int e = 0; e < DEG + 2; e++) {

@@ -85,6 +85,18 @@ def gpu_lib():
         L.drx_ldpc_new.restype = vp
         L.drx_ldpc_free.argtypes = [vp]
         L.drx_ldpc_decode.argtypes = [vp, vp, vp, i32, i32]
+        f32 = C.c_float
+        L.ldpc_nms_trainer_create.argtypes = [i32, i32, i32, i32, C.POINTER(vp)]
+        L.ldpc_nms_trainer_destroy.argtypes = [vp]
+        L.ldpc_nms_trainer_stream.argtypes = [vp]
+        L.ldpc_nms_trainer_stream.restype = vp
+        L.ldpc_nms_trainer_error.argtypes = [vp]
+        L.ldpc_nms_trainer_error.restype = C.c_char_p
+        L.ldpc_nms_trainer_synchronize.argtypes = [vp]
+        L.ldpc_nms_train_step.argtypes = [vp, vp, vp, vp, i32, vp, vp, f32, vp, vp, vp]
+        L.ldpc_nms_adam.argtypes = [vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, i32]
+        L.ldpc_nms_quantize.argtypes = [vp, vp, i32, vp, vp]
+        L.ldpc_gpu_channel.argtypes = [i32, i32, vp, vp, i32, vp, C.c_uint64, vp, vp]
         _gpu = L
     return _gpu
 
@@ -231,6 +243,11 @@ class Decoder:
     def __exit__(self, *a):
         self.close()
 
+    @property
+    def stream(self):
+        """The context's hipStream_t (non-blocking: order other work on it)."""
+        return self.L.ldpc_gpu_stream(self.ctx)
+
     def error(self):
         return self.L.ldpc_gpu_last_error(self.ctx).decode()
 
@@ -292,6 +309,62 @@ class Decoder:
             "decode_host",
         )
         return out, iters, synd
+
+
+NMS_DEGS = 33
+
+
+def nms_quantize(alpha, beta):
+    """Integer table (S, B) of float learned parameters (T x NMS_DEGS)."""
+    alpha = np.ascontiguousarray(alpha, np.float32)
+    beta = np.ascontiguousarray(beta, np.float32)
+    assert alpha.shape == beta.shape and alpha.shape[1] == NMS_DEGS
+    S = np.zeros(alpha.shape, np.uint8)
+    B = np.zeros(alpha.shape, np.int8)
+    gpu_lib().ldpc_nms_quantize(alpha.ctypes.data, beta.ctypes.data, alpha.shape[0], S.ctypes.data, B.ctypes.data)
+    return S, B
+
+
+class NmsTrainer:
+    """Learned offset min-sum trainer for one code (include/ldpc_nms.h)."""
+
+    def __init__(self, code, T, max_frames, device=0):
+        self.L = gpu_lib()
+        self.t = C.c_void_p()
+        rc = self.L.ldpc_nms_trainer_create(device, code, T, max_frames, C.byref(self.t))
+        if rc:
+            raise RuntimeError(f"ldpc_nms_trainer_create: error {rc}")
+        self.code, self.T, self.max_frames = code, T, max_frames
+
+    def close(self):
+        if self.t:
+            self.L.ldpc_nms_trainer_destroy(self.t)
+            self.t = C.c_void_p()
+
+    def _check(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what}: {self.L.ldpc_nms_trainer_error(self.t).decode()}")
+
+    @property
+    def stream(self):
+        return self.L.ldpc_nms_trainer_stream(self.t)
+
+    def synchronize(self):
+        self._check(self.L.ldpc_nms_trainer_synchronize(self.t), "synchronize")
+
+    def step(self, d_llr, d_cw, d_row, nframes, d_params, d_w, temp, d_loss, d_grad, d_post=None):
+        self._check(self.L.ldpc_nms_train_step(self.t, d_llr, d_cw, d_row, nframes, d_params, d_w, temp, d_loss,
+                                               d_grad, d_post), "train_step")
+
+    def adam(self, d_param, d_grad, d_m, d_v, n, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._check(self.L.ldpc_nms_adam(self.t, d_param, d_grad, d_m, d_v, n, lr, beta1, beta2, eps, step), "adam")
+
+
+def channel(pls, n, d_cw, d_row, nframes, d_esn0_db, seed, d_llr, stream=None):
+    """GPU channel: codeword rows -> MODCOD -> AWGN -> int8 max-log LLRs."""
+    rc = gpu_lib().ldpc_gpu_channel(pls, n, d_cw, d_row, nframes, d_esn0_db, seed, d_llr, stream)
+    if rc:
+        raise RuntimeError(f"ldpc_gpu_channel: error {rc}")
 
 
 class DrxLdpc:

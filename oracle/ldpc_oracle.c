@@ -6,6 +6,7 @@
  */
 #include "ldpc_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -296,4 +297,341 @@ int oracle_ldpc_check(oracle_ldpc* d, const int8_t* llr)
         return 0;
     load(d, llr, 1);
     return !bad(d, 1);
+}
+
+void oracle_ldpc_run(oracle_ldpc* d, const int8_t* llr, int nframes, int T, int8_t* post)
+{
+    for (int f = 0; f < nframes; f++) {
+        if (reserve(d, 1))
+            return;
+        load(d, llr + (size_t)f * d->N, 1);
+        for (int t = 0; t < T; t++)
+            update(d, 1, t == 0);
+        int8_t* p = post + (size_t)f * d->N;
+        memcpy(p, d->data, (size_t)d->K);
+        for (int i = 0; i < d->q; i++)
+            for (int j = 0; j < d->M; j++)
+                p[d->K + d->q * j + i] = d->pty[d->M * i + j];
+    }
+}
+
+/* ==== learned (neural) offset min-sum ==================================== */
+
+int oracle_nms_scale_q(float alpha)
+{
+    const float r = rintf(alpha * 16.0f);
+    return r < 0.0f ? 0 : r > 255.0f ? 255 : (int)r;
+}
+
+int oracle_nms_offset_q(float beta)
+{
+    const float r = rintf(beta);
+    return r < -128.0f ? -128 : r > 127.0f ? 127 : (int)r;
+}
+
+/* learned magnitude; S = 16, B = 1 gives subsu(minu(m, 127), 1) */
+static inline int8_t nms_mag(uint8_t m, int S, int B)
+{
+    const int v = (((int)minu(m, 127) * S) >> 4) - B;
+    return (int8_t)(v < 0 ? 0 : v > 127 ? 127 : v);
+}
+
+/* check_node_lane with the learned magnitudes */
+static inline void check_node_lane_nms(int8_t* const* v, int8_t* bl, int deg, int first, int S, int B)
+{
+    int8_t inp[MAX_DEG];
+    uint8_t min0 = 255, min1 = 255;
+    int8_t signs = 0;
+    for (int c = 0; c < deg; c++) {
+        const int8_t x = first ? *v[c] : qsub(*v[c], bl[c]);
+        const uint8_t a = absu(x);
+        inp[c] = x;
+        min1 = minu(min1, maxu(min0, a));
+        min0 = minu(min0, a);
+        signs = (int8_t)(signs ^ x);
+    }
+    /* the map m -> mn is monotonic, so the smaller magnitude still goes
+     * to every edge but the smallest input's */
+    const int8_t mn0 = nms_mag(min0, S, B), mn1 = nms_mag(min1, S, B);
+    for (int c = 0; c < deg; c++) {
+        const int8_t x = inp[c];
+        const int8_t eq = absu(x) == min0 ? (int8_t)-1 : 0;
+        const int8_t o = vsign(smax(mn0, (int8_t)(eq & mn1)), (int8_t)((signs ^ x) | 127));
+        *v[c] = qadd(x, o);
+        bl[c] = smin(smax(o, -32), 31);
+    }
+}
+
+void oracle_nms_check_node(int deg, const int8_t* L, const int8_t* m, int first, int S, int B,
+                           int8_t* L_out, int8_t* m_out)
+{
+    int8_t* v[MAX_DEG];
+    for (int c = 0; c < deg && c < MAX_DEG; c++) {
+        L_out[c] = L[c];
+        m_out[c] = m[c];
+        v[c] = &L_out[c];
+    }
+    check_node_lane_nms(v, m_out, deg, first, S, B);
+}
+
+/* one learned iteration of one frame (lanes = 1), parameter row t */
+static void update_nms(oracle_ldpc* d, int first, const uint8_t* S, const int8_t* B, int t)
+{
+    int8_t* bl = d->bnl;
+    int8_t* v[MAX_DEG];
+    for (int i = 0; i < d->q; i++) {
+        const int cnt = d->cnc[i];
+        for (int j = 0; j < d->M; j++) {
+            const int cn = d->M * i + j;
+            const int pp = prev_parity(d, i, j);
+            const int deg = cnt + 1 + (pp >= 0);
+            const uint16_t* p = d->pos + (size_t)d->CNL * cn;
+            for (int c = 0; c < cnt; c++)
+                v[c] = &d->data[p[c]];
+            v[cnt] = &d->pty[cn];
+            if (pp >= 0)
+                v[cnt + 1] = &d->pty[pp];
+            check_node_lane_nms(v, bl, deg, first, S[t * ORACLE_NMS_DEGS + deg], B[t * ORACLE_NMS_DEGS + deg]);
+            bl += deg;
+        }
+    }
+}
+
+void oracle_nms_decode_frames(oracle_ldpc* d, const uint8_t* S, const int8_t* B, int T, const int8_t* llr,
+                              uint8_t* out, int nframes, int learned_max, int fallback_max, int32_t* iters,
+                              uint8_t* ok, uint8_t* fell_back)
+{
+    const int nbytes = (d->kldpc + 7) / 8;
+    for (int f = 0; f < nframes; f++) {
+        const int8_t* fl = llr + (size_t)f * d->N;
+        uint8_t* fo = out + (size_t)f * nbytes;
+        if (reserve(d, 1))
+            return;
+        load(d, fl, 1);
+        int it = 0, good;
+        while (!(good = !bad(d, 1)) && it < learned_max) {
+            update_nms(d, it == 0, S, B, it < T ? it : T - 1);
+            it++;
+        }
+        int fb = 0;
+        if (!good && fallback_max > 0) {
+            const int r = oracle_ldpc_decode(d, fl, fo, 1, fallback_max);
+            it += d->last_it;
+            good = r >= 0;
+            fb = 1;
+        } else {
+            store(d, fo, 1);
+        }
+        iters[f] = it;
+        ok[f] = (uint8_t)good;
+        if (fell_back)
+            fell_back[f] = (uint8_t)fb;
+    }
+}
+
+/* ---- float model and its gradient ---------------------------------------
+ *
+ * Variables in the decoder's order: information bits 0..K-1, then parity
+ * in layer order (K + M*i + j holds codeword bit K + q*j + i).  Each node
+ * visit saves raw_e = L_v - m_e (before saturation); the backward pass
+ * recomputes the node from it, in the exact reverse of the forward order. */
+
+static inline float clampf(float v, float lo, float hi) { return v < lo ? lo : v > hi ? hi : v; }
+
+struct nms_node {
+    int deg, e0, idx[MAX_DEG];
+};
+
+/* node (i, j): its variables and the offset of its first message */
+static void nms_node_at(const oracle_ldpc* d, int i, int j, int e0, struct nms_node* n)
+{
+    const int cnt = d->cnc[i];
+    const int cn = d->M * i + j;
+    const int pp = prev_parity(d, i, j);
+    const uint16_t* p = d->pos + (size_t)d->CNL * cn;
+    n->deg = cnt + 1 + (pp >= 0);
+    n->e0 = e0;
+    for (int c = 0; c < cnt; c++)
+        n->idx[c] = p[c];
+    n->idx[cnt] = d->K + cn;
+    if (pp >= 0)
+        n->idx[cnt + 1] = d->K + pp;
+}
+
+/* the node's arithmetic from its saturated inputs x: magnitudes, signs,
+ * outputs o; i0/i1: the smallest and second smallest |x| (first index on
+ * ties), mu[c]: the smallest |x| over the other edges */
+struct nms_eval {
+    int i0, i1;
+    float mu[MAX_DEG], pre[MAX_DEG], o[MAX_DEG];
+    int neg[MAX_DEG];
+};
+
+static void nms_eval_node(int mode, int deg, const float* x, float alpha, float beta, struct nms_eval* r)
+{
+    float m0 = INFINITY, m1 = INFINITY;
+    int i0 = -1, i1 = -1, nneg = 0;
+    for (int c = 0; c < deg; c++) {
+        const float a = fabsf(x[c]);
+        if (a < m0) {
+            m1 = m0;
+            i1 = i0;
+            m0 = a;
+            i0 = c;
+        } else if (a < m1) {
+            m1 = a;
+            i1 = c;
+        }
+        nneg += x[c] < 0.0f;
+    }
+    r->i0 = i0;
+    r->i1 = i1;
+    const int S = oracle_nms_scale_q(alpha), B = oracle_nms_offset_q(beta);
+    for (int c = 0; c < deg; c++) {
+        const float mu = c == i0 ? m1 : m0;
+        const float mup = fminf(mu, 127.0f);
+        const float pre = alpha * mup - beta;
+        float mag;
+        if (mode == 0)
+            mag = clampf(pre, 0.0f, 127.0f);
+        else
+            mag = (float)nms_mag((uint8_t)mu, S, B); /* integer inputs: mu in 0..128 */
+        r->mu[c] = mu;
+        r->pre[c] = pre;
+        r->neg[c] = (nneg - (x[c] < 0.0f)) & 1;
+        r->o[c] = r->neg[c] ? -mag : mag;
+    }
+}
+
+double oracle_nms_train(oracle_ldpc* d, int mode, const float* llr, const uint8_t* bits, int nframes, int T,
+                        const float* alpha, const float* beta, const float* w, float temp, float* galpha,
+                        float* gbeta, float* post)
+{
+    const int N = d->N, K = d->K, LT = d->LT;
+    const size_t np = (size_t)T * ORACLE_NMS_DEGS;
+    float* L = malloc(sizeof(float) * N);
+    float* m = malloc(sizeof(float) * LT);
+    float* raw = malloc(sizeof(float) * (size_t)T * LT);
+    float* Ls = malloc(sizeof(float) * (size_t)T * N);
+    double* gL = malloc(sizeof(double) * N);
+    double* gm = malloc(sizeof(double) * LT);
+    double* ga = calloc(np, sizeof(double));
+    double* gb = calloc(np, sizeof(double));
+    uint8_t* b = malloc((size_t)N);
+    int* cw = malloc(sizeof(int) * N); /* variable -> codeword position */
+    double loss = 0;
+    if (!L || !m || !raw || !Ls || !gL || !gm || !ga || !gb || !b || !cw)
+        goto out;
+    for (int v = 0; v < N; v++)
+        cw[v] = v < K ? v : K + d->q * ((v - K) % d->M) + (v - K) / d->M;
+    const double inv = 1.0 / ((double)N * nframes);
+    for (int f = 0; f < nframes; f++) {
+        const float* fl = llr + (size_t)f * N;
+        for (int v = 0; v < N; v++) {
+            L[v] = fl[cw[v]];
+            b[v] = bits[(size_t)f * N + cw[v]];
+        }
+        /* forward */
+        struct nms_node n;
+        struct nms_eval r;
+        float x[MAX_DEG];
+        for (int t = 0; t < T; t++) {
+            int e = 0;
+            for (int i = 0; i < d->q; i++)
+                for (int j = 0; j < d->M; j++) {
+                    nms_node_at(d, i, j, e, &n);
+                    const float al = alpha[t * ORACLE_NMS_DEGS + n.deg], be = beta[t * ORACLE_NMS_DEGS + n.deg];
+                    for (int c = 0; c < n.deg; c++) {
+                        const float rw = t ? L[n.idx[c]] - m[e + c] : L[n.idx[c]];
+                        raw[(size_t)t * LT + e + c] = rw;
+                        x[c] = clampf(rw, -128.0f, 127.0f);
+                    }
+                    nms_eval_node(mode, n.deg, x, al, be, &r);
+                    for (int c = 0; c < n.deg; c++) {
+                        L[n.idx[c]] = clampf(x[c] + r.o[c], -128.0f, 127.0f);
+                        m[e + c] = clampf(r.o[c], -32.0f, 31.0f);
+                    }
+                    e += n.deg;
+                }
+            memcpy(Ls + (size_t)t * N, L, sizeof(float) * N);
+            if (w[t] != 0.0f)
+                for (int v = 0; v < N; v++) {
+                    const double z = -(double)temp * L[v];
+                    const double sp = (z > 0 ? z : 0) + log1p(exp(-fabs(z)));
+                    loss += w[t] * (sp - b[v] * z) * inv;
+                }
+        }
+        if (post)
+            for (int v = 0; v < N; v++)
+                post[(size_t)f * N + cw[v]] = L[v];
+        if (!galpha && !gbeta)
+            continue;
+        /* backward */
+        for (int v = 0; v < N; v++)
+            gL[v] = 0;
+        for (int e = 0; e < LT; e++)
+            gm[e] = 0;
+        double gx[MAX_DEG];
+        for (int t = T - 1; t >= 0; t--) {
+            if (w[t] != 0.0f)
+                for (int v = 0; v < N; v++) {
+                    const double s = 1.0 / (1.0 + exp((double)temp * Ls[(size_t)t * N + v]));
+                    gL[v] += w[t] * temp * (b[v] - s) * inv;
+                }
+            int e = LT;
+            for (int i = d->q - 1; i >= 0; i--)
+                for (int j = d->M - 1; j >= 0; j--) {
+                    const int cnt = d->cnc[i];
+                    const int deg = cnt + 1 + (prev_parity(d, i, j) >= 0);
+                    e -= deg;
+                    nms_node_at(d, i, j, e, &n);
+                    const float al = alpha[t * ORACLE_NMS_DEGS + deg], be = beta[t * ORACLE_NMS_DEGS + deg];
+                    const float* rw = raw + (size_t)t * LT + e;
+                    for (int c = 0; c < deg; c++)
+                        x[c] = clampf(rw[c], -128.0f, 127.0f);
+                    nms_eval_node(mode, deg, x, al, be, &r);
+                    for (int c = 0; c < deg; c++)
+                        gx[c] = 0;
+                    for (int c = 0; c < deg; c++) {
+                        const float Ln = x[c] + r.o[c];
+                        const double gLn = Ln >= -128.0f && Ln <= 127.0f ? gL[n.idx[c]] : 0.0;
+                        const double go = gLn + (r.o[c] >= -32.0f && r.o[c] <= 31.0f ? gm[e + c] : 0.0);
+                        gx[c] += gLn;
+                        if (r.pre[c] > 0.0f && r.pre[c] < 127.0f) {
+                            const double dmag = r.neg[c] ? -go : go;
+                            const float mup = fminf(r.mu[c], 127.0f);
+                            ga[t * ORACLE_NMS_DEGS + deg] += dmag * mup;
+                            gb[t * ORACLE_NMS_DEGS + deg] -= dmag;
+                            if (r.mu[c] < 127.0f) {
+                                const int k = c == r.i0 ? r.i1 : r.i0;
+                                gx[k] += dmag * al * (x[k] < 0.0f ? -1.0 : 1.0);
+                            }
+                        }
+                    }
+                    for (int c = 0; c < deg; c++) {
+                        const double g = rw[c] >= -128.0f && rw[c] <= 127.0f ? gx[c] : 0.0;
+                        gL[n.idx[c]] = g;
+                        gm[e + c] = t ? -g : 0.0;
+                    }
+                }
+        }
+    }
+    if (galpha)
+        for (size_t k = 0; k < np; k++)
+            galpha[k] = (float)ga[k];
+    if (gbeta)
+        for (size_t k = 0; k < np; k++)
+            gbeta[k] = (float)gb[k];
+out:
+    free(L);
+    free(m);
+    free(raw);
+    free(Ls);
+    free(gL);
+    free(gm);
+    free(ga);
+    free(gb);
+    free(b);
+    free(cw);
+    return loss;
 }

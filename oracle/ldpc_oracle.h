@@ -43,4 +43,53 @@ int oracle_ldpc_check(oracle_ldpc* d, const int8_t* llr);
 void oracle_check_node(int deg, const int8_t* L, const int8_t* m, int first, int8_t* L_out,
                        int8_t* m_out);
 
+/* nframes x T plain updates of the reference, no early stop; posteriors in
+ * codeword order (N per frame). */
+void oracle_ldpc_run(oracle_ldpc* d, const int8_t* llr, int nframes, int T, int8_t* post);
+
+/* ---- learned (neural) offset min-sum ------------------------------------
+ *
+ * Parameters are per iteration t and per check degree dc (the node's edge
+ * count, row index 0..ORACLE_NMS_DEGS-1); iterations past the table's T
+ * rows use row T-1.  Integer form: the check node's magnitudes become
+ *     mn = clamp(((min(m, 127) * S[t][dc]) >> 4) - B[t][dc], 0, 127)
+ * in place of max(min(m, 127) - 1, 0); S = 16, B = 1 is the reference.
+ * Float form (training): mag = clamp(alpha * min(mu, 127) - beta, 0, 127),
+ * everything else as the integer decoder (saturation to [-128, 127],
+ * stored messages clamped to [-32, 31]).
+ */
+#define ORACLE_NMS_DEGS 33
+
+/* Quantisation of a float (alpha, beta) pair: S = clamp(rint(16 alpha), 0,
+ * 255), B = clamp(rint(beta), -128, 127). */
+int oracle_nms_scale_q(float alpha);
+int oracle_nms_offset_q(float beta);
+
+/* One learned check node (integer form), as oracle_check_node. */
+void oracle_nms_check_node(int deg, const int8_t* L, const int8_t* m, int first, int S, int B,
+                           int8_t* L_out, int8_t* m_out);
+
+/* Per-frame learned decode: up to learned_max learned iterations with the
+ * syndrome early stop; a frame still failing is decoded again from its
+ * channel LLRs by the plain decoder for up to fallback_max iterations
+ * (0: no fallback).  iters[f] = learned + plain iterations, ok[f] the
+ * final syndrome, fell_back[f] (may be NULL) 1 if the fallback ran. */
+void oracle_nms_decode_frames(oracle_ldpc* d, const uint8_t* S, const int8_t* B, int T, const int8_t* llr,
+                              uint8_t* out, int nframes, int learned_max, int fallback_max, int32_t* iters,
+                              uint8_t* ok, uint8_t* fell_back);
+
+/* Training step of the float model over T unrolled iterations (no early
+ * stop) on nframes frames: llr (nframes x N, float), bits (nframes x N
+ * codeword bits 0/1), alpha/beta (T x ORACLE_NMS_DEGS).  Loss: sum over t
+ * of w[t] times the mean over frames and bits of the binary cross-entropy
+ * of the posteriors after iteration t, read as logits temp * L (L > 0:
+ * bit 0).  mode 0: float magnitudes; mode 1: the integer forward with
+ * S, B quantised from alpha, beta (posteriors equal oracle_nms_decode's)
+ * and the float model's gradient (straight-through).  Returns the loss;
+ * galpha/gbeta (T x ORACLE_NMS_DEGS, may be NULL) receive its gradient,
+ * post (nframes x N, may be NULL) the final posteriors in codeword order. */
+double oracle_nms_train(oracle_ldpc* d, int mode, const float* llr, const uint8_t* bits, int nframes, int T,
+                        const float* alpha, const float* beta, const float* w, float temp, float* galpha,
+                        float* gbeta, float* post);
+
 #endif

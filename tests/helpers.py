@@ -43,6 +43,7 @@ def esn0_for(short, rate):
 
 
 _oracle = None
+NMS_DEGS = 33
 
 
 def oracle():
@@ -57,6 +58,13 @@ def oracle():
         L.oracle_ldpc_decode_frames.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp]
         L.oracle_ldpc_check.argtypes = [vp, vp]
         L.oracle_check_node.argtypes = [C.c_int, vp, vp, C.c_int, vp, vp]
+        L.oracle_ldpc_run.argtypes = [vp, vp, C.c_int, C.c_int, vp]
+        L.oracle_nms_scale_q.argtypes = [C.c_float]
+        L.oracle_nms_offset_q.argtypes = [C.c_float]
+        L.oracle_nms_check_node.argtypes = [C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp]
+        L.oracle_nms_decode_frames.argtypes = [vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
+        L.oracle_nms_train.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp, C.c_float, vp, vp, vp]
+        L.oracle_nms_train.restype = C.c_double
         _oracle = L
     return _oracle
 
@@ -88,6 +96,48 @@ class Oracle:
         out = np.zeros((n, self.kb), np.uint8)
         r = self.L.oracle_ldpc_decode(self.h, llr.ctypes.data, out.ctypes.data, n, max_iter)
         return r, out
+
+    def run(self, llr, T):
+        """T plain updates, no early stop: posteriors (n, N) in codeword order."""
+        llr = np.ascontiguousarray(llr, np.int8)
+        n = llr.shape[0]
+        post = np.zeros_like(llr)
+        self.L.oracle_ldpc_run(self.h, llr.ctypes.data, n, T, post.ctypes.data)
+        return post
+
+    def nms_frames(self, S, B, llr, learned_max, fallback_max):
+        """Learned decode per frame: (bits, iters, ok, fell_back)."""
+        S = np.ascontiguousarray(S, np.uint8)
+        B = np.ascontiguousarray(B, np.int8)
+        assert S.shape == B.shape and S.shape[1] == NMS_DEGS
+        llr = np.ascontiguousarray(llr, np.int8)
+        n = llr.shape[0]
+        out = np.zeros((n, self.kb), np.uint8)
+        it = np.zeros(n, np.int32)
+        ok = np.zeros(n, np.uint8)
+        fb = np.zeros(n, np.uint8)
+        self.L.oracle_nms_decode_frames(self.h, S.ctypes.data, B.ctypes.data, S.shape[0], llr.ctypes.data,
+                                        out.ctypes.data, n, learned_max, fallback_max, it.ctypes.data,
+                                        ok.ctypes.data, fb.ctypes.data)
+        return out, it, ok, fb
+
+    def nms_train(self, mode, llr, bits, alpha, beta, w, temp=0.5, grad=True):
+        """Float model over T = alpha.shape[0] iterations: (loss, galpha,
+        gbeta, final posteriors)."""
+        llr = np.ascontiguousarray(llr, np.float32)
+        bits = np.ascontiguousarray(bits, np.uint8)
+        alpha = np.ascontiguousarray(alpha, np.float32)
+        beta = np.ascontiguousarray(beta, np.float32)
+        w = np.ascontiguousarray(w, np.float32)
+        n = llr.shape[0]
+        T = alpha.shape[0]
+        ga = np.zeros_like(alpha)
+        gb = np.zeros_like(beta)
+        post = np.zeros_like(llr)
+        loss = self.L.oracle_nms_train(self.h, mode, llr.ctypes.data, bits.ctypes.data, n, T, alpha.ctypes.data,
+                                       beta.ctypes.data, w.ctypes.data, temp, ga.ctypes.data if grad else None,
+                                       gb.ctypes.data if grad else None, post.ctypes.data)
+        return loss, ga, gb, post
 
     def check(self, llr):
         llr = np.ascontiguousarray(llr, np.int8)
@@ -191,3 +241,16 @@ def sha256(a):
     import hashlib
 
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def codewords(short_frame, rate, ref):
+    """LDPC codewords (n, N) of unpacked bits from the BCH codewords gen()
+    returns (kldpc bits packed; kldpc = K for every rate but the
+    shortened VLSNR codes, which are not used here)."""
+    L = ldpcgpu.sim_lib()
+    fec = ldpcgpu.fec_lookup(short_frame, rate)
+    out = np.zeros((ref.shape[0], fec.nldpc), np.uint8)
+    for f in range(ref.shape[0]):
+        info = np.ascontiguousarray(np.unpackbits(ref[f])[:fec.kldpc], np.uint8)
+        assert L.sim_ldpc_encode(short_frame, rate.encode(), info.ctypes.data, out[f].ctypes.data) == 0
+    return out

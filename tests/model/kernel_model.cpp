@@ -138,9 +138,16 @@ int kmodel_decode(int code, const int8_t* llr, int max_iter, int nbits, uint8_t*
 
 /* ---- the two-frame packed path (dvb-rx_amd/csrc/ldpc_pk.h) --------------- */
 
-template <int DEG>
+/* learned parameters of the two slots for this layer: rows (S, B) by
+ * check degree; NULL rows: the plain decoder's arithmetic (pk_mags) */
+struct PkRows {
+    const uint8_t* S[2];
+    const int8_t* B[2];
+};
+
+template <int DEG, bool NMS>
 static void pk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t sel_even,
-                           uint32_t sel_odd)
+                           uint32_t sel_odd, const PkRows& rows)
 {
     const int K = c->k, q = c->q;
     const uint32_t* ent = c->ent + c->layer_start[i];
@@ -149,40 +156,54 @@ static void pk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
     const int W = LG_PK_WORDS(DEG);
     static uint32_t mw[360][LG_PK_WORDS(28)];
     uint32_t* m = msg + (size_t)c->pk_msg_off[i] * 360;
-    for (int j = 0; j < 360; j++)
+    lg_pk ps[360], pb[360];
+    for (int j = 0; j < 360; j++) {
         for (int w = 0; w < W; w++)
             mw[j][w] = m[w * 360 + j];
+        ps[j] = lg_pk{ 16, 16 };
+        pb[j] = lg_pk{ 1, 1 };
+        if (NMS) {
+            const int dc = DEG + 2 - (i == 0 && j == 0);
+            ps[j] = lg_pk{ (short)rows.S[0][dc], (short)rows.S[1][dc] };
+            pb[j] = lg_pk{ (short)rows.B[0][dc], (short)rows.B[1][dc] };
+        }
+    }
     for (int j = 359; j >= 0; j--) {
         if (c->nlev[i] > 1)
             pk_phase_a<DEG, true>(nodes[j], post, ent, i, j, K, q, LG_META_DEP(meta[j]), mw[j], sel_even, sel_odd);
         else
             pk_phase_a<DEG, false>(nodes[j], post, ent, i, j, K, q, 0, mw[j], sel_even, sel_odd);
         if (LG_META_LEVEL(meta[j]) == 0)
-            pk_write_succ<DEG>(nodes[j], post, LG_META_SUCC(meta[j]));
+            pk_write_succ<DEG, NMS>(nodes[j], post, LG_META_SUCC(meta[j]), ps[j], pb[j]);
     }
     for (int l = 1; l < c->nlev[i]; l++)
         for (int j = 359; j >= 0; j--)
             if (LG_META_LEVEL(meta[j]) == l) {
                 pk_step<DEG>(nodes[j], post, LG_META_DEP(meta[j]), mw[j], sel_even, sel_odd);
-                pk_write_succ<DEG>(nodes[j], post, LG_META_SUCC(meta[j]));
+                pk_write_succ<DEG, NMS>(nodes[j], post, LG_META_SUCC(meta[j]), ps[j], pb[j]);
             }
     for (int j = 359; j >= 0; j--) {
         uint32_t o[LG_PK_WORDS(DEG)];
         if (c->nlev[i] > 1)
-            pk_phase_c<DEG, true>(nodes[j], post, LG_META_SUCC(meta[j]), o);
+            pk_phase_c<DEG, true, NMS>(nodes[j], post, LG_META_SUCC(meta[j]), o, ps[j], pb[j]);
         else
-            pk_phase_c<DEG, false>(nodes[j], post, 0, o);
+            pk_phase_c<DEG, false, NMS>(nodes[j], post, 0, o, ps[j], pb[j]);
         for (int w = 0; w < W; w++)
             m[w * 360 + j] = o[w];
     }
 }
 
-static int pk_layer_any(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t se, uint32_t so)
+static int pk_layer_any(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t se, uint32_t so,
+                        const PkRows* rows)
 {
+    static const PkRows none = { { nullptr, nullptr }, { nullptr, nullptr } };
     switch (c->layer_start[i + 1] - c->layer_start[i]) {
-#define X(D)                                          \
-    case D:                                           \
-        pk_model_layer<D>(c, i, post, msg, se, so);   \
+#define X(D)                                                         \
+    case D:                                                          \
+        if (rows)                                                    \
+            pk_model_layer<D, true>(c, i, post, msg, se, so, *rows); \
+        else                                                         \
+            pk_model_layer<D, false>(c, i, post, msg, se, so, none); \
         return 0;
         DEG_CASES(X)
 #undef X
@@ -205,14 +226,20 @@ static uint32_t pk_check_any(const lg_code* c, int i, int j, const uint16_t* pos
 
 /* Two frames of one code in the two slots; frame B enters slot 1 `delay`
  * iterations after frame A started in slot 0 (as when a slot is refilled
- * while the other one runs).  Each slot stops on its own syndrome. */
-extern "C" int kmodel_pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay, int max_iter,
-                              int nbits, uint8_t* out_a, uint8_t* out_b, int* it_out, int* ok_out)
+ * while the other one runs).  Each slot stops on its own syndrome.
+ * With a learned table (S, B: T rows of 33, NULL: plain decoder) a slot
+ * runs up to max_iter learned iterations (row min(it, T-1)); if its
+ * syndrome still fails and fallback > 0 the frame is loaded again and
+ * decoded by the plain arithmetic for up to fallback iterations, as the
+ * kernel's learned mode does. */
+static int pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay, int max_iter, int nbits,
+                   const uint8_t* S, const int8_t* B, int T, int fallback, uint8_t* out_a, uint8_t* out_b,
+                   int* it_out, int* ok_out, int* fb_out)
 {
     const lg_code* c = lg_code_get(code);
     if (!c)
         return -1;
-    const int K = c->k, R = c->r, q = c->q, N = c->n;
+    const int K = c->k, q = c->q, N = c->n;
     uint16_t* post = (uint16_t*)calloc((size_t)N, sizeof(uint16_t));
     uint32_t* msg = (uint32_t*)calloc((size_t)c->pk_msg_words, sizeof(uint32_t));
     if (!post || !msg) {
@@ -220,22 +247,30 @@ extern "C" int kmodel_pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b
         free(msg);
         return -1;
     }
+    static uint8_t plainS[33];
+    static int8_t plainB[33];
+    for (int d = 0; d < 33; d++) {
+        plainS[d] = 16;
+        plainB[d] = 1;
+    }
     const int8_t* src[2] = { llr_a, llr_b };
     uint8_t* dst[2] = { out_a, out_b };
-    int it[2] = { 0, 0 }, active[2] = { 0, 0 }, started[2] = { 0, 0 };
+    int it[2] = { 0, 0 }, active[2] = { 0, 0 }, started[2] = { 0, 0 }, phase[2] = { 0, 0 }, lit[2] = { 0, 0 };
     if (nbits <= 0 || nbits > K)
         nbits = K;
+    auto load = [&](int s) {
+        for (int v = 0; v < N; v++) {
+            const int pos = v < K ? v : K + 360 * ((v - K) % q) + (v - K) / q;
+            post[pos] = (uint16_t)((post[pos] & (s ? 0x00ff : 0xff00)) | (uint16_t)(uint8_t)src[s][v] << (8 * s));
+        }
+        it[s] = 0;
+    };
     for (int t = 0;; t++) {
         for (int s = 0; s < 2; s++) {
             if (started[s] || t < (s ? delay : 0))
                 continue;
-            /* load the frame's LLRs into the slot's bytes */
-            for (int v = 0; v < N; v++) {
-                const int pos = v < K ? v : K + 360 * ((v - K) % q) + (v - K) / q;
-                post[pos] = (uint16_t)((post[pos] & (s ? 0x00ff : 0xff00)) | (uint16_t)(uint8_t)src[s][v] << (8 * s));
-            }
+            load(s);
             started[s] = active[s] = 1;
-            it[s] = 0;
         }
         uint32_t good = 3;
         for (int i = 0; i < q; i++)
@@ -245,13 +280,22 @@ extern "C" int kmodel_pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b
             if (!active[s])
                 continue;
             const int ok = (good >> s) & 1;
-            if (ok || it[s] >= max_iter) {
+            const int cap = S && phase[s] == 1 ? fallback : max_iter;
+            if (ok || it[s] >= cap) {
+                if (S && !ok && phase[s] == 0 && fallback > 0) {
+                    lit[s] = it[s];
+                    phase[s] = 1;
+                    load(s);
+                    continue;
+                }
                 memset(dst[s], 0, (size_t)(nbits + 7) / 8);
                 for (int v = 0; v < nbits; v++)
                     if ((int8_t)(post[v] >> (8 * s)) < 0)
                         dst[s][v / 8] |= (uint8_t)(0x80 >> (v % 8));
-                it_out[s] = it[s];
+                it_out[s] = lit[s] + it[s];
                 ok_out[s] = ok;
+                if (fb_out)
+                    fb_out[s] = phase[s];
                 active[s] = 0;
             }
         }
@@ -259,9 +303,16 @@ extern "C" int kmodel_pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b
             break;
         const uint32_t se = pk_msg_sel(0, !active[0] || it[0] == 0, !active[1] || it[1] == 0);
         const uint32_t so = pk_msg_sel(1, !active[0] || it[0] == 0, !active[1] || it[1] == 0);
+        PkRows rows;
+        for (int s = 0; s < 2; s++) {
+            const int learned = S && active[s] && phase[s] == 0;
+            const int r = it[s] < T ? it[s] : T - 1;
+            rows.S[s] = learned ? S + (size_t)r * 33 : plainS;
+            rows.B[s] = learned ? B + (size_t)r * 33 : plainB;
+        }
         if (active[0] || active[1])
             for (int i = 0; i < q; i++)
-                if (pk_layer_any(c, i, post, msg, se, so)) {
+                if (pk_layer_any(c, i, post, msg, se, so, S ? &rows : nullptr)) {
                     free(post);
                     free(msg);
                     return -1;
@@ -269,10 +320,24 @@ extern "C" int kmodel_pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b
         for (int s = 0; s < 2; s++)
             it[s] += active[s];
     }
-    (void)R;
     free(post);
     free(msg);
     return 0;
+}
+
+extern "C" int kmodel_pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay, int max_iter,
+                              int nbits, uint8_t* out_a, uint8_t* out_b, int* it_out, int* ok_out)
+{
+    return pk_pair(code, llr_a, llr_b, delay, max_iter, nbits, nullptr, nullptr, 0, 0, out_a, out_b, it_out, ok_out,
+                   nullptr);
+}
+
+extern "C" int kmodel_pk_pair_nms(int code, const int8_t* llr_a, const int8_t* llr_b, int delay, const uint8_t* S,
+                                  const int8_t* B, int T, int max_iter, int fallback, int nbits, uint8_t* out_a,
+                                  uint8_t* out_b, int* it_out, int* ok_out, int* fb_out)
+{
+    return pk_pair(code, llr_a, llr_b, delay, max_iter, nbits, S, B, T, fallback, out_a, out_b, it_out, ok_out,
+                   fb_out);
 }
 
 /* The schedule's own invariants: no two nodes of one level of a layer

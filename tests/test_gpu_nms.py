@@ -1,0 +1,164 @@
+"""GPU parity of the learned offset min-sum training kernels and the GPU
+channel (include/ldpc_nms.h), against the CPU oracle.
+
+* the training step's forward (integer, quantised parameters) gives the
+  oracle's posteriors exactly; its loss and gradient agree with the
+  oracle's float64 accumulation to float32 rounding;
+* Adam on the GPU equals the textbook update;
+* the GPU channel: deterministic per seed, unit-channel LLR statistics,
+  noiseless hard decisions equal the codeword, and MODCOD frames at the
+  quasi-error-free threshold + 1 dB decode.
+"""
+import numpy as np
+import pytest
+
+from helpers import NMS_DEGS, Oracle, codewords, gen, ldpcgpu
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dec():
+    d = ldpcgpu.Decoder(0)
+    yield d
+    d.close()
+
+
+class Dev:
+    """Device copies of host arrays through a decoder context."""
+
+    def __init__(self, dec):
+        self.dec = dec
+        self.ptrs = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.dec.malloc(max(a.nbytes, 1))
+        self.dec.upload(p, a)
+        self.ptrs.append(p)
+        return p
+
+    def empty(self, nbytes):
+        p = self.dec.malloc(nbytes)
+        self.ptrs.append(p)
+        return p
+
+    def get(self, p, shape, dtype):
+        out = np.zeros(shape, dtype)
+        self.dec.download(out, p)
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.dec.free(p)
+        self.ptrs = []
+
+
+@pytest.mark.parametrize("short,rate,esn0,nf,T", [(1, "1_2", 1.0, 5, 4), (0, "3_5", 2.4, 2, 3), (1, "8_9", 6.0, 3, 3)])
+def test_train_step_matches_oracle(dec, short, rate, esn0, nf, T):
+    fec, llr, ref = gen(short, rate, esn0, nf, 500 + nf)
+    cw = codewords(short, rate, ref)
+    code = ldpcgpu.code_lookup(short, rate)
+    rng = np.random.default_rng(nf)
+    alpha = rng.uniform(0.6, 1.1, (T, NMS_DEGS)).astype(np.float32)
+    beta = rng.uniform(0.0, 2.0, (T, NMS_DEGS)).astype(np.float32)
+    w = np.linspace(0.2, 1.0, T).astype(np.float32)
+    o = Oracle(short, rate, fec.kldpc)
+    loss, ga, gb, post = o.nms_train(1, llr.astype(np.float32), cw, alpha, beta, w, temp=0.5)
+
+    dv = Dev(dec)
+    tr = ldpcgpu.NmsTrainer(code, T, nf)
+    try:
+        d_llr = dv.put(llr)
+        d_cw = dv.put(cw[::-1].copy())  # rows reversed: frame f uses row nf-1-f
+        d_row = dv.put(np.arange(nf - 1, -1, -1, dtype=np.int32))
+        d_par = dv.put(np.concatenate([alpha, beta]))
+        d_w = dv.put(w)
+        d_loss = dv.empty(4)
+        d_grad = dv.empty(4 * 2 * T * NMS_DEGS)
+        d_post = dv.empty(llr.nbytes)
+        tr.step(d_llr, d_cw, d_row, nf, d_par, d_w, 0.5, d_loss, d_grad, d_post)
+        tr.synchronize()
+        gpost = dv.get(d_post, llr.shape, np.int8)
+        gloss = float(dv.get(d_loss, 1, np.float32)[0])
+        grad = dv.get(d_grad, (2 * T, NMS_DEGS), np.float32)
+    finally:
+        tr.close()
+        dv.free()
+    assert (gpost.astype(np.float32) == post).all()
+    assert abs(gloss - loss) <= 1e-4 * abs(loss)
+    g_ref = np.concatenate([ga, gb])
+    scale = np.abs(g_ref).max()
+    assert scale > 0
+    assert np.abs(grad - g_ref).max() <= 2e-3 * scale, np.abs(grad - g_ref).max() / scale
+    assert ((grad != 0) == (g_ref != 0)).all()
+
+
+def test_adam_matches_textbook(dec):
+    rng = np.random.default_rng(9)
+    n = 2 * 3 * NMS_DEGS
+    p = rng.normal(size=n).astype(np.float32)
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    dv = Dev(dec)
+    tr = ldpcgpu.NmsTrainer(ldpcgpu.code_lookup(1, "1_2"), 3, 1)
+    try:
+        d_p, d_m, d_v = dv.put(p), dv.put(m), dv.put(v)
+        for step in range(1, 4):
+            g = rng.normal(size=n).astype(np.float32)
+            d_g = dv.put(g)
+            tr.adam(d_p, d_g, d_m, d_v, n, 1e-2, step)
+            tr.synchronize()
+            m = 0.9 * m + 0.1 * g
+            v = 0.999 * v + 0.001 * g * g
+            p = p - 1e-2 * (m / (1 - 0.9 ** step)) / (np.sqrt(v / (1 - 0.999 ** step)) + 1e-8)
+        got = dv.get(d_p, n, np.float32)
+    finally:
+        tr.close()
+        dv.free()
+    assert np.allclose(got, p, rtol=1e-5, atol=1e-6)
+
+
+def _channel(dec, pls, n, cw, esn0, seed):
+    dv = Dev(dec)
+    try:
+        nf = len(esn0)
+        d_cw = dv.put(cw)
+        d_row = dv.put(np.arange(nf, dtype=np.int32) % cw.shape[0])
+        d_e = dv.put(np.asarray(esn0, np.float32))
+        d_llr = dv.empty(nf * n)
+        ldpcgpu.channel(pls, n, d_cw, d_row, nf, d_e, seed, d_llr, stream=dec.stream)
+        dec.synchronize()
+        return dv.get(d_llr, (nf, n), np.int8)
+    finally:
+        dv.free()
+
+
+def test_channel_unit_statistics_and_seed(dec):
+    fec, _, ref = gen(0, "1_2", 1000.0, 2, 3)
+    cw = codewords(0, "1_2", ref)
+    esn0 = 2.0
+    a = _channel(dec, -1, fec.nldpc, cw, [esn0, esn0], 11)
+    b = _channel(dec, -1, fec.nldpc, cw, [esn0, esn0], 11)
+    c = _channel(dec, -1, fec.nldpc, cw, [esn0, esn0], 12)
+    assert (a == b).all() and (a != c).any()
+    s2 = 1.0 / 10 ** (esn0 / 10)
+    signed = a.astype(np.float64) * np.where(cw == 1, -1, 1)
+    assert abs(signed.mean() - 4 / s2) < 0.03 * (4 / s2)
+    # the spread of 4y/s2 is 4/s
+    assert abs(signed.std() - 4 / np.sqrt(s2)) < 0.05 * (4 / np.sqrt(s2))
+
+
+@pytest.mark.parametrize("pls", [0x10, 0x30, 0x48, 0x58, 0x5c])
+def test_channel_noiseless_and_threshold(dec, pls):
+    m = ldpcgpu.pls_lookup(pls)
+    short, rate = m["short_frame"], m["rate"]
+    fec, _, ref = gen(short, rate, 1000.0, 4, 21)
+    cw = codewords(short, rate, ref)
+    hi = _channel(dec, pls, fec.nldpc, cw, [40.0] * 4, 5)
+    assert ((hi < 0) == (cw == 1)).all() and (hi != 0).all()
+    llr = _channel(dec, pls, fec.nldpc, cw, [ldpcgpu.qef_db(pls) + 1.0] * 8, 6)
+    o = Oracle(short, rate, fec.kldpc)
+    bits, it, ok = o.frames(llr, 50)
+    assert ok.all()
+    assert (bits == np.concatenate([ref, ref])).all()

@@ -15,7 +15,7 @@ from helpers import ROOT, ldpcgpu
 
 LIB = os.path.join(ROOT, "dvb-rx_amd", "lib")
 HEADERS = {
-    "libldpcgpu.so": ["include/ldpc_gpu.h", "include/dvbrx/ldpc.h"],
+    "libldpcgpu.so": ["include/ldpc_gpu.h", "include/dvbrx/ldpc.h", "include/ldpc_nms.h"],
     "libdvbs2sim.so": ["include/dvbs2_sim.h"],
 }
 

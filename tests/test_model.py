@@ -76,3 +76,42 @@ def test_packed_pair_extremes():
     oa, ob, pit, pok = pk_pair(code, a, b, 0, 4, fec.kldpc)
     assert list(pit) == list(it) and list(pok) == list(ok)
     assert (oa == bits[0]).all() and (ob == bits[1]).all()
+
+
+def pk_pair_nms(code, a, b, delay, S, B, max_iter, fallback, nbits):
+    L = kmodel()
+    vp = C.c_void_p
+    L.kmodel_pk_pair_nms.argtypes = [C.c_int, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
+                                     vp, vp, vp]
+    a = np.ascontiguousarray(a, np.int8)
+    b = np.ascontiguousarray(b, np.int8)
+    S = np.ascontiguousarray(S, np.uint8)
+    B = np.ascontiguousarray(B, np.int8)
+    kb = (nbits + 7) // 8
+    oa, ob = np.zeros(kb, np.uint8), np.zeros(kb, np.uint8)
+    it, ok, fb = np.zeros(2, np.int32), np.zeros(2, np.int32), np.zeros(2, np.int32)
+    rc = L.kmodel_pk_pair_nms(code, a.ctypes.data, b.ctypes.data, delay, S.ctypes.data, B.ctypes.data, S.shape[0],
+                              max_iter, fallback, nbits, oa.ctypes.data, ob.ctypes.data, it.ctypes.data,
+                              ok.ctypes.data, fb.ctypes.data)
+    assert rc == 0
+    return oa, ob, it, ok, fb
+
+
+@pytest.mark.parametrize("short,rate,delay,esn0", [(0, "3_5", 2, 2.3), (1, "1_2", 3, 1.2), (0, "2_3", 0, 3.1),
+                                                   (1, "8_9", 1, 6.0), (0, "2_9_VLSNR", 2, -2.0)])
+def test_packed_pair_learned_model(short, rate, delay, esn0):
+    """The packed learned arithmetic (per-slot scale / offset by
+    iteration and check degree, plain fallback after the learned cap)
+    against the oracle's learned decoder."""
+    fec, llr, _ = gen(short, rate, esn0, 2, 60 + delay, stream0=3)
+    code = ldpcgpu.code_lookup(short, rate)
+    T = 6
+    rng = np.random.default_rng(delay)
+    S = rng.integers(8, 20, (T, 33)).astype(np.uint8)
+    B = rng.integers(-1, 3, (T, 33)).astype(np.int8)
+    o = Oracle(short, rate, fec.kldpc)
+    for learned_max, fallback in ((T, 30), (3, 0), (8, 30)):
+        bits, it, ok, fb = o.nms_frames(S, B, llr, learned_max, fallback)
+        oa, ob, pit, pok, pfb = pk_pair_nms(code, llr[0], llr[1], delay, S, B, learned_max, fallback, fec.kldpc)
+        assert list(pit) == list(it) and list(pok) == list(ok) and list(pfb) == list(fb), (learned_max, fallback)
+        assert (oa == bits[0]).all() and (ob == bits[1]).all()
