@@ -44,6 +44,7 @@
 
 #include "ldpc_codes.h"
 #include "ldpc_gpu.h"
+#include "ldpc_nms.h"
 #include "ldpc_node.h"
 #include "ldpc_pk.h"
 
@@ -64,6 +65,7 @@ struct DCode {
     int dmin;      /* smallest */
     int msg_words; /* 32-bit message words per frame */
     int pk_msg_words; /* the same for the two-frame layout */
+    int code_id;   /* index in the registry */
 };
 
 struct DFrame {
@@ -378,6 +380,23 @@ struct PkArgs {
     uint8_t* synd;
     uint32_t* msg;         /* message block per workgroup */
     size_t msg_stride;
+    /* learned mode: parameter rows (row 0: the plain decoder's S = 16,
+     * B = 1), per code its first row and row count, plain fallback cap,
+     * per-frame fallback flag (may be NULL) */
+    const uint8_t* nS;
+    const int8_t* nB;
+    const int* nrow0;
+    const int* nT;
+    int fallback;
+    uint8_t* fell;
+};
+
+/* parameter rows of the two slots for one iteration (learned mode) */
+struct PkRowPtrs {
+    const uint8_t* S0;
+    const uint8_t* S1;
+    const int8_t* B0;
+    const int8_t* B1;
 };
 
 /* AND of a 2-bit value over the workgroup, one barrier: three vote words
@@ -436,9 +455,10 @@ __device__ __forceinline__ uint32_t pk_syndrome(const uint16_t* sm, const uint32
     return good;
 }
 
-template <int DEG, bool TAIL>
+template <int DEG, bool TAIL, bool NMS>
 __device__ __forceinline__ void pk_layer_t(uint16_t* sm, const uint32_t* ent, int i, int j, int K, int q, int nlev,
-                                           const uint32_t* mw, uint64_t meta, uint32_t* mrow, uint32_t se, uint32_t so)
+                                           const uint32_t* mw, uint64_t meta, uint32_t* mrow, uint32_t se, uint32_t so,
+                                           lg_pk ps, lg_pk pb)
 {
     const bool act = j < 360;
     const uint32_t dep = TAIL ? LG_META_DEP(meta) : 0, succ = TAIL ? LG_META_SUCC(meta) : 0;
@@ -447,20 +467,20 @@ __device__ __forceinline__ void pk_layer_t(uint16_t* sm, const uint32_t* ent, in
     if (act) {
         pk_phase_a<DEG, TAIL>(n, sm, ent, i, j, K, q, dep, mw, se, so);
         if (TAIL && lvl == 0)
-            pk_write_succ<DEG>(n, sm, succ);
+            pk_write_succ<DEG, NMS>(n, sm, succ, ps, pb);
     }
     if (TAIL) {
         for (int l = 1; l < nlev; l++) {
             __syncthreads();
             if (act && lvl == l) {
                 pk_step<DEG>(n, sm, dep, mw, se, so);
-                pk_write_succ<DEG>(n, sm, succ);
+                pk_write_succ<DEG, NMS>(n, sm, succ, ps, pb);
             }
         }
     }
     if (act) {
         uint32_t o[LG_PK_WORDS(DEG)];
-        pk_phase_c<DEG, TAIL>(n, sm, succ, o);
+        pk_phase_c<DEG, TAIL, NMS>(n, sm, succ, o, ps, pb);
 #pragma unroll
         for (int w = 0; w < LG_PK_WORDS(DEG); w++)
             mrow[360 * w + j] = o[w];
@@ -469,10 +489,10 @@ __device__ __forceinline__ void pk_layer_t(uint16_t* sm, const uint32_t* ent, in
 
 /* one iteration (all layers) on both slots; the code's tables are read
  * through uniform pointers (scalar loads) */
-template <int CLS>
+template <int CLS, bool NMS>
 __device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, const uint2* lay,
                                              const uint32_t* pkoff, const uint64_t* meta, uint32_t* msg, int K,
-                                             int q, int first0, int first1)
+                                             int q, int first0, int first1, PkRowPtrs rows)
 {
     const int j = threadIdx.x;
     const bool act = j < 360;
@@ -511,12 +531,20 @@ __device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, 
         const uint32_t* e = ent + lay_start(ly);
         uint32_t* mrow = msg + 360 * pkoff[i];
         const int nlev = lay_nlev(ly);
-#define X(D)                                                                        \
-    case D:                                                                         \
-        if (nlev > 1)                                                               \
-            pk_layer_t<D, true>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so);     \
-        else                                                                        \
-            pk_layer_t<D, false>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so);    \
+        /* learned scale / offset of the layer's check degree, per slot;
+         * node (0, 0) has no previous parity, one edge less */
+        lg_pk ps = pk_dup(16), pb = pk_dup(1);
+        if constexpr (NMS) {
+            const int dc = lay_deg(ly) + 2 - (i == 0 && j == 0);
+            ps = lg_pk{ (short)rows.S0[dc], (short)rows.S1[dc] };
+            pb = lg_pk{ (short)rows.B0[dc], (short)rows.B1[dc] };
+        }
+#define X(D)                                                                                      \
+    case D:                                                                                       \
+        if (nlev > 1)                                                                             \
+            pk_layer_t<D, true, NMS>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so, ps, pb);      \
+        else                                                                                      \
+            pk_layer_t<D, false, NMS>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so, ps, pb);     \
         break;
         if constexpr (CLS == 0) {
             switch (lay_deg(ly)) {
@@ -540,7 +568,7 @@ __device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, 
  * continues.  When the segment is exhausted and both slots are empty, the
  * workgroup moves to another segment with frames left, or exits.
  */
-template <int CLS>
+template <int CLS, bool NMS>
 __global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
 {
     extern __shared__ __align__(16) uint16_t sm16[];
@@ -553,6 +581,7 @@ __global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
     int seg = -1;                 /* current segment (uniform) */
     int frame[2] = { -1, -1 };    /* frame in each slot, -1: empty */
     int it[2] = { 0, 0 }, maxit[2] = { 0, 0 };
+    int phase[2] = { 0, 0 }, lit[2] = { 0, 0 }; /* learned mode: 1 = plain fallback; learned iterations */
     uint32_t* msg = a.msg + a.msg_stride * (size_t)blockIdx.x;
     DCode cd;
     const uint64_t* meta = nullptr;
@@ -611,14 +640,10 @@ __global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
         const uint2* lay = a.lay + cd.lay_off;
         const uint32_t* pkoff = a.pkoff + cd.lay_off;
         const int K = cd.k, R = cd.r, q = cd.q;
-        /* ---- load the new frames into their slot bytes ---- */
-        for (int s = 0; s < 2; s++) {
-            const int f = s ? cl1 : cl0;
-            if (f < 0)
-                continue;
+        /* a frame's LLRs into its slot bytes: information bits in order,
+         * parity bit K + q*jj + ii at K + 360*ii + jj (layer order) */
+        auto load_slot = [&](int s, int f) {
             const DFrame fr = a.frames[f];
-            it[s] = 0;
-            maxit[s] = fr.max_iter;
             const uint32_t* src = (const uint32_t*)(a.llr + fr.llr_off);
             uint8_t* sb = (uint8_t*)sm16 + s;
             for (int v4 = tid; v4 < K / 4; v4 += NT) {
@@ -631,6 +656,17 @@ __global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
                 const int jj = t / q, ii = t - jj * q;
                 sb[2 * (K + 360 * ii + jj)] = par[t];
             }
+        };
+        /* ---- load the new frames into their slot bytes ---- */
+        for (int s = 0; s < 2; s++) {
+            const int f = s ? cl1 : cl0;
+            if (f < 0)
+                continue;
+            it[s] = 0;
+            maxit[s] = a.frames[f].max_iter;
+            phase[s] = 0;
+            lit[s] = 0;
+            load_slot(s, f);
         }
         __syncthreads();
 
@@ -639,13 +675,24 @@ __global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
             const uint32_t live = (frame[0] >= 0 ? 1u : 0u) | (frame[1] >= 0 ? 2u : 0u);
             const uint32_t good = pk_syndrome<CLS>(sm16, ent, lay, tid, K, q, live, s_vote, nvote);
             uint32_t done = 0;
-            for (int s = 0; s < 2; s++)
-                if (frame[s] >= 0 && (((good >> s) & 1) || it[s] >= maxit[s]))
+            for (int s = 0; s < 2; s++) {
+                const int cap = NMS && phase[s] ? a.fallback : maxit[s];
+                if (frame[s] >= 0 && (((good >> s) & 1) || it[s] >= cap))
                     done |= 1u << s;
+            }
             if (done) {
+                uint32_t reload = 0;
                 for (int s = 0; s < 2; s++) {
                     if (!((done >> s) & 1))
                         continue;
+                    if constexpr (NMS) {
+                        /* learned iterations failed: decode the frame again
+                         * from its LLRs with the plain arithmetic */
+                        if (!((good >> s) & 1) && phase[s] == 0 && a.fallback > 0) {
+                            reload |= 1u << s;
+                            continue;
+                        }
+                    }
                     const DFrame fr = a.frames[frame[s]];
                     const int nbytes = (fr.nbits + 7) / 8;
                     uint8_t* o = a.out + fr.out_off;
@@ -660,16 +707,46 @@ __global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
                             o[b4 + k] = (uint8_t)(v >> (24 - 8 * k));
                     }
                     if (tid == 0) {
-                        a.iters[frame[s]] = it[s];
+                        a.iters[frame[s]] = lit[s] + it[s];
                         a.synd[frame[s]] = (uint8_t)((good >> s) & 1);
+                        if (NMS && a.fell)
+                            a.fell[frame[s]] = (uint8_t)phase[s];
                     }
                     frame[s] = -1;
                 }
                 __syncthreads(); /* slot bytes read before a refill overwrites them */
+                if constexpr (NMS) {
+                    if (reload) {
+                        for (int s = 0; s < 2; s++) {
+                            if (!((reload >> s) & 1))
+                                continue;
+                            lit[s] = it[s];
+                            it[s] = 0;
+                            phase[s] = 1;
+                            load_slot(s, frame[s]);
+                        }
+                        __syncthreads();
+                    }
+                    if (reload == done)
+                        continue; /* no slot emptied: keep iterating */
+                }
                 break;
             }
-            pk_iteration<CLS>(sm16, ent, lay, pkoff, meta, msg, K, q, frame[0] < 0 || it[0] == 0,
-                              frame[1] < 0 || it[1] == 0);
+            PkRowPtrs rows;
+            if constexpr (NMS) {
+                const int r0c = a.nrow0[cd.code_id], tc = max(a.nT[cd.code_id], 1);
+                int row[2];
+                for (int s = 0; s < 2; s++)
+                    row[s] = frame[s] >= 0 && phase[s] == 0 ? r0c + min(it[s], tc - 1) : 0;
+                rows.S0 = a.nS + LDPC_NMS_DEGS * row[0];
+                rows.S1 = a.nS + LDPC_NMS_DEGS * row[1];
+                rows.B0 = a.nB + LDPC_NMS_DEGS * row[0];
+                rows.B1 = a.nB + LDPC_NMS_DEGS * row[1];
+            } else {
+                rows = PkRowPtrs{ nullptr, nullptr, nullptr, nullptr };
+            }
+            pk_iteration<CLS, NMS>(sm16, ent, lay, pkoff, meta, msg, K, q, frame[0] < 0 || it[0] == 0,
+                                   frame[1] < 0 || it[1] == 0, rows);
             for (int s = 0; s < 2; s++)
                 it[s] += frame[s] >= 0;
         }
@@ -722,6 +799,16 @@ struct ldpc_gpu_ctx {
     int cls_di[NLAUNCH];     /* its degree class bound */
     int timed;
     int force_scalar;        /* per-frame batches through the one-frame kernel */
+    /* learned tables: row 0 plain (S = 16, B = 1), then per code its rows */
+    uint8_t* h_nS;
+    int8_t* h_nB;
+    int* h_nrow0;
+    int* h_nT;
+    uint8_t* d_nS;
+    int8_t* d_nB;
+    int* d_nrow0;
+    int* d_nT;
+    int nms_dirty;
 };
 
 static int set_err(ldpc_gpu_ctx* c, int code, const char* fmt, ...)
@@ -774,6 +861,7 @@ static int upload_tables(ldpc_gpu_ctx* c)
         d.dmin = 255;
         d.msg_words = cd->msg_words;
         d.pk_msg_words = cd->pk_msg_words;
+        d.code_id = id;
         ent.insert(ent.end(), cd->ent, cd->ent + cd->nent);
         for (int i = 0; i < cd->q; i++) {
             const int st = cd->layer_start[i], deg = cd->layer_start[i + 1] - st;
@@ -871,6 +959,14 @@ void ldpc_gpu_destroy(ldpc_gpu_ctx* c)
         if (c->ev_cls[k])
             (void)hipEventDestroy(c->ev_cls[k]);
     (void)hipFree(c->d_seg);
+    (void)hipFree(c->d_nS);
+    (void)hipFree(c->d_nB);
+    (void)hipFree(c->d_nrow0);
+    (void)hipFree(c->d_nT);
+    free(c->h_nS);
+    free(c->h_nB);
+    free(c->h_nrow0);
+    free(c->h_nT);
     for (int k = 0; k < NLAUNCH; k++)
         if (c->aux[k]) {
             (void)hipStreamSynchronize(c->aux[k]);
@@ -1147,11 +1243,11 @@ static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFr
     return LDPC_GPU_OK;
 }
 
-template <int CLS>
+template <int CLS, bool NMS>
 static int pk_grid(ldpc_gpu_ctx* c, size_t lds, int nframes, int* grid)
 {
     int per_cu = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_pk_kernel<CLS>, NT, lds));
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_pk_kernel<CLS, NMS>, NT, lds));
     if (per_cu < 1)
         return set_err(c, LDPC_GPU_EDEVICE, "two-slot kernel does not fit (LDS %zu)", lds);
     *grid = std::min((nframes + 1) / 2, per_cu * c->ncu);
@@ -1162,7 +1258,8 @@ static int pk_grid(ldpc_gpu_ctx* c, size_t lds, int nframes, int* grid)
  * One launch per (degree class, frame size), each over segments of frames
  * of one code. */
 static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFrame>& hf, const int8_t* d_llr,
-                     uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd)
+                     uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, bool nms = false, int fallback = 0,
+                     uint8_t* d_fell = nullptr)
 {
     const std::vector<DCode>& hc = c->tab->h_codes;
     std::vector<int> gcodes[NLAUNCH];
@@ -1196,7 +1293,9 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
         if (!nfr[k])
             continue;
         const size_t lds = 2 * (size_t)maxn[k];
-        const int rc = k < 2 ? pk_grid<0>(c, lds, nfr[k], &grid[k]) : pk_grid<1>(c, lds, nfr[k], &grid[k]);
+        const int rc = nms ? (k < 2 ? pk_grid<0, true>(c, lds, nfr[k], &grid[k]) : pk_grid<1, true>(c, lds, nfr[k], &grid[k]))
+                           : (k < 2 ? pk_grid<0, false>(c, lds, nfr[k], &grid[k])
+                                    : pk_grid<1, false>(c, lds, nfr[k], &grid[k]));
         if (rc)
             return rc;
         maxgrid = std::max(maxgrid, grid[k]);
@@ -1230,6 +1329,14 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
     a.synd = d_synd;
     a.msg = c->msg;
     a.msg_stride = (size_t)maxw;
+    if (nms) {
+        a.nS = c->d_nS;
+        a.nB = c->d_nB;
+        a.nrow0 = c->d_nrow0;
+        a.nT = c->d_nT;
+        a.fallback = fallback;
+        a.fell = d_fell;
+    }
     /* the launches run concurrently on forked streams, so that the last
      * frames of one overlap the others' work; each has its own messages */
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -1248,10 +1355,17 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
         hipStream_t st = c->aux[k];
         HIPCHK(c, hipStreamWaitEvent(st, c->ev0, 0));
         HIPCHK(c, hipEventRecord(c->ev_cls[2 * k], st));
-        if (k < 2)
-            hipLaunchKernelGGL(ldpc_pk_kernel<0>, dim3(grid[k]), dim3(NT), lds, st, a);
-        else
-            hipLaunchKernelGGL(ldpc_pk_kernel<1>, dim3(grid[k]), dim3(NT), lds, st, a);
+        if (nms) {
+            if (k < 2)
+                hipLaunchKernelGGL((ldpc_pk_kernel<0, true>), dim3(grid[k]), dim3(NT), lds, st, a);
+            else
+                hipLaunchKernelGGL((ldpc_pk_kernel<1, true>), dim3(grid[k]), dim3(NT), lds, st, a);
+        } else {
+            if (k < 2)
+                hipLaunchKernelGGL((ldpc_pk_kernel<0, false>), dim3(grid[k]), dim3(NT), lds, st, a);
+            else
+                hipLaunchKernelGGL((ldpc_pk_kernel<1, false>), dim3(grid[k]), dim3(NT), lds, st, a);
+        }
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev_cls[2 * k + 1], st));
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cls[2 * k + 1], 0));
@@ -1261,19 +1375,51 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
     return LDPC_GPU_OK;
 }
 
-extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
-                                     uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd)
+/* ---- learned offset min-sum tables -------------------------------------- */
+
+#define NMS_ROWS (1 + 64 * LDPC_NMS_MAX_ITERS) /* row 0 plain, then per code LDPC_NMS_MAX_ITERS rows */
+
+extern "C" int ldpc_gpu_set_nms(ldpc_gpu_ctx* c, int code, int T, const uint8_t* S, const int8_t* B)
 {
     if (!c)
         return LDPC_GPU_EINVAL;
-    if (nframes == 0)
-        return LDPC_GPU_OK;
-    if (nframes < 0 || !frames || !d_llr || !d_out || !d_iters || !d_synd)
-        return set_err(c, LDPC_GPU_EINVAL, "bad arguments");
-    HIPCHK(c, hipSetDevice(c->device));
+    const int nc = (int)c->tab->h_codes.size();
+    if (code < 0 || code >= nc || nc > 64 || T < 0 || T > LDPC_NMS_MAX_ITERS || (T > 0 && (!S || !B)))
+        return set_err(c, LDPC_GPU_EINVAL, "set_nms: bad arguments (code %d, T %d)", code, T);
+    if (!c->h_nS) {
+        c->h_nS = (uint8_t*)malloc(NMS_ROWS * LDPC_NMS_DEGS);
+        c->h_nB = (int8_t*)malloc(NMS_ROWS * LDPC_NMS_DEGS);
+        c->h_nrow0 = (int*)calloc(64, sizeof(int));
+        c->h_nT = (int*)calloc(64, sizeof(int));
+        if (!c->h_nS || !c->h_nB || !c->h_nrow0 || !c->h_nT)
+            return set_err(c, LDPC_GPU_ENOMEM, "set_nms: host tables");
+        memset(c->h_nS, 16, NMS_ROWS * LDPC_NMS_DEGS);
+        memset(c->h_nB, 1, NMS_ROWS * LDPC_NMS_DEGS);
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipMalloc((void**)&c->d_nS, NMS_ROWS * LDPC_NMS_DEGS));
+        HIPCHK(c, hipMalloc((void**)&c->d_nB, NMS_ROWS * LDPC_NMS_DEGS));
+        HIPCHK(c, hipMalloc((void**)&c->d_nrow0, 64 * sizeof(int)));
+        HIPCHK(c, hipMalloc((void**)&c->d_nT, 64 * sizeof(int)));
+    }
+    const int r0 = 1 + code * LDPC_NMS_MAX_ITERS;
+    c->h_nrow0[code] = r0;
+    c->h_nT[code] = T;
+    if (T > 0) {
+        memcpy(c->h_nS + (size_t)r0 * LDPC_NMS_DEGS, S, (size_t)T * LDPC_NMS_DEGS);
+        memcpy(c->h_nB + (size_t)r0 * LDPC_NMS_DEGS, B, (size_t)T * LDPC_NMS_DEGS);
+    }
+    c->nms_dirty = 1;
+    return LDPC_GPU_OK;
+}
+
+/* frame table checks shared by the decode entry points */
+static int check_frames(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, std::vector<DFrame>& df,
+                        int* maxw, int* ngroup)
+{
     const std::vector<DCode>& hc = c->tab->h_codes;
-    std::vector<DFrame> df(nframes);
-    int maxw = 0, ngroup = 0;
+    df.resize(nframes);
+    *maxw = 0;
+    *ngroup = 0;
     for (int f = 0; f < nframes; f++) {
         const ldpc_gpu_frame& s = frames[f];
         if (s.code < 0 || s.code >= (int)hc.size())
@@ -1287,11 +1433,63 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
         df[f].pad = 0;
         df[f].llr_off = s.llr_off;
         df[f].out_off = s.out_off;
-        maxw = std::max(maxw, cd.msg_words);
-        ngroup += s.group >= 0;
+        *maxw = std::max(*maxw, cd.msg_words);
+        *ngroup += s.group >= 0;
     }
-    int rc = grow(c, &c->d_frames, &c->frames_bytes, sizeof(DFrame) * nframes);
+    return LDPC_GPU_OK;
+}
+
+extern "C" int ldpc_gpu_decode_batch_nms(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes,
+                                         const int8_t* d_llr, uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd,
+                                         int fallback_max_iter, uint8_t* d_fallback)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    if (nframes == 0)
+        return LDPC_GPU_OK;
+    if (nframes < 0 || !frames || !d_llr || !d_out || !d_iters || !d_synd || fallback_max_iter < 0)
+        return set_err(c, LDPC_GPU_EINVAL, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<DFrame> df;
+    int maxw, ngroup;
+    int rc = check_frames(c, frames, nframes, df, &maxw, &ngroup);
     if (rc)
+        return rc;
+    if (ngroup)
+        return set_err(c, LDPC_GPU_EINVAL, "learned decoding takes frames with their own early stop (group -1)");
+    for (int f = 0; f < nframes; f++)
+        if (!c->h_nT || c->h_nT[df[f].code] < 1)
+            return set_err(c, LDPC_GPU_EINVAL, "frame %d: no learned table for code %d", f, df[f].code);
+    if (c->nms_dirty) {
+        HIPCHK(c, hipMemcpyAsync(c->d_nS, c->h_nS, NMS_ROWS * LDPC_NMS_DEGS, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_nB, c->h_nB, NMS_ROWS * LDPC_NMS_DEGS, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_nrow0, c->h_nrow0, 64 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_nT, c->h_nT, 64 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->nms_dirty = 0;
+    }
+    if ((rc = grow(c, &c->d_frames, &c->frames_bytes, sizeof(DFrame) * nframes)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_frames, df.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice, c->stream));
+    return launch_pk(c, c->d_frames, df, d_llr, d_out, d_iters, d_synd, true, fallback_max_iter, d_fallback);
+}
+
+extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
+                                     uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    if (nframes == 0)
+        return LDPC_GPU_OK;
+    if (nframes < 0 || !frames || !d_llr || !d_out || !d_iters || !d_synd)
+        return set_err(c, LDPC_GPU_EINVAL, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<DFrame> df;
+    int maxw, ngroup;
+    int rc = check_frames(c, frames, nframes, df, &maxw, &ngroup);
+    if (rc)
+        return rc;
+    if ((rc = grow(c, &c->d_frames, &c->frames_bytes, sizeof(DFrame) * nframes)))
         return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_frames, df.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice, c->stream));
     std::vector<int> every(nframes);

@@ -80,6 +80,8 @@ def gpu_lib():
         L.ldpc_gpu_last_launches.argtypes = [vp, ip, ip, C.POINTER(C.c_float), i32]
         L.ldpc_gpu_code_max_degree.argtypes = [i32]
         L.ldpc_gpu_set_kernel.argtypes = [vp, i32]
+        L.ldpc_gpu_set_nms.argtypes = [vp, i32, i32, vp, vp]
+        L.ldpc_gpu_decode_batch_nms.argtypes = [vp, vp, i32, vp, vp, vp, vp, i32, vp]
         L.drx_ldpc_lanes.argtypes = []
         L.drx_ldpc_new.argtypes = [vp, C.c_char_p]
         L.drx_ldpc_new.restype = vp
@@ -281,6 +283,22 @@ class Decoder:
             self.L.ldpc_gpu_decode_batch(self.ctx, frames.ctypes.data, len(frames), d_llr, d_out, d_iters, d_synd),
             "decode_batch",
         )
+
+    def set_nms(self, code, S, B):
+        """Learned table of a code (T x NMS_DEGS scales S and offsets B);
+        S = None removes it."""
+        if S is None:
+            self._check(self.L.ldpc_gpu_set_nms(self.ctx, code, 0, None, None), "set_nms")
+            return
+        S = np.ascontiguousarray(S, np.uint8)
+        B = np.ascontiguousarray(B, np.int8)
+        assert S.shape == B.shape and S.shape[1] == NMS_DEGS
+        self._check(self.L.ldpc_gpu_set_nms(self.ctx, code, S.shape[0], S.ctypes.data, B.ctypes.data), "set_nms")
+
+    def decode_batch_nms(self, frames, d_llr, d_out, d_iters, d_synd, fallback, d_fell=None):
+        frames = np.ascontiguousarray(frames, FRAME_DTYPE)
+        self._check(self.L.ldpc_gpu_decode_batch_nms(self.ctx, frames.ctypes.data, len(frames), d_llr, d_out, d_iters,
+                                                     d_synd, fallback, d_fell), "decode_batch_nms")
 
     def last_kernel_ms(self):
         return self.L.ldpc_gpu_last_kernel_ms(self.ctx)

@@ -90,6 +90,28 @@ int ldpc_gpu_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nf
                          const int8_t* llr, size_t llr_bytes, uint8_t* out, size_t out_bytes,
                          int32_t* iters, uint8_t* synd);
 
+/*
+ * Learned offset min-sum decoding (no counterpart in the reference, whose
+ * offset is fixed at 1, libdvbrx/src/ldpc.c:353; the tables come from
+ * include/ldpc_nms.h's trainer).  ldpc_gpu_set_nms stores, for one code,
+ * T rows (T <= LDPC_NMS_MAX_ITERS) of LDPC_NMS_DEGS scales S (Q4, 16 =
+ * 1.0) and offsets B indexed by check degree; the check node's magnitude
+ * becomes clamp(((min(m, 127) * S[t][dc]) >> 4) - B[t][dc], 0, 127) at
+ * iteration t (row min(t, T-1)).  T = 0 removes the table.
+ *
+ * ldpc_gpu_decode_batch_nms decodes frames (group -1 only) with their
+ * code's table for up to frames[f].max_iter learned iterations; a frame
+ * whose syndrome still fails is decoded again from its LLRs by the plain
+ * decoder for up to fallback_max_iter iterations (0: no fallback), in the
+ * same kernel.  d_iters[f] = learned + plain iterations; d_fallback (may
+ * be NULL) = 1 where the fallback ran.  Enqueues only, as
+ * ldpc_gpu_decode_batch.
+ */
+int ldpc_gpu_set_nms(ldpc_gpu_ctx* ctx, int code, int T, const uint8_t* S, const int8_t* B);
+int ldpc_gpu_decode_batch_nms(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
+                              uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, int fallback_max_iter,
+                              uint8_t* d_fallback);
+
 /* Kernel choice for frames with their own early stop (group -1):
  * LDPC_GPU_KERNEL_AUTO runs two frames of one code per workgroup in packed
  * 16-bit arithmetic; LDPC_GPU_KERNEL_SINGLE one frame per workgroup (the

@@ -162,3 +162,124 @@ def test_channel_noiseless_and_threshold(dec, pls):
     bits, it, ok = o.frames(llr, 50)
     assert ok.all()
     assert (bits == np.concatenate([ref, ref])).all()
+
+
+def _decode_nms(dec, llr, codes, S, B, learned_max, fallback, nbits):
+    """Learned decode of frames (rows of llr, one code each) through the
+    C-ABI: (bits rows, iters, ok, fell)."""
+    n = llr.shape[1]
+    kb = (nbits + 7) // 8
+    frames = np.zeros(len(llr), ldpcgpu.FRAME_DTYPE)
+    frames["code"] = codes
+    frames["max_iter"] = learned_max
+    frames["nbits"] = nbits
+    frames["group"] = -1
+    frames["llr_off"] = np.arange(len(llr)) * n
+    frames["out_off"] = np.arange(len(llr)) * kb
+    dv = Dev(dec)
+    try:
+        d_llr = dv.put(llr)
+        d_out = dv.empty(len(llr) * kb)
+        d_it = dv.empty(4 * len(llr))
+        d_ok = dv.empty(len(llr))
+        d_fb = dv.empty(len(llr))
+        dec.decode_batch_nms(frames, d_llr, d_out, d_it, d_ok, fallback, d_fb)
+        dec.synchronize()
+        return (dv.get(d_out, (len(llr), kb), np.uint8), dv.get(d_it, len(llr), np.int32),
+                dv.get(d_ok, len(llr), np.uint8), dv.get(d_fb, len(llr), np.uint8))
+    finally:
+        dv.free()
+
+
+@pytest.mark.parametrize("short,rate,esn0", [(0, "3_5", 2.3), (0, "2_3", 3.1), (1, "1_2", 1.2), (0, "9_10", 6.4),
+                                             (1, "8_9", 6.1), (0, "2_9_VLSNR", -2.0), (0, "13_45", -1.0)])
+def test_learned_decode_matches_oracle(dec, short, rate, esn0):
+    fec, llr, _ = gen(short, rate, esn0, 12, 700, stream0=short)
+    code = ldpcgpu.code_lookup(short, rate)
+    T = 6
+    rng = np.random.default_rng(len(rate))
+    S = rng.integers(9, 20, (T, NMS_DEGS)).astype(np.uint8)
+    B = rng.integers(-1, 3, (T, NMS_DEGS)).astype(np.int8)
+    o = Oracle(short, rate, fec.kldpc)
+    dec.set_nms(code, S, B)
+    try:
+        for learned_max, fallback in ((T, 30), (3, 0), (9, 40)):
+            want = o.nms_frames(S, B, llr, learned_max, fallback)
+            got = _decode_nms(dec, llr, code, S, B, learned_max, fallback, fec.kldpc)
+            for w, g, name in zip(want, got, ("bits", "iters", "ok", "fell")):
+                assert (w == g).all(), (name, learned_max, fallback, w, g)
+    finally:
+        dec.set_nms(code, None, None)
+
+
+def test_learned_decode_mixed_batch_refill(dec):
+    """Several codes in one call, many frames per workgroup slot, fallback
+    in the middle of refills: every frame as the oracle decodes it."""
+    sets = [(0, "3_5", 2.7), (0, "2_3", 3.5), (1, "1_2", 1.6)]
+    tabs = {}
+    rows = []
+    rng = np.random.default_rng(77)
+    for k, (short, rate, esn0) in enumerate(sets):
+        fec, llr, _ = gen(short, rate, esn0, 40, 800 + k)
+        code = ldpcgpu.code_lookup(short, rate)
+        T = 5
+        S = rng.integers(13, 17, (T, NMS_DEGS)).astype(np.uint8)
+        B = rng.integers(0, 2, (T, NMS_DEGS)).astype(np.int8)
+        tabs[code] = (S, B, fec, llr, Oracle(short, rate, fec.kldpc))
+        dec.set_nms(code, S, B)
+    try:
+        # one buffer: frames of all codes interleaved, N-byte slots of 64800
+        order = [(code, f) for f in range(40) for code in tabs]
+        n = 64800
+        buf = np.zeros((len(order), n), np.int8)
+        for r, (code, f) in enumerate(order):
+            llr = tabs[code][3][f]
+            buf[r, :len(llr)] = llr
+        frames = np.zeros(len(order), ldpcgpu.FRAME_DTYPE)
+        kb = max((t[2].kldpc + 7) // 8 for t in tabs.values())
+        for r, (code, f) in enumerate(order):
+            frames[r] = (code, 12, tabs[code][2].kldpc, -1, r * n, r * kb)
+        dv = Dev(dec)
+        try:
+            d_llr = dv.put(buf)
+            d_out = dv.empty(len(order) * kb)
+            d_it = dv.empty(4 * len(order))
+            d_ok = dv.empty(len(order))
+            d_fb = dv.empty(len(order))
+            dec.decode_batch_nms(frames, d_llr, d_out, d_it, d_ok, 30, d_fb)
+            dec.synchronize()
+            out = dv.get(d_out, (len(order), kb), np.uint8)
+            it = dv.get(d_it, len(order), np.int32)
+            ok = dv.get(d_ok, len(order), np.uint8)
+            fb = dv.get(d_fb, len(order), np.uint8)
+        finally:
+            dv.free()
+        assert fb.any() and not fb.all()
+        for code, (S, B, fec, llr, o) in tabs.items():
+            wb, wit, wok, wfb = o.nms_frames(S, B, llr, 12, 30)
+            rows_c = [r for r, (c, f) in enumerate(order) if c == code]
+            nb = (fec.kldpc + 7) // 8
+            assert (out[rows_c, :nb] == wb).all()
+            assert (it[rows_c] == wit).all() and (ok[rows_c] == wok).all() and (fb[rows_c] == wfb).all()
+    finally:
+        for code in tabs:
+            dec.set_nms(code, None, None)
+
+
+def test_learned_decode_rejects_bad_calls(dec):
+    code = ldpcgpu.code_lookup(1, "1_2")
+    fr = np.zeros(1, ldpcgpu.FRAME_DTYPE)
+    fr["code"] = code
+    fr["max_iter"] = 5
+    fr["group"] = -1
+    with pytest.raises(RuntimeError):  # no table for the code
+        dec.decode_batch_nms(fr, 1, 1, 1, 1, 10)
+    S = np.full((3, NMS_DEGS), 16, np.uint8)
+    B = np.ones((3, NMS_DEGS), np.int8)
+    dec.set_nms(code, S, B)
+    try:
+        fr["group"] = 0
+        with pytest.raises(RuntimeError):  # groups are not learned-decoded
+            dec.decode_batch_nms(fr, 1, 1, 1, 1, 10)
+    finally:
+        dec.set_nms(code, None, None)
