@@ -204,6 +204,71 @@ def config1(dec, reps=20):
     return res
 
 
+def config4(dec, nf=8192, db=0.2, reps=3):
+    """Learned vs plain decoding on the two trained MODCODs (DVB-S2 normal
+    3/5 8PSK and 2/3 16APSK) at Es/N0 = QEF + db: frames from pool
+    codewords through the GPU channel, decoded by the plain decoder (up to
+    50 iterations) and by the learned table (up to 30 learned iterations,
+    plain fallback up to 50); frame errors after the BCH code's t."""
+    from nms_train import DevBuf, codeword_pool
+
+    out = {}
+    for name in ("s2_n3_5_8psk", "s2_n2_3_16apsk"):
+        table = json.load(open(os.path.join(ROOT, "dvb-rx_amd", "nms", name + ".json")))
+        pls, short, rate = table["pls"], table["short_frame"], table["rate"]
+        code = ldpcgpu.code_lookup(short, rate)
+        fec = ldpcgpu.fec_lookup(short, rate)
+        N, kl = fec.nldpc, fec.kldpc
+        kb = (kl + 7) // 8
+        pool = codeword_pool(short, rate, 512, 5)
+        ref = np.packbits(pool[:, :kl], axis=1)
+        rows = np.random.default_rng(6).integers(0, 512, nf).astype(np.int32)
+        dec.set_nms(code, np.array(table["S"], np.uint8), np.array(table["B"], np.int16))
+        dv = DevBuf(dec)
+        rec = {"esn0_db": round(ldpcgpu.qef_db(pls) + db, 3), "frames": nf}
+        try:
+            d_pool = dv.put(pool)
+            d_rows = dv.put(rows)
+            d_e = dv.put(np.full(nf, ldpcgpu.qef_db(pls) + db, np.float32))
+            d_llr = dv.empty(nf * N)
+            ldpcgpu.channel(pls, N, d_pool, d_rows, nf, d_e, 77, d_llr, stream=dec.stream)
+            dec.synchronize()
+            frames = np.zeros(nf, ldpcgpu.FRAME_DTYPE)
+            frames["code"] = code
+            frames["nbits"] = kl
+            frames["group"] = -1
+            frames["llr_off"] = np.arange(nf) * N
+            frames["out_off"] = np.arange(nf) * kb
+            d_out, d_it, d_ok, d_fb = dv.empty(nf * kb), dv.empty(4 * nf), dv.empty(nf), dv.empty(nf)
+            for mode in ("plain", "learned"):
+                fr = frames.copy()
+                fr["max_iter"] = 50 if mode == "plain" else 30
+                dt = []
+                for _ in range(reps):
+                    dec.synchronize()
+                    t = time.perf_counter()
+                    if mode == "plain":
+                        dec.decode_batch(fr, d_llr, d_out, d_it, d_ok)
+                    else:
+                        dec.decode_batch_nms(fr, d_llr, d_out, d_it, d_ok, 50, d_fb)
+                    dec.synchronize()
+                    dt.append(time.perf_counter() - t)
+                bits = dv.get(d_out, (nf, kb), np.uint8)
+                errs = np.unpackbits(bits ^ ref[rows], axis=1)[:, :kl].sum(axis=1)
+                it = dv.get(d_it, nf, np.int32)
+                sec = float(np.median(dt))
+                rec[mode] = {"gbit_s": round(nf * kl / sec / 1e9, 3), "ms": round(sec * 1e3, 3),
+                             "avg_iterations": round(float(it.mean()), 3),
+                             "fer_after_bch": float((errs > fec.t).mean()), "fer_ldpc": float((errs > 0).mean())}
+                if mode == "learned":
+                    rec[mode]["fallback_share"] = float(dv.get(d_fb, nf, np.uint8).mean())
+        finally:
+            dv.free()
+            dec.set_nms(code, None, None)
+        out[name] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,7 +277,8 @@ def main():
     ap.add_argument("--frames", type=int, default=8192)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-config1", action="store_true", help="skip the 64-frame rate-1/2 side measurement")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the side measurements (64-frame rate 1/2 batch, learned vs plain)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -319,6 +385,7 @@ def main():
             traffic = None
 
     c1 = None if args.no_config1 else config1(dec)
+    c4 = None if args.no_config1 else config4(dec)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(batch, args.cpu_seconds)
@@ -374,6 +441,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "config1": c1,
+        "learned_vs_plain": c4,
     }
     if cpu:
         res["gpu_vs_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None

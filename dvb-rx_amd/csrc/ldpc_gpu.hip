@@ -384,7 +384,7 @@ struct PkArgs {
      * B = 1), per code its first row and row count, plain fallback cap,
      * per-frame fallback flag (may be NULL) */
     const uint8_t* nS;
-    const int8_t* nB;
+    const int16_t* nB;
     const int* nrow0;
     const int* nT;
     int fallback;
@@ -395,8 +395,8 @@ struct PkArgs {
 struct PkRowPtrs {
     const uint8_t* S0;
     const uint8_t* S1;
-    const int8_t* B0;
-    const int8_t* B1;
+    const int16_t* B0;
+    const int16_t* B1;
 };
 
 /* AND of a 2-bit value over the workgroup, one barrier: three vote words
@@ -533,7 +533,7 @@ __device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, 
         const int nlev = lay_nlev(ly);
         /* learned scale / offset of the layer's check degree, per slot;
          * node (0, 0) has no previous parity, one edge less */
-        lg_pk ps = pk_dup(16), pb = pk_dup(1);
+        lg_pk ps = pk_dup(16), pb = pk_dup(16);
         if constexpr (NMS) {
             const int dc = lay_deg(ly) + 2 - (i == 0 && j == 0);
             ps = lg_pk{ (short)rows.S0[dc], (short)rows.S1[dc] };
@@ -801,11 +801,11 @@ struct ldpc_gpu_ctx {
     int force_scalar;        /* per-frame batches through the one-frame kernel */
     /* learned tables: row 0 plain (S = 16, B = 1), then per code its rows */
     uint8_t* h_nS;
-    int8_t* h_nB;
+    int16_t* h_nB;
     int* h_nrow0;
     int* h_nT;
     uint8_t* d_nS;
-    int8_t* d_nB;
+    int16_t* d_nB;
     int* d_nrow0;
     int* d_nT;
     int nms_dirty;
@@ -1379,7 +1379,7 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
 
 #define NMS_ROWS (1 + 64 * LDPC_NMS_MAX_ITERS) /* row 0 plain, then per code LDPC_NMS_MAX_ITERS rows */
 
-extern "C" int ldpc_gpu_set_nms(ldpc_gpu_ctx* c, int code, int T, const uint8_t* S, const int8_t* B)
+extern "C" int ldpc_gpu_set_nms(ldpc_gpu_ctx* c, int code, int T, const uint8_t* S, const int16_t* B)
 {
     if (!c)
         return LDPC_GPU_EINVAL;
@@ -1388,16 +1388,17 @@ extern "C" int ldpc_gpu_set_nms(ldpc_gpu_ctx* c, int code, int T, const uint8_t*
         return set_err(c, LDPC_GPU_EINVAL, "set_nms: bad arguments (code %d, T %d)", code, T);
     if (!c->h_nS) {
         c->h_nS = (uint8_t*)malloc(NMS_ROWS * LDPC_NMS_DEGS);
-        c->h_nB = (int8_t*)malloc(NMS_ROWS * LDPC_NMS_DEGS);
+        c->h_nB = (int16_t*)malloc(NMS_ROWS * LDPC_NMS_DEGS * sizeof(int16_t));
         c->h_nrow0 = (int*)calloc(64, sizeof(int));
         c->h_nT = (int*)calloc(64, sizeof(int));
         if (!c->h_nS || !c->h_nB || !c->h_nrow0 || !c->h_nT)
             return set_err(c, LDPC_GPU_ENOMEM, "set_nms: host tables");
         memset(c->h_nS, 16, NMS_ROWS * LDPC_NMS_DEGS);
-        memset(c->h_nB, 1, NMS_ROWS * LDPC_NMS_DEGS);
+        for (int k = 0; k < NMS_ROWS * LDPC_NMS_DEGS; k++)
+            c->h_nB[k] = 16;
         HIPCHK(c, hipSetDevice(c->device));
         HIPCHK(c, hipMalloc((void**)&c->d_nS, NMS_ROWS * LDPC_NMS_DEGS));
-        HIPCHK(c, hipMalloc((void**)&c->d_nB, NMS_ROWS * LDPC_NMS_DEGS));
+        HIPCHK(c, hipMalloc((void**)&c->d_nB, NMS_ROWS * LDPC_NMS_DEGS * sizeof(int16_t)));
         HIPCHK(c, hipMalloc((void**)&c->d_nrow0, 64 * sizeof(int)));
         HIPCHK(c, hipMalloc((void**)&c->d_nT, 64 * sizeof(int)));
     }
@@ -1406,7 +1407,11 @@ extern "C" int ldpc_gpu_set_nms(ldpc_gpu_ctx* c, int code, int T, const uint8_t*
     c->h_nT[code] = T;
     if (T > 0) {
         memcpy(c->h_nS + (size_t)r0 * LDPC_NMS_DEGS, S, (size_t)T * LDPC_NMS_DEGS);
-        memcpy(c->h_nB + (size_t)r0 * LDPC_NMS_DEGS, B, (size_t)T * LDPC_NMS_DEGS);
+        for (int k = 0; k < T * LDPC_NMS_DEGS; k++) {
+            if (S[k] > 63 || B[k] < -1024 || B[k] > 1023)
+                return set_err(c, LDPC_GPU_EINVAL, "set_nms: entry %d out of range (S 0..63, B -1024..1023)", k);
+        }
+        memcpy(c->h_nB + (size_t)r0 * LDPC_NMS_DEGS, B, (size_t)T * LDPC_NMS_DEGS * sizeof(int16_t));
     }
     c->nms_dirty = 1;
     return LDPC_GPU_OK;
@@ -1462,7 +1467,8 @@ extern "C" int ldpc_gpu_decode_batch_nms(ldpc_gpu_ctx* c, const ldpc_gpu_frame* 
             return set_err(c, LDPC_GPU_EINVAL, "frame %d: no learned table for code %d", f, df[f].code);
     if (c->nms_dirty) {
         HIPCHK(c, hipMemcpyAsync(c->d_nS, c->h_nS, NMS_ROWS * LDPC_NMS_DEGS, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_nB, c->h_nB, NMS_ROWS * LDPC_NMS_DEGS, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_nB, c->h_nB, NMS_ROWS * LDPC_NMS_DEGS * sizeof(int16_t), hipMemcpyHostToDevice,
+                                 c->stream));
         HIPCHK(c, hipMemcpyAsync(c->d_nrow0, c->h_nrow0, 64 * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->d_nT, c->h_nT, 64 * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -4,7 +4,7 @@
  *
  * Training model.  The forward pass is the integer layered decoder of the
  * reference (libdvbrx/src/ldpc.c:321-362, :465-497) with the check-node
- * magnitude replaced by clamp(((min(m, 127) * S) >> 4) - B, 0, 127), run
+ * magnitude replaced by clamp((min(m, 127) * S - B) >> 4, 0, 127), run
  * for T iterations without early stop.  The backward pass differentiates
  * the float model mag = clamp(alpha * min(m, 127) - beta, 0, 127) at the
  * forward's values (straight-through): the gradient of o_e reaches
@@ -76,7 +76,7 @@ __device__ __forceinline__ int cw_pos(int v, int K, int q)
 
 __device__ __forceinline__ int nms_mag(int m, int S, int B)
 {
-    const int v = ((min(m, 127) * S) >> 4) - B;
+    const int v = (min(m, 127) * S - B) >> 4;
     return v < 0 ? 0 : v > 127 ? 127 : v;
 }
 
@@ -226,9 +226,9 @@ __global__ void __launch_bounds__(NT) nms_train_kernel(NmsArgs a)
     const int tid = threadIdx.x;
     for (int k = tid; k < np; k += NT) {
         const float al = a.params[k], be = a.params[np + k];
-        const float rs = rintf(al * 16.0f), rb = rintf(be);
-        sS[k] = rs < 0.0f ? 0 : rs > 255.0f ? 255 : (int)rs;
-        sB[k] = rb < -128.0f ? -128 : rb > 127.0f ? 127 : (int)rb;
+        const float rs = rintf(al * 16.0f), rb = rintf(be * 16.0f);
+        sS[k] = rs < 0.0f ? 0 : rs > 63.0f ? 63 : (int)rs;
+        sB[k] = rb < -1024.0f ? -1024 : rb > 1023.0f ? 1023 : (int)rb;
         sA[k] = al;
         sBe[k] = be;
     }
@@ -330,6 +330,7 @@ __global__ void nms_reduce_kernel(const float* part, int nwg, int n, float* grad
         *loss = s;
 }
 
+/* Adam, then the projection onto the parameter box (alpha: first half) */
 __global__ void nms_adam_kernel(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2,
                                 float eps, float c1, float c2)
 {
@@ -341,7 +342,8 @@ __global__ void nms_adam_kernel(float* p, const float* g, float* m, float* v, in
     const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
     m[k] = mk;
     v[k] = vk;
-    p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    const float q = p[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    p[k] = k < n / 2 ? fminf(fmaxf(q, 0.25f), 1.5f) : fminf(fmaxf(q, -8.0f), 16.0f);
 }
 
 /* ---- channel ------------------------------------------------------------ */
@@ -461,12 +463,12 @@ static int terr(ldpc_nms_trainer* t, int code, const char* fmt, ...)
 
 extern "C" {
 
-void ldpc_nms_quantize(const float* alpha, const float* beta, int T, uint8_t* S, int8_t* B)
+void ldpc_nms_quantize(const float* alpha, const float* beta, int T, uint8_t* S, int16_t* B)
 {
     for (int k = 0; k < T * DEGS; k++) {
-        const float rs = rintf(alpha[k] * 16.0f), rb = rintf(beta[k]);
-        S[k] = (uint8_t)(rs < 0.0f ? 0 : rs > 255.0f ? 255 : (int)rs);
-        B[k] = (int8_t)(rb < -128.0f ? -128 : rb > 127.0f ? 127 : (int)rb);
+        const float rs = rintf(alpha[k] * 16.0f), rb = rintf(beta[k] * 16.0f);
+        S[k] = (uint8_t)(rs < 0.0f ? 0 : rs > 63.0f ? 63 : (int)rs);
+        B[k] = (int16_t)(rb < -1024.0f ? -1024 : rb > 1023.0f ? 1023 : (int)rb);
     }
 }
 

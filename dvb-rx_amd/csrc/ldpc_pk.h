@@ -174,14 +174,15 @@ LG_PK_HD PkOut<DEG> pk_mags(const PkNode<DEG>& n)
     return PkOut<DEG>{ mn1, mn0 - mn1 };
 }
 
-/* Learned magnitude of one minimum, per lane: clamp(((min(m, 127) * S) >>
- * 4) - B, 0, 127) in units of 256, S (0..255) and B (-128..127) in the
- * lanes of ps / pb.  S = 16, B = 1 gives pk_mags' values. */
+/* Learned magnitude of one minimum, per lane: clamp((min(m, 127) * S - B)
+ * >> 4, 0, 127) in units of 256, S (0..63) and B (-1024..1023) in the
+ * lanes of ps / pb, so |min * S - B| < 2^15.  S = 16, B = 16 gives
+ * pk_mags' values. */
 LG_PK_HD lg_pk pk_nms_mag(lg_pk m, lg_pk ps, lg_pk pb)
 {
     const lg_pku v = pku(pk_minu(m, pk_dup(0x7F00))) >> (unsigned short)8;
-    const lg_pk p = pks((lg_pku)(v * pku(ps)) >> (unsigned short)4);
-    return pk_min(pk_max(p - pb, pk_dup(0)), pk_dup(127)) << (short)8;
+    const lg_pk p = (pks(v * pku(ps)) - pb) >> (short)4;
+    return pk_min(pk_max(p, pk_dup(0)), pk_dup(127)) << (short)8;
 }
 
 template <int DEG, bool NMS>
@@ -228,7 +229,7 @@ LG_PK_HD uint16_t pk_post_bytes(lg_pk y) { return (uint16_t)lg_perm(0, pk_bits(y
 
 template <int DEG, bool NMS = false>
 LG_PK_HD void pk_write_succ(const PkNode<DEG>& n, uint16_t* post, uint32_t succ, lg_pk ps = lg_pk{ 16, 16 },
-                            lg_pk pb = lg_pk{ 1, 1 })
+                            lg_pk pb = lg_pk{ 16, 16 })
 {
     const PkOut<DEG> m = pk_mags_p<DEG, NMS>(n, ps, pb);
 #pragma unroll
@@ -241,7 +242,7 @@ LG_PK_HD void pk_write_succ(const PkNode<DEG>& n, uint16_t* post, uint32_t succ,
  * clamped to [-32, 31] (ldpc.c:343) */
 template <int DEG, bool TAIL, bool NMS = false>
 LG_PK_HD void pk_phase_c(const PkNode<DEG>& n, uint16_t* post, uint32_t succ, uint32_t* mw_out,
-                         lg_pk ps = lg_pk{ 16, 16 }, lg_pk pb = lg_pk{ 1, 1 })
+                         lg_pk ps = lg_pk{ 16, 16 }, lg_pk pb = lg_pk{ 16, 16 })
 {
     const PkOut<DEG> m = pk_mags_p<DEG, NMS>(n, ps, pb);
     lg_pk mo[DEG + 2];

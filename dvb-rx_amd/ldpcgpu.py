@@ -291,7 +291,7 @@ class Decoder:
             self._check(self.L.ldpc_gpu_set_nms(self.ctx, code, 0, None, None), "set_nms")
             return
         S = np.ascontiguousarray(S, np.uint8)
-        B = np.ascontiguousarray(B, np.int8)
+        B = np.ascontiguousarray(B, np.int16)
         assert S.shape == B.shape and S.shape[1] == NMS_DEGS
         self._check(self.L.ldpc_gpu_set_nms(self.ctx, code, S.shape[0], S.ctypes.data, B.ctypes.data), "set_nms")
 
@@ -338,7 +338,7 @@ def nms_quantize(alpha, beta):
     beta = np.ascontiguousarray(beta, np.float32)
     assert alpha.shape == beta.shape and alpha.shape[1] == NMS_DEGS
     S = np.zeros(alpha.shape, np.uint8)
-    B = np.zeros(alpha.shape, np.int8)
+    B = np.zeros(alpha.shape, np.int16)
     gpu_lib().ldpc_nms_quantize(alpha.ctypes.data, beta.ctypes.data, alpha.shape[0], S.ctypes.data, B.ctypes.data)
     return S, B
 

@@ -94,10 +94,11 @@ int ldpc_gpu_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nf
  * Learned offset min-sum decoding (no counterpart in the reference, whose
  * offset is fixed at 1, libdvbrx/src/ldpc.c:353; the tables come from
  * include/ldpc_nms.h's trainer).  ldpc_gpu_set_nms stores, for one code,
- * T rows (T <= LDPC_NMS_MAX_ITERS) of LDPC_NMS_DEGS scales S (Q4, 16 =
- * 1.0) and offsets B indexed by check degree; the check node's magnitude
- * becomes clamp(((min(m, 127) * S[t][dc]) >> 4) - B[t][dc], 0, 127) at
- * iteration t (row min(t, T-1)).  T = 0 removes the table.
+ * T rows (T <= LDPC_NMS_MAX_ITERS) of LDPC_NMS_DEGS scales S (0..63) and
+ * offsets B (-1024..1023), both in 1/16 units, indexed by check degree;
+ * the check node's magnitude becomes
+ * clamp((min(m, 127) * S[t][dc] - B[t][dc]) >> 4, 0, 127) at iteration t
+ * (row min(t, T-1)).  T = 0 removes the table.
  *
  * ldpc_gpu_decode_batch_nms decodes frames (group -1 only) with their
  * code's table for up to frames[f].max_iter learned iterations; a frame
@@ -107,7 +108,7 @@ int ldpc_gpu_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nf
  * be NULL) = 1 where the fallback ran.  Enqueues only, as
  * ldpc_gpu_decode_batch.
  */
-int ldpc_gpu_set_nms(ldpc_gpu_ctx* ctx, int code, int T, const uint8_t* S, const int8_t* B);
+int ldpc_gpu_set_nms(ldpc_gpu_ctx* ctx, int code, int T, const uint8_t* S, const int16_t* B);
 int ldpc_gpu_decode_batch_nms(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
                               uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, int fallback_max_iter,
                               uint8_t* d_fallback);

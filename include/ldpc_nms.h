@@ -6,9 +6,10 @@
  * offset of 1 (libdvbrx/src/ldpc.c:353).  The learned form replaces that
  * magnitude, per iteration t and check degree dc, by
  *     float (training):  clamp(alpha[t][dc] * min(m, 127) - beta[t][dc], 0, 127)
- *     integer (decoding): clamp(((min(m, 127) * S[t][dc]) >> 4) - B[t][dc], 0, 127)
- * with S = clamp(rint(16 alpha), 0, 255), B = clamp(rint(beta), -128, 127);
- * S = 16, B = 1 is the reference.  Everything else (int8 saturation,
+ *     integer (decoding): clamp((min(m, 127) * S[t][dc] - B[t][dc]) >> 4, 0, 127)
+ * (arithmetic shift) with S = clamp(rint(16 alpha), 0, 63) and
+ * B = clamp(rint(16 beta), -1024, 1023): scale and offset in 1/16 units;
+ * S = 16, B = 16 is the reference.  Everything else (int8 saturation,
  * messages clamped to [-32, 31], layer order) is the reference's.
  *
  * Training uses the integer forward with the quantised parameters and the
@@ -57,12 +58,15 @@ int ldpc_nms_train_step(ldpc_nms_trainer* t, const int8_t* d_llr, const uint8_t*
                         int nframes, const float* d_params, const float* d_w, float temp, float* d_loss,
                         float* d_grad, int8_t* d_post);
 
-/* Adam (Kingma & Ba) on n parameters: m, v moments, step >= 1. */
+/* Adam (Kingma & Ba) on n parameters: m, v moments, step >= 1.  The
+ * first n/2 parameters (alpha) are then kept in [0.25, 1.5], the others
+ * (beta) in [-8, 16]: min-sum overestimates the magnitudes, a learned
+ * correction scales them down or offsets them. */
 int ldpc_nms_adam(ldpc_nms_trainer* t, float* d_param, const float* d_grad, float* d_m, float* d_v, int n,
                   float lr, float beta1, float beta2, float eps, int step);
 
 /* Quantised integer table of float parameters (host arrays, T rows). */
-void ldpc_nms_quantize(const float* alpha, const float* beta, int T, uint8_t* S, int8_t* B);
+void ldpc_nms_quantize(const float* alpha, const float* beta, int T, uint8_t* S, int16_t* B);
 
 /*
  * GPU channel for training and evaluation data: codeword rows (one byte

@@ -320,19 +320,20 @@ void oracle_ldpc_run(oracle_ldpc* d, const int8_t* llr, int nframes, int T, int8
 int oracle_nms_scale_q(float alpha)
 {
     const float r = rintf(alpha * 16.0f);
-    return r < 0.0f ? 0 : r > 255.0f ? 255 : (int)r;
+    return r < 0.0f ? 0 : r > 63.0f ? 63 : (int)r;
 }
 
 int oracle_nms_offset_q(float beta)
 {
-    const float r = rintf(beta);
-    return r < -128.0f ? -128 : r > 127.0f ? 127 : (int)r;
+    const float r = rintf(beta * 16.0f);
+    return r < -1024.0f ? -1024 : r > 1023.0f ? 1023 : (int)r;
 }
 
-/* learned magnitude; S = 16, B = 1 gives subsu(minu(m, 127), 1) */
+/* learned magnitude (the shift rounds toward minus infinity); S = 16,
+ * B = 16 gives subsu(minu(m, 127), 1) */
 static inline int8_t nms_mag(uint8_t m, int S, int B)
 {
-    const int v = (((int)minu(m, 127) * S) >> 4) - B;
+    const int v = ((int)minu(m, 127) * S - B) >> 4;
     return (int8_t)(v < 0 ? 0 : v > 127 ? 127 : v);
 }
 
@@ -375,7 +376,7 @@ void oracle_nms_check_node(int deg, const int8_t* L, const int8_t* m, int first,
 }
 
 /* one learned iteration of one frame (lanes = 1), parameter row t */
-static void update_nms(oracle_ldpc* d, int first, const uint8_t* S, const int8_t* B, int t)
+static void update_nms(oracle_ldpc* d, int first, const uint8_t* S, const int16_t* B, int t)
 {
     int8_t* bl = d->bnl;
     int8_t* v[MAX_DEG];
@@ -397,7 +398,7 @@ static void update_nms(oracle_ldpc* d, int first, const uint8_t* S, const int8_t
     }
 }
 
-void oracle_nms_decode_frames(oracle_ldpc* d, const uint8_t* S, const int8_t* B, int T, const int8_t* llr,
+void oracle_nms_decode_frames(oracle_ldpc* d, const uint8_t* S, const int16_t* B, int T, const int8_t* llr,
                               uint8_t* out, int nframes, int learned_max, int fallback_max, int32_t* iters,
                               uint8_t* ok, uint8_t* fell_back)
 {

@@ -52,8 +52,9 @@ void oracle_ldpc_run(oracle_ldpc* d, const int8_t* llr, int nframes, int T, int8
  * Parameters are per iteration t and per check degree dc (the node's edge
  * count, row index 0..ORACLE_NMS_DEGS-1); iterations past the table's T
  * rows use row T-1.  Integer form: the check node's magnitudes become
- *     mn = clamp(((min(m, 127) * S[t][dc]) >> 4) - B[t][dc], 0, 127)
- * in place of max(min(m, 127) - 1, 0); S = 16, B = 1 is the reference.
+ *     mn = clamp((min(m, 127) * S[t][dc] - B[t][dc]) >> 4, 0, 127)
+ * (scale and offset in 1/16 units, arithmetic shift) in place of
+ * max(min(m, 127) - 1, 0); S = 16, B = 16 is the reference.
  * Float form (training): mag = clamp(alpha * min(mu, 127) - beta, 0, 127),
  * everything else as the integer decoder (saturation to [-128, 127],
  * stored messages clamped to [-32, 31]).
@@ -61,7 +62,7 @@ void oracle_ldpc_run(oracle_ldpc* d, const int8_t* llr, int nframes, int T, int8
 #define ORACLE_NMS_DEGS 33
 
 /* Quantisation of a float (alpha, beta) pair: S = clamp(rint(16 alpha), 0,
- * 255), B = clamp(rint(beta), -128, 127). */
+ * 63), B = clamp(rint(16 beta), -1024, 1023). */
 int oracle_nms_scale_q(float alpha);
 int oracle_nms_offset_q(float beta);
 
@@ -74,7 +75,7 @@ void oracle_nms_check_node(int deg, const int8_t* L, const int8_t* m, int first,
  * channel LLRs by the plain decoder for up to fallback_max iterations
  * (0: no fallback).  iters[f] = learned + plain iterations, ok[f] the
  * final syndrome, fell_back[f] (may be NULL) 1 if the fallback ran. */
-void oracle_nms_decode_frames(oracle_ldpc* d, const uint8_t* S, const int8_t* B, int T, const int8_t* llr,
+void oracle_nms_decode_frames(oracle_ldpc* d, const uint8_t* S, const int16_t* B, int T, const int8_t* llr,
                               uint8_t* out, int nframes, int learned_max, int fallback_max, int32_t* iters,
                               uint8_t* ok, uint8_t* fell_back);
 

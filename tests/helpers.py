@@ -108,7 +108,7 @@ class Oracle:
     def nms_frames(self, S, B, llr, learned_max, fallback_max):
         """Learned decode per frame: (bits, iters, ok, fell_back)."""
         S = np.ascontiguousarray(S, np.uint8)
-        B = np.ascontiguousarray(B, np.int8)
+        B = np.ascontiguousarray(B, np.int16)
         assert S.shape == B.shape and S.shape[1] == NMS_DEGS
         llr = np.ascontiguousarray(llr, np.int8)
         n = llr.shape[0]

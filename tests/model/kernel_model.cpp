@@ -142,7 +142,7 @@ int kmodel_decode(int code, const int8_t* llr, int max_iter, int nbits, uint8_t*
  * check degree; NULL rows: the plain decoder's arithmetic (pk_mags) */
 struct PkRows {
     const uint8_t* S[2];
-    const int8_t* B[2];
+    const int16_t* B[2];
 };
 
 template <int DEG, bool NMS>
@@ -161,7 +161,7 @@ static void pk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
         for (int w = 0; w < W; w++)
             mw[j][w] = m[w * 360 + j];
         ps[j] = lg_pk{ 16, 16 };
-        pb[j] = lg_pk{ 1, 1 };
+        pb[j] = lg_pk{ 16, 16 };
         if (NMS) {
             const int dc = DEG + 2 - (i == 0 && j == 0);
             ps[j] = lg_pk{ (short)rows.S[0][dc], (short)rows.S[1][dc] };
@@ -233,7 +233,7 @@ static uint32_t pk_check_any(const lg_code* c, int i, int j, const uint16_t* pos
  * decoded by the plain arithmetic for up to fallback iterations, as the
  * kernel's learned mode does. */
 static int pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay, int max_iter, int nbits,
-                   const uint8_t* S, const int8_t* B, int T, int fallback, uint8_t* out_a, uint8_t* out_b,
+                   const uint8_t* S, const int16_t* B, int T, int fallback, uint8_t* out_a, uint8_t* out_b,
                    int* it_out, int* ok_out, int* fb_out)
 {
     const lg_code* c = lg_code_get(code);
@@ -248,10 +248,10 @@ static int pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay
         return -1;
     }
     static uint8_t plainS[33];
-    static int8_t plainB[33];
+    static int16_t plainB[33];
     for (int d = 0; d < 33; d++) {
         plainS[d] = 16;
-        plainB[d] = 1;
+        plainB[d] = 16;
     }
     const int8_t* src[2] = { llr_a, llr_b };
     uint8_t* dst[2] = { out_a, out_b };
@@ -333,7 +333,7 @@ extern "C" int kmodel_pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b
 }
 
 extern "C" int kmodel_pk_pair_nms(int code, const int8_t* llr_a, const int8_t* llr_b, int delay, const uint8_t* S,
-                                  const int8_t* B, int T, int max_iter, int fallback, int nbits, uint8_t* out_a,
+                                  const int16_t* B, int T, int max_iter, int fallback, int nbits, uint8_t* out_a,
                                   uint8_t* out_b, int* it_out, int* ok_out, int* fb_out)
 {
     return pk_pair(code, llr_a, llr_b, delay, max_iter, nbits, S, B, T, fallback, out_a, out_b, it_out, ok_out,

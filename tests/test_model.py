@@ -86,7 +86,7 @@ def pk_pair_nms(code, a, b, delay, S, B, max_iter, fallback, nbits):
     a = np.ascontiguousarray(a, np.int8)
     b = np.ascontiguousarray(b, np.int8)
     S = np.ascontiguousarray(S, np.uint8)
-    B = np.ascontiguousarray(B, np.int8)
+    B = np.ascontiguousarray(B, np.int16)
     kb = (nbits + 7) // 8
     oa, ob = np.zeros(kb, np.uint8), np.zeros(kb, np.uint8)
     it, ok, fb = np.zeros(2, np.int32), np.zeros(2, np.int32), np.zeros(2, np.int32)
@@ -108,7 +108,7 @@ def test_packed_pair_learned_model(short, rate, delay, esn0):
     T = 6
     rng = np.random.default_rng(delay)
     S = rng.integers(8, 20, (T, 33)).astype(np.uint8)
-    B = rng.integers(-1, 3, (T, 33)).astype(np.int8)
+    B = rng.integers(-16, 48, (T, 33)).astype(np.int16)
     o = Oracle(short, rate, fec.kldpc)
     for learned_max, fallback in ((T, 30), (3, 0), (8, 30)):
         bits, it, ok, fb = o.nms_frames(S, B, llr, learned_max, fallback)

@@ -27,7 +27,7 @@ def textbook_nms_node(L, m, first, S, B):
     for e in range(d):
         others = [x[f] for f in range(d) if f != e]
         mn = min(min(abs(v) for v in others), 127)
-        mag = max(0, min(127, (mn * S >> 4) - B))
+        mag = max(0, min(127, (mn * S - B) >> 4))  # Python's >> floors, as the arithmetic shift
         o = -mag if sum(1 for v in others if v < 0) % 2 else mag
         Lo.append(sat8(x[e] + o))
         mo.append(max(-32, min(31, o)))
@@ -50,24 +50,24 @@ def test_learned_node_definition(deg):
     for trial in range(200):
         L = rng.integers(-128, 128, deg)
         m = rng.integers(-32, 32, deg)
-        S = int(rng.integers(0, 256))
-        B = int(rng.integers(-8, 9))
+        S = int(rng.integers(0, 64))
+        B = int(rng.integers(-1024, 1024)) if trial % 3 == 0 else int(rng.integers(-40, 80))
         first = trial % 4 == 0
         got = nms_node(deg, L, m, first, S, B)
         want = textbook_nms_node(L, m, first, S, B)
         assert (got[0] == want[0]).all() and (got[1] == want[1]).all(), (L, m, S, B)
         if trial % 5 == 0:
             ref = check_node(deg, L, m, first)
-            one = nms_node(deg, L, m, first, 16, 1)
+            one = nms_node(deg, L, m, first, 16, 16)
             assert (ref[0] == one[0]).all() and (ref[1] == one[1]).all()
 
 
 def test_quantiser():
     L = oracle()
-    assert L.oracle_nms_scale_q(1.0) == 16 and L.oracle_nms_offset_q(1.0) == 1
-    assert L.oracle_nms_scale_q(0.78) == 12 and L.oracle_nms_scale_q(-1.0) == 0 and L.oracle_nms_scale_q(40.0) == 255
-    assert L.oracle_nms_offset_q(0.5) == 0 and L.oracle_nms_offset_q(1.5) == 2  # round half to even
-    assert L.oracle_nms_offset_q(-300.0) == -128
+    assert L.oracle_nms_scale_q(1.0) == 16 and L.oracle_nms_offset_q(1.0) == 16
+    assert L.oracle_nms_scale_q(0.78) == 12 and L.oracle_nms_scale_q(-1.0) == 0 and L.oracle_nms_scale_q(40.0) == 63
+    assert L.oracle_nms_offset_q(0.5 / 16) == 0 and L.oracle_nms_offset_q(1.5 / 16) == 2  # round half to even
+    assert L.oracle_nms_offset_q(-300.0) == -1024 and L.oracle_nms_offset_q(0.7) == 11
 
 
 def _frames(short, rate, esn0, n, seed):
@@ -98,7 +98,7 @@ def test_quantised_forward_is_the_integer_learned_decoder():
     alpha = rng.uniform(0.6, 1.2, (T, NMS_DEGS)).astype(np.float32)
     beta = rng.uniform(-0.5, 2.5, (T, NMS_DEGS)).astype(np.float32)
     S = np.array([[o.L.oracle_nms_scale_q(float(a)) for a in r] for r in alpha], np.uint8)
-    B = np.array([[o.L.oracle_nms_offset_q(float(b)) for b in r] for r in beta], np.int8)
+    B = np.array([[o.L.oracle_nms_offset_q(float(b)) for b in r] for r in beta], np.int16)
     bits, it, ok, fb = o.nms_frames(S, B, llr, T, 0)
     assert (it == T).all() and not ok.any() and not fb.any()
     _, _, _, post = o.nms_train(1, llr.astype(np.float32), cw, alpha, beta, np.ones(T), grad=False)
@@ -162,7 +162,7 @@ def test_learned_decode_with_fallback():
     fec, llr, _ = gen(0, "1_2", 1.5, 8, 74)
     o = Oracle(0, "1_2", fec.kldpc)
     S = np.full((10, NMS_DEGS), 16, np.uint8)
-    B = np.full((10, NMS_DEGS), 1, np.int8)
+    B = np.full((10, NMS_DEGS), 16, np.int16)
     # unit parameters + fallback: the learned phase is the plain decoder,
     # so a frame either converges within 10 or restarts and runs plain
     bits, it, ok, fb = o.nms_frames(S, B, llr, 10, 50)
