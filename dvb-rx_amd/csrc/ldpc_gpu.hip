@@ -40,6 +40,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "ldpc_codes.h"
@@ -811,6 +812,32 @@ struct ldpc_gpu_ctx {
     int nms_dirty;
 };
 
+/* A prepared batch (ldpc_gpu_plan_*): its own copies of the frame table
+ * and launch layout, its own scratch, and the launches captured as one
+ * graph; variants of the launch shape for the autotuner. */
+struct ldpc_gpu_plan {
+    ldpc_gpu_ctx* c;
+    int nframes;
+    std::vector<DFrame> hf;
+    const int8_t* d_llr;
+    uint8_t* d_out;
+    int32_t* d_iters;
+    uint8_t* d_synd;
+    DFrame* d_frames;
+    int* d_list;
+    int2* d_seg;
+    unsigned* d_cnt;
+    uint32_t* d_msg;
+    size_t list_cap, seg_cap, msg_cap;
+    hipStream_t aux[NLAUNCH];
+    hipEvent_t fork, join[NLAUNCH], t0, t1;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    int variant;
+    float variant_ms[8];
+    int launched;
+};
+
 static int set_err(ldpc_gpu_ctx* c, int code, const char* fmt, ...)
 {
     if (c) {
@@ -1254,17 +1281,26 @@ static int pk_grid(ldpc_gpu_ctx* c, size_t lds, int nframes, int* grid)
     return LDPC_GPU_OK;
 }
 
-/* Frames with their own early stop: two per workgroup (ldpc_pk_kernel).
- * One launch per (degree class, frame size), each over segments of frames
- * of one code. */
-static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFrame>& hf, const int8_t* d_llr,
-                     uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, bool nms = false, int fallback = 0,
-                     uint8_t* d_fell = nullptr)
+/* Launch layout of frames with their own early stop (two per workgroup,
+ * ldpc_pk_kernel): one launch per (degree class, frame size), frames
+ * grouped by code into segments of one list, persistent grid per launch
+ * (occupancy x CUs, scaled by grid_scale). */
+struct PkLayout {
+    std::vector<int> list;
+    std::vector<int2> segs;
+    int seg0[NLAUNCH], nseg[NLAUNCH], nfr[NLAUNCH], grid[NLAUNCH], maxn[NLAUNCH];
+    int maxw;
+    size_t blocks; /* message blocks: sum of the grids */
+};
+
+static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, float grid_scale, PkLayout& L)
 {
     const std::vector<DCode>& hc = c->tab->h_codes;
     std::vector<int> gcodes[NLAUNCH];
     std::vector<std::vector<int>> gseg[NLAUNCH];
-    int maxn[NLAUNCH] = { 0 }, maxw = 0;
+    L.maxw = 0;
+    for (int k = 0; k < NLAUNCH; k++)
+        L.maxn[k] = 0;
     for (int f = 0; f < (int)hf.size(); f++) {
         const DCode& cd = hc[hf[f].code];
         const int k = class_of(cd) * 2 + (cd.n < 64800);
@@ -1275,45 +1311,94 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
             gseg[k].emplace_back();
         }
         gseg[k][g].push_back(f);
-        maxn[k] = std::max(maxn[k], cd.n);
-        maxw = std::max(maxw, cd.pk_msg_words);
+        L.maxn[k] = std::max(L.maxn[k], cd.n);
+        L.maxw = std::max(L.maxw, cd.pk_msg_words);
     }
-    std::vector<int> list;
-    std::vector<int2> segs;
-    int seg0[NLAUNCH], nseg[NLAUNCH], nfr[NLAUNCH], grid[NLAUNCH] = { 0 }, maxgrid = 0;
+    L.list.clear();
+    L.segs.clear();
+    L.blocks = 0;
     for (int k = 0; k < NLAUNCH; k++) {
-        seg0[k] = (int)segs.size();
-        nseg[k] = (int)gseg[k].size();
-        nfr[k] = 0;
+        L.seg0[k] = (int)L.segs.size();
+        L.nseg[k] = (int)gseg[k].size();
+        L.nfr[k] = 0;
+        L.grid[k] = 0;
         for (auto& sg : gseg[k]) {
-            segs.push_back(make_int2((int)list.size(), (int)sg.size()));
-            list.insert(list.end(), sg.begin(), sg.end());
-            nfr[k] += (int)sg.size();
+            L.segs.push_back(make_int2((int)L.list.size(), (int)sg.size()));
+            L.list.insert(L.list.end(), sg.begin(), sg.end());
+            L.nfr[k] += (int)sg.size();
         }
-        if (!nfr[k])
+        if (!L.nfr[k])
             continue;
-        const size_t lds = 2 * (size_t)maxn[k];
-        const int rc = nms ? (k < 2 ? pk_grid<0, true>(c, lds, nfr[k], &grid[k]) : pk_grid<1, true>(c, lds, nfr[k], &grid[k]))
-                           : (k < 2 ? pk_grid<0, false>(c, lds, nfr[k], &grid[k])
-                                    : pk_grid<1, false>(c, lds, nfr[k], &grid[k]));
+        const size_t lds = 2 * (size_t)L.maxn[k];
+        const int rc = nms ? (k < 2 ? pk_grid<0, true>(c, lds, L.nfr[k], &L.grid[k])
+                                    : pk_grid<1, true>(c, lds, L.nfr[k], &L.grid[k]))
+                           : (k < 2 ? pk_grid<0, false>(c, lds, L.nfr[k], &L.grid[k])
+                                    : pk_grid<1, false>(c, lds, L.nfr[k], &L.grid[k]));
         if (rc)
             return rc;
-        maxgrid = std::max(maxgrid, grid[k]);
+        L.grid[k] = std::max(1, (int)((float)L.grid[k] * grid_scale));
+        L.blocks += (size_t)L.grid[k];
     }
-    if (segs.size() > NCOUNTERS)
+    if (L.segs.size() > NCOUNTERS)
         return set_err(c, LDPC_GPU_EINVAL, "too many codes in one batch");
-    int rc;
-    size_t blocks = 0;
-    for (int k = 0; k < NLAUNCH; k++)
-        blocks += (size_t)grid[k];
-    (void)maxgrid;
-    if ((rc = grow(c, &c->d_list, &c->list_bytes, sizeof(int) * list.size())) ||
-        (rc = grow(c, &c->d_seg, &c->seg_bytes, sizeof(int2) * segs.size())) ||
-        (rc = grow(c, &c->msg, &c->msg_bytes, blocks * maxw * sizeof(uint32_t))))
-        return rc;
-    HIPCHK(c, hipMemcpyAsync(c->d_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_seg, segs.data(), sizeof(int2) * segs.size(), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(NCOUNTERS), 0, c->stream, c->ticket);
+    return LDPC_GPU_OK;
+}
+
+/* Enqueue the launches of a layout: the counter reset (a kernel) and one
+ * persistent launch per (class, size), each on its own forked stream when
+ * concurrent (fork / join through events), else one after the other on
+ * the origin stream.  Kernels and stream dependencies only, no copies or
+ * allocations, so the sequence can be captured in a graph.  tev (may be
+ * NULL): start / end timing events per launch. */
+static int pk_enqueue(ldpc_gpu_ctx* c, const PkLayout& L, PkArgs a, const int2* d_seg, unsigned* d_cnt,
+                      uint32_t* d_msg, hipStream_t origin, const hipStream_t* aux, hipEvent_t fork,
+                      const hipEvent_t* join, const hipEvent_t* tev, bool concurrent, bool nms)
+{
+    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(NCOUNTERS), 0, origin, d_cnt);
+    HIPCHK(c, hipGetLastError());
+    if (concurrent)
+        HIPCHK(c, hipEventRecord(fork, origin));
+    a.msg_stride = (size_t)L.maxw;
+    size_t moff = 0;
+    for (int k = 0; k < NLAUNCH; k++) {
+        if (!L.nfr[k])
+            continue;
+        a.seg = d_seg + L.seg0[k];
+        a.nseg = L.nseg[k];
+        a.cnt = d_cnt + L.seg0[k];
+        a.msg = d_msg + moff;
+        moff += (size_t)L.grid[k] * L.maxw;
+        const size_t lds = 2 * (size_t)L.maxn[k];
+        hipStream_t st = concurrent ? aux[k] : origin;
+        if (concurrent)
+            HIPCHK(c, hipStreamWaitEvent(st, fork, 0));
+        if (tev)
+            HIPCHK(c, hipEventRecord(tev[2 * k], st));
+        if (nms) {
+            if (k < 2)
+                hipLaunchKernelGGL((ldpc_pk_kernel<0, true>), dim3(L.grid[k]), dim3(NT), lds, st, a);
+            else
+                hipLaunchKernelGGL((ldpc_pk_kernel<1, true>), dim3(L.grid[k]), dim3(NT), lds, st, a);
+        } else {
+            if (k < 2)
+                hipLaunchKernelGGL((ldpc_pk_kernel<0, false>), dim3(L.grid[k]), dim3(NT), lds, st, a);
+            else
+                hipLaunchKernelGGL((ldpc_pk_kernel<1, false>), dim3(L.grid[k]), dim3(NT), lds, st, a);
+        }
+        HIPCHK(c, hipGetLastError());
+        if (tev)
+            HIPCHK(c, hipEventRecord(tev[2 * k + 1], st));
+        if (concurrent) {
+            HIPCHK(c, hipEventRecord(join[k], st));
+            HIPCHK(c, hipStreamWaitEvent(origin, join[k], 0));
+        }
+    }
+    return LDPC_GPU_OK;
+}
+
+static PkArgs pk_args(ldpc_gpu_ctx* c, const DFrame* d_frames, const int* d_list, const int8_t* d_llr, uint8_t* d_out,
+                      int32_t* d_iters, uint8_t* d_synd, bool nms, int fallback, uint8_t* d_fell)
+{
     PkArgs a;
     memset(&a, 0, sizeof(a));
     a.codes = c->tab->codes;
@@ -1322,13 +1407,11 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
     a.pkoff = c->tab->pkoff;
     a.meta = c->tab->meta;
     a.frames = d_frames;
-    a.list = c->d_list;
+    a.list = d_list;
     a.llr = d_llr;
     a.out = d_out;
     a.iters = d_iters;
     a.synd = d_synd;
-    a.msg = c->msg;
-    a.msg_stride = (size_t)maxw;
     if (nms) {
         a.nS = c->d_nS;
         a.nB = c->d_nB;
@@ -1337,39 +1420,36 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
         a.fallback = fallback;
         a.fell = d_fell;
     }
-    /* the launches run concurrently on forked streams, so that the last
-     * frames of one overlap the others' work; each has its own messages */
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    size_t moff = 0;
+    return a;
+}
+
+/* Frames with their own early stop, decoded now: the layout's tables go to
+ * the context's scratch, the launches run concurrently on the forked
+ * streams (the last frames of one launch overlap the others' work), each
+ * with its own message blocks, timed per launch. */
+static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFrame>& hf, const int8_t* d_llr,
+                     uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, bool nms = false, int fallback = 0,
+                     uint8_t* d_fell = nullptr)
+{
+    PkLayout L;
+    int rc = pk_layout(c, hf, nms, 1.0f, L);
+    if (rc)
+        return rc;
+    if ((rc = grow(c, &c->d_list, &c->list_bytes, sizeof(int) * L.list.size())) ||
+        (rc = grow(c, &c->d_seg, &c->seg_bytes, sizeof(int2) * L.segs.size())) ||
+        (rc = grow(c, &c->msg, &c->msg_bytes, L.blocks * L.maxw * sizeof(uint32_t))))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_list, L.list.data(), sizeof(int) * L.list.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_seg, L.segs.data(), sizeof(int2) * L.segs.size(), hipMemcpyHostToDevice, c->stream));
+    const PkArgs a = pk_args(c, d_frames, c->d_list, d_llr, d_out, d_iters, d_synd, nms, fallback, d_fell);
+    hipEvent_t join[NLAUNCH];
     for (int k = 0; k < NLAUNCH; k++) {
-        c->cls_frames[k] = nfr[k];
+        join[k] = c->ev_cls[2 * k + 1];
+        c->cls_frames[k] = L.nfr[k];
         c->cls_di[k] = CLASS_DI[k / 2];
-        if (!nfr[k])
-            continue;
-        a.seg = c->d_seg + seg0[k];
-        a.nseg = nseg[k];
-        a.cnt = c->ticket + seg0[k];
-        a.msg = c->msg + moff;
-        moff += (size_t)grid[k] * maxw;
-        const size_t lds = 2 * (size_t)maxn[k];
-        hipStream_t st = c->aux[k];
-        HIPCHK(c, hipStreamWaitEvent(st, c->ev0, 0));
-        HIPCHK(c, hipEventRecord(c->ev_cls[2 * k], st));
-        if (nms) {
-            if (k < 2)
-                hipLaunchKernelGGL((ldpc_pk_kernel<0, true>), dim3(grid[k]), dim3(NT), lds, st, a);
-            else
-                hipLaunchKernelGGL((ldpc_pk_kernel<1, true>), dim3(grid[k]), dim3(NT), lds, st, a);
-        } else {
-            if (k < 2)
-                hipLaunchKernelGGL((ldpc_pk_kernel<0, false>), dim3(grid[k]), dim3(NT), lds, st, a);
-            else
-                hipLaunchKernelGGL((ldpc_pk_kernel<1, false>), dim3(grid[k]), dim3(NT), lds, st, a);
-        }
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(c->ev_cls[2 * k + 1], st));
-        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cls[2 * k + 1], 0));
     }
+    if ((rc = pk_enqueue(c, L, a, c->d_seg, c->ticket, c->msg, c->stream, c->aux, c->ev0, join, c->ev_cls, true, nms)))
+        return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timed = 1;
     return LDPC_GPU_OK;
@@ -1589,6 +1669,226 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
             break;
     }
     return LDPC_GPU_OK;
+}
+
+/* ---- prepared batches: graph capture and the launch-shape autotuner ---- */
+
+/* launch shapes the autotuner chooses from: grid scale (persistent
+ * workgroups per launch relative to occupancy x CUs, i.e. how many frames
+ * run at once) and whether the (class, size) launches run concurrently */
+static const struct {
+    float scale;
+    bool concurrent;
+} PLAN_VARIANTS[] = { { 1.0f, true }, { 1.0f, false }, { 0.75f, true }, { 0.5f, true }, { 0.5f, false } };
+#define PLAN_NVARIANTS ((int)(sizeof(PLAN_VARIANTS) / sizeof(PLAN_VARIANTS[0])))
+
+static int plan_build(ldpc_gpu_plan* p, int variant)
+{
+    ldpc_gpu_ctx* c = p->c;
+    PkLayout L;
+    int rc = pk_layout(c, p->hf, false, PLAN_VARIANTS[variant].scale, L);
+    if (rc)
+        return rc;
+    if (L.list.size() > p->list_cap || L.segs.size() > p->seg_cap || L.blocks * L.maxw > p->msg_cap)
+        return set_err(c, LDPC_GPU_EINVAL, "plan scratch too small");
+    HIPCHK(c, hipMemcpy(p->d_list, L.list.data(), sizeof(int) * L.list.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(p->d_seg, L.segs.data(), sizeof(int2) * L.segs.size(), hipMemcpyHostToDevice));
+    if (p->exec) {
+        (void)hipGraphExecDestroy(p->exec);
+        p->exec = nullptr;
+    }
+    if (p->graph) {
+        (void)hipGraphDestroy(p->graph);
+        p->graph = nullptr;
+    }
+    const PkArgs a = pk_args(c, p->d_frames, p->d_list, p->d_llr, p->d_out, p->d_iters, p->d_synd, false, 0, nullptr);
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = pk_enqueue(c, L, a, p->d_seg, p->d_cnt, p->d_msg, c->stream, p->aux, p->fork, p->join, nullptr,
+                    PLAN_VARIANTS[variant].concurrent, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc || e != hipSuccess) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc ? rc : set_err(c, LDPC_GPU_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+    }
+    p->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+    p->variant = variant;
+    return LDPC_GPU_OK;
+}
+
+extern "C" void ldpc_gpu_plan_destroy(ldpc_gpu_plan* p)
+{
+    if (!p)
+        return;
+    ldpc_gpu_ctx* c = p->c;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (p->exec)
+        (void)hipGraphExecDestroy(p->exec);
+    if (p->graph)
+        (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->d_frames);
+    (void)hipFree(p->d_list);
+    (void)hipFree(p->d_seg);
+    (void)hipFree(p->d_cnt);
+    (void)hipFree(p->d_msg);
+    for (int k = 0; k < NLAUNCH; k++) {
+        if (p->aux[k])
+            (void)hipStreamDestroy(p->aux[k]);
+        if (p->join[k])
+            (void)hipEventDestroy(p->join[k]);
+    }
+    if (p->fork)
+        (void)hipEventDestroy(p->fork);
+    if (p->t0)
+        (void)hipEventDestroy(p->t0);
+    if (p->t1)
+        (void)hipEventDestroy(p->t1);
+    delete p;
+}
+
+extern "C" int ldpc_gpu_plan_create(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
+                                    uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, ldpc_gpu_plan** out)
+{
+    if (!c || !out)
+        return LDPC_GPU_EINVAL;
+    *out = nullptr;
+    if (nframes < 1 || !frames || !d_llr || !d_out || !d_iters || !d_synd)
+        return set_err(c, LDPC_GPU_EINVAL, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<DFrame> df;
+    int maxw, ngroup;
+    int rc = check_frames(c, frames, nframes, df, &maxw, &ngroup);
+    if (rc)
+        return rc;
+    if (ngroup)
+        return set_err(c, LDPC_GPU_EINVAL, "plans take frames with their own early stop (group -1)");
+    ldpc_gpu_plan* p = new (std::nothrow) ldpc_gpu_plan();
+    if (!p)
+        return LDPC_GPU_ENOMEM;
+    p->c = c;
+    p->nframes = nframes;
+    p->hf = df;
+    p->d_llr = d_llr;
+    p->d_out = d_out;
+    p->d_iters = d_iters;
+    p->d_synd = d_synd;
+    p->variant = -1;
+    for (int v = 0; v < 8; v++)
+        p->variant_ms[v] = -1.0f;
+    /* scratch sized for the largest variant (full grid) */
+    PkLayout L;
+    if ((rc = pk_layout(c, df, false, 1.0f, L))) {
+        delete p;
+        return rc;
+    }
+    p->list_cap = L.list.size();
+    p->seg_cap = L.segs.size();
+    p->msg_cap = L.blocks * L.maxw;
+    auto fail = [&](int r) {
+        ldpc_gpu_plan_destroy(p);
+        return r;
+    };
+    if (hipMalloc((void**)&p->d_frames, sizeof(DFrame) * nframes) != hipSuccess ||
+        hipMalloc((void**)&p->d_list, sizeof(int) * p->list_cap) != hipSuccess ||
+        hipMalloc((void**)&p->d_seg, sizeof(int2) * p->seg_cap) != hipSuccess ||
+        hipMalloc((void**)&p->d_cnt, sizeof(unsigned) * NCOUNTERS) != hipSuccess ||
+        hipMalloc((void**)&p->d_msg, sizeof(uint32_t) * p->msg_cap) != hipSuccess)
+        return fail(set_err(c, LDPC_GPU_ENOMEM, "plan scratch"));
+    if (hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&p->t0) != hipSuccess || hipEventCreate(&p->t1) != hipSuccess)
+        return fail(LDPC_GPU_EDEVICE);
+    for (int k = 0; k < NLAUNCH; k++)
+        if (hipStreamCreateWithFlags(&p->aux[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->join[k], hipEventDisableTiming) != hipSuccess)
+            return fail(LDPC_GPU_EDEVICE);
+    if (hipMemcpy(p->d_frames, df.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(LDPC_GPU_EDEVICE);
+    if ((rc = plan_build(p, 0)))
+        return fail(rc);
+    *out = p;
+    return LDPC_GPU_OK;
+}
+
+extern "C" int ldpc_gpu_plan_launch(ldpc_gpu_plan* p)
+{
+    if (!p)
+        return LDPC_GPU_EINVAL;
+    ldpc_gpu_ctx* c = p->c;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipEventRecord(p->t0, c->stream));
+    HIPCHK(c, hipGraphLaunch(p->exec, c->stream));
+    HIPCHK(c, hipEventRecord(p->t1, c->stream));
+    p->launched = 1;
+    return LDPC_GPU_OK;
+}
+
+extern "C" float ldpc_gpu_plan_last_ms(ldpc_gpu_plan* p)
+{
+    if (!p || !p->launched)
+        return -1.0f;
+    float ms = -1.0f;
+    if (hipEventSynchronize(p->t1) != hipSuccess || hipEventElapsedTime(&ms, p->t0, p->t1) != hipSuccess)
+        return -1.0f;
+    return ms;
+}
+
+extern "C" int ldpc_gpu_plan_tune(ldpc_gpu_plan* p, int reps)
+{
+    if (!p || reps < 1)
+        return LDPC_GPU_EINVAL;
+    ldpc_gpu_ctx* c = p->c;
+    int best = 0, rc;
+    for (int v = 0; v < PLAN_NVARIANTS; v++) {
+        if ((rc = plan_build(p, v)))
+            return rc;
+        if ((rc = ldpc_gpu_plan_launch(p)))
+            return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float tot = 0.0f;
+        for (int r = 0; r < reps; r++) {
+            if ((rc = ldpc_gpu_plan_launch(p)))
+                return rc;
+            const float ms = ldpc_gpu_plan_last_ms(p);
+            if (ms < 0.0f)
+                return set_err(c, LDPC_GPU_EDEVICE, "plan timing failed");
+            tot += ms;
+        }
+        p->variant_ms[v] = tot / (float)reps;
+        if (p->variant_ms[v] < p->variant_ms[best])
+            best = v;
+    }
+    return plan_build(p, best);
+}
+
+extern "C" int ldpc_gpu_plan_select(ldpc_gpu_plan* p, int variant)
+{
+    if (!p)
+        return LDPC_GPU_EINVAL;
+    if (variant < 0 || variant >= PLAN_NVARIANTS)
+        return set_err(p->c, LDPC_GPU_EINVAL, "plan variant %d out of range", variant);
+    HIPCHK(p->c, hipSetDevice(p->c->device));
+    HIPCHK(p->c, hipStreamSynchronize(p->c->stream));
+    return plan_build(p, variant);
+}
+
+extern "C" int ldpc_gpu_plan_info(ldpc_gpu_plan* p, int* variant, float* scale, int* concurrent, float* ms, int max)
+{
+    if (!p)
+        return LDPC_GPU_EINVAL;
+    if (variant)
+        *variant = p->variant;
+    for (int v = 0; v < PLAN_NVARIANTS && v < max; v++) {
+        if (scale)
+            scale[v] = PLAN_VARIANTS[v].scale;
+        if (concurrent)
+            concurrent[v] = PLAN_VARIANTS[v].concurrent;
+        if (ms)
+            ms[v] = p->variant_ms[v];
+    }
+    return PLAN_NVARIANTS;
 }
 
 extern "C" int ldpc_gpu_decode_host(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, const int8_t* llr,

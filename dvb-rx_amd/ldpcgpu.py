@@ -81,6 +81,14 @@ def gpu_lib():
         L.ldpc_gpu_code_max_degree.argtypes = [i32]
         L.ldpc_gpu_set_kernel.argtypes = [vp, i32]
         L.ldpc_gpu_set_nms.argtypes = [vp, i32, i32, vp, vp]
+        L.ldpc_gpu_plan_create.argtypes = [vp, vp, i32, vp, vp, vp, vp, C.POINTER(vp)]
+        L.ldpc_gpu_plan_launch.argtypes = [vp]
+        L.ldpc_gpu_plan_tune.argtypes = [vp, i32]
+        L.ldpc_gpu_plan_last_ms.argtypes = [vp]
+        L.ldpc_gpu_plan_last_ms.restype = C.c_float
+        L.ldpc_gpu_plan_info.argtypes = [vp, ip, vp, vp, vp, i32]
+        L.ldpc_gpu_plan_select.argtypes = [vp, i32]
+        L.ldpc_gpu_plan_destroy.argtypes = [vp]
         L.ldpc_gpu_decode_batch_nms.argtypes = [vp, vp, i32, vp, vp, vp, vp, i32, vp]
         L.drx_ldpc_lanes.argtypes = []
         L.drx_ldpc_new.argtypes = [vp, C.c_char_p]
@@ -327,6 +335,44 @@ class Decoder:
             "decode_host",
         )
         return out, iters, synd
+
+
+class Plan:
+    """A prepared batch replayed as one graph (ldpc_gpu_plan_*)."""
+
+    def __init__(self, dec, frames, d_llr, d_out, d_iters, d_synd):
+        self.dec = dec
+        self.L = dec.L
+        self.frames = np.ascontiguousarray(frames, FRAME_DTYPE)
+        self.p = C.c_void_p()
+        dec._check(self.L.ldpc_gpu_plan_create(dec.ctx, self.frames.ctypes.data, len(self.frames), d_llr, d_out,
+                                               d_iters, d_synd, C.byref(self.p)), "plan_create")
+
+    def close(self):
+        if self.p:
+            self.L.ldpc_gpu_plan_destroy(self.p)
+            self.p = C.c_void_p()
+
+    def launch(self):
+        self.dec._check(self.L.ldpc_gpu_plan_launch(self.p), "plan_launch")
+
+    def tune(self, reps=3):
+        self.dec._check(self.L.ldpc_gpu_plan_tune(self.p, reps), "plan_tune")
+
+    def select(self, variant):
+        self.dec._check(self.L.ldpc_gpu_plan_select(self.p, variant), "plan_select")
+
+    def last_ms(self):
+        return self.L.ldpc_gpu_plan_last_ms(self.p)
+
+    def info(self):
+        v = C.c_int()
+        sc = np.zeros(8, np.float32)
+        cc = np.zeros(8, np.int32)
+        ms = np.zeros(8, np.float32)
+        n = self.L.ldpc_gpu_plan_info(self.p, C.byref(v), sc.ctypes.data, cc.ctypes.data, ms.ctypes.data, 8)
+        return {"variant": v.value, "variants": [{"grid_scale": float(sc[k]), "concurrent": bool(cc[k]),
+                                                  "ms": round(float(ms[k]), 4)} for k in range(n)]}
 
 
 NMS_DEGS = 33

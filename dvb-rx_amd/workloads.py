@@ -90,3 +90,109 @@ def mixed_s2_specs(nframes=8192, seed=7, margin_db=0.3, short_share=0.25):
 def mixed_s2(nframes=8192, seed=7, max_iter=50, margin_db=0.3, short_share=0.25):
     specs = mixed_s2_specs(nframes, seed, margin_db, short_share)
     return Batch(f"mixed_s2_x{nframes}", specs, seed, max_iter)
+
+
+# ---- batches generated on the GPU ----------------------------------------
+
+S2X_NORMAL_HIGH = [0xb8, 0xba, 0xbe, 0xc2, 0xc6, 0xc8, 0xca, 0xcc, 0xce, 0xd0, 0xd2, 0xd4, 0xd6]
+S2X_SHORT_LOW = [0xd8, 0xda, 0xdc, 0xde, 0xe0, 0xe2, 0xe4, 0xe6, 0xe8, 0xea]
+
+
+class GpuBatch:
+    """Frames generated on the device: per MODCOD a pool of codewords (host
+    encoder) sent through ldpc_gpu_channel with fresh noise.  Frames of a
+    MODCOD are contiguous, each in its own N-byte slot."""
+
+    def __init__(self, dec, groups, margin_db, max_iter=50, pool_n=16, seed=11):
+        from nms_train import codeword_pool
+
+        self.dec = dec
+        self.groups = groups
+        self.ptrs = []
+        frames, refs, kls = [], [], []
+        layout = []
+        off = 0
+        out_off = 0
+        rng = np.random.default_rng(seed)
+        for g, (pls, count) in enumerate(groups):
+            m = ldpcgpu.pls_lookup(pls)
+            fec = ldpcgpu.fec_lookup(m["short_frame"], m["rate"])
+            code = ldpcgpu.code_lookup(m["short_frame"], m["rate"])
+            pool = codeword_pool(m["short_frame"], m["rate"], pool_n, seed + g)
+            rows = rng.integers(0, pool_n, count).astype(np.int32)
+            layout.append((pls, code, fec.nldpc, off, count, pool, rows, ldpcgpu.qef_db(pls) + margin_db))
+            kb = (fec.kldpc + 7) // 8
+            ref = np.packbits(pool[:, :fec.kldpc], axis=1)
+            for f in range(count):
+                frames.append((code, max_iter, fec.kldpc, -1, off + f * fec.nldpc, out_off))
+                refs.append(ref[rows[f]])
+                kls.append(fec.kldpc)
+                out_off += kb
+            off += count * fec.nldpc
+            off = (off + 7) // 8 * 8
+        self.frames = np.array(frames, dtype=ldpcgpu.FRAME_DTYPE)
+        self.nframes = len(frames)
+        self.llr_bytes = off
+        self.out_bytes = out_off
+        self.refs = refs
+        self.kldpc = np.array(kls, np.int64)
+        self.info_bits = int(self.kldpc.sum())
+        self.d_llr = self._alloc(max(off, 8))
+        self.d_out = self._alloc(max(out_off, 1))
+        self.d_it = self._alloc(4 * self.nframes)
+        self.d_ok = self._alloc(self.nframes)
+        for pls, code, N, base, count, pool, rows, esn0 in layout:
+            d_pool = self._put(pool)
+            d_rows = self._put(rows)
+            d_e = self._put(np.full(count, esn0, np.float32))
+            ldpcgpu.channel(pls, N, d_pool, d_rows, count, d_e, seed * 131 + pls, self.d_llr + base,
+                            stream=dec.stream)
+        dec.synchronize()
+        self.layout = [(pls, code, N, base, count, esn0) for pls, code, N, base, count, _, _, esn0 in layout]
+
+    def _alloc(self, n):
+        p = self.dec.malloc(n)
+        self.ptrs.append(p)
+        return p
+
+    def _put(self, a):
+        a = np.ascontiguousarray(a)
+        p = self._alloc(a.nbytes)
+        self.dec.upload(p, a)
+        return p
+
+    def results(self):
+        out = np.zeros(self.out_bytes, np.uint8)
+        it = np.zeros(self.nframes, np.int32)
+        ok = np.zeros(self.nframes, np.uint8)
+        self.dec.download(out, self.d_out)
+        self.dec.download(it, self.d_it)
+        self.dec.download(ok, self.d_ok)
+        return out, it, ok
+
+    def frame_errors(self, out, t_of=None):
+        """Frames whose decoded kldpc bits differ from the codeword (any bit)."""
+        bad = 0
+        for f in range(self.nframes):
+            o = int(self.frames["out_off"][f])
+            ref = self.refs[f]
+            bad += int(not np.array_equal(out[o:o + len(ref)], ref))
+        return bad
+
+    def free(self):
+        for p in self.ptrs:
+            self.dec.free(p)
+        self.ptrs = []
+
+
+def s2x_high_order(dec, n_normal=16384, n_short=4096, margin_db=0.2, max_iter=50, seed=11):
+    """DVB-S2X high-order batch: normal frames spread evenly over the
+    64/128/256APSK MODCODs (and the 4+12+20+28 / 8+16+20+20 APSK ones of
+    those rates), short frames over the QPSK / 8PSK ones; AWGN at the
+    MODCOD's threshold + margin_db."""
+    groups = []
+    for lst, n in ((S2X_NORMAL_HIGH, n_normal), (S2X_SHORT_LOW, n_short)):
+        base, extra = divmod(n, len(lst))
+        for k, pls in enumerate(lst):
+            groups.append((pls, base + (1 if k < extra else 0)))
+    return GpuBatch(dec, groups, margin_db, max_iter, seed=seed)

@@ -113,6 +113,32 @@ int ldpc_gpu_decode_batch_nms(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, i
                               uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, int fallback_max_iter,
                               uint8_t* d_fallback);
 
+/*
+ * Prepared batches (no counterpart in the reference, whose batches are
+ * built per call): a fixed frame table (group -1 only) bound to fixed
+ * device buffers, prepared once - frame table, launch layout and scratch
+ * of its own - with its launches captured as one hipGraph that
+ * ldpc_gpu_plan_launch replays on the context's stream (enqueue only).
+ * The caller refills d_llr between launches; results as
+ * ldpc_gpu_decode_batch.
+ *
+ * ldpc_gpu_plan_tune is the autotuner: it times every launch shape (the
+ * persistent grid per launch, i.e. frames in flight, at 100 / 75 / 50 % of
+ * what fits, and concurrent or serial (class, size) launches) over reps
+ * replays and keeps the fastest.  ldpc_gpu_plan_info reports the chosen
+ * shape and the timings (ms per launch, < 0 if not timed).
+ */
+typedef struct ldpc_gpu_plan ldpc_gpu_plan;
+int ldpc_gpu_plan_create(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
+                         uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd, ldpc_gpu_plan** out);
+int ldpc_gpu_plan_launch(ldpc_gpu_plan* plan);
+int ldpc_gpu_plan_tune(ldpc_gpu_plan* plan, int reps);
+float ldpc_gpu_plan_last_ms(ldpc_gpu_plan* plan);
+int ldpc_gpu_plan_info(ldpc_gpu_plan* plan, int* variant, float* scale, int* concurrent, float* ms, int max);
+/* Use launch shape `variant` (0 .. count returned by ldpc_gpu_plan_info - 1). */
+int ldpc_gpu_plan_select(ldpc_gpu_plan* plan, int variant);
+void ldpc_gpu_plan_destroy(ldpc_gpu_plan* plan);
+
 /* Kernel choice for frames with their own early stop (group -1):
  * LDPC_GPU_KERNEL_AUTO runs two frames of one code per workgroup in packed
  * 16-bit arithmetic; LDPC_GPU_KERNEL_SINGLE one frame per workgroup (the

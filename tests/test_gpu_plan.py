@@ -1,0 +1,141 @@
+"""Prepared batches replayed as one graph (ldpc_gpu_plan_*): every launch
+shape the autotuner chooses from gives the eager decoder's results; the
+graph is replayed several times with other work in between, and with the
+LLR buffer refilled, every replay checked."""
+import numpy as np
+import pytest
+
+from helpers import gen, ldpcgpu
+
+pytestmark = pytest.mark.gpu
+
+SETS = [(0, "1_2", 1.2), (0, "9_10", 6.5), (1, "3_5", 2.5), (1, "8_9", 6.3), (0, "2_3_t2", 3.2), (0, "135_180", 4.2)]
+
+
+def _batch(seed, per=30):
+    """Frames of several codes in 64800-byte slots: (llr buffer, frame table, out bytes)."""
+    rows, frames = [], []
+    off = 0
+    for k, (short, rate, esn0) in enumerate(SETS):
+        fec, llr, _ = gen(short, rate, esn0, per, seed + k)
+        code = ldpcgpu.code_lookup(short, rate)
+        for f in range(per):
+            slot = np.zeros(64800, np.int8)
+            slot[:llr.shape[1]] = llr[f]
+            rows.append(slot)
+            frames.append((code, 40, fec.kldpc, -1, 64800 * len(frames), off))
+            off += (fec.kldpc + 7) // 8
+    fr = np.array(frames, dtype=ldpcgpu.FRAME_DTYPE)
+    return np.stack(rows), fr, off
+
+
+class Bufs:
+    def __init__(self, dec, nbytes_llr, nout, nf):
+        self.dec = dec
+        self.llr = dec.malloc(nbytes_llr)
+        self.out = dec.malloc(nout)
+        self.it = dec.malloc(4 * nf)
+        self.ok = dec.malloc(nf)
+        self.nout, self.nf = nout, nf
+
+    def results(self):
+        out = np.zeros(self.nout, np.uint8)
+        it = np.zeros(self.nf, np.int32)
+        ok = np.zeros(self.nf, np.uint8)
+        self.dec.download(out, self.out)
+        self.dec.download(it, self.it)
+        self.dec.download(ok, self.ok)
+        return out, it, ok
+
+    def clear(self):
+        z = np.zeros(self.nout, np.uint8)
+        self.dec.upload(self.out, z)
+        self.dec.upload(self.it, np.full(self.nf, -7, np.int32))
+
+    def free(self):
+        for p in (self.llr, self.out, self.it, self.ok):
+            self.dec.free(p)
+
+
+@pytest.fixture(scope="module")
+def dec():
+    d = ldpcgpu.Decoder(0)
+    yield d
+    d.close()
+
+
+def test_plan_variants_match_eager(dec):
+    llr, fr, nout = _batch(100)
+    b = Bufs(dec, llr.nbytes, nout, len(fr))
+    other = Bufs(dec, llr.nbytes, nout, len(fr))
+    plan = None
+    try:
+        dec.upload(b.llr, llr)
+        dec.upload(other.llr, llr)
+        dec.decode_batch(fr, b.llr, b.out, b.it, b.ok)
+        dec.synchronize()
+        want = b.results()
+        plan = ldpcgpu.Plan(dec, fr, b.llr, b.out, b.it, b.ok)
+        nvar = len(plan.info()["variants"])
+        assert nvar >= 4
+        for v in range(nvar):
+            plan.select(v)
+            for rep in range(3):
+                b.clear()
+                plan.launch()
+                dec.synchronize()
+                got = b.results()
+                for w, g in zip(want, got):
+                    assert (w == g).all(), (v, rep)
+                # eager work of the program between replays
+                dec.decode_batch(fr, other.llr, other.out, other.it, other.ok)
+                dec.synchronize()
+    finally:
+        if plan:
+            plan.close()
+        b.free()
+        other.free()
+
+
+def test_plan_refill_and_tune(dec):
+    llr1, fr, nout = _batch(200)
+    llr2, fr2, _ = _batch(300)
+    assert (fr == fr2).all()
+    b = Bufs(dec, llr1.nbytes, nout, len(fr))
+    ref = Bufs(dec, llr1.nbytes, nout, len(fr))
+    plan = None
+    try:
+        plan = ldpcgpu.Plan(dec, fr, b.llr, b.out, b.it, b.ok)
+        dec.upload(b.llr, llr1)
+        plan.tune(2)
+        info = plan.info()
+        assert 0 <= info["variant"] < len(info["variants"])
+        assert all(v["ms"] > 0 for v in info["variants"])
+        best = min(range(len(info["variants"])), key=lambda k: info["variants"][k]["ms"])
+        assert info["variant"] == best
+        for llr in (llr2, llr1, llr2):
+            dec.upload(b.llr, llr)
+            dec.upload(ref.llr, llr)
+            b.clear()
+            plan.launch()
+            dec.decode_batch(fr, ref.llr, ref.out, ref.it, ref.ok)
+            dec.synchronize()
+            assert plan.last_ms() > 0
+            for w, g in zip(ref.results(), b.results()):
+                assert (w == g).all()
+    finally:
+        if plan:
+            plan.close()
+        b.free()
+        ref.free()
+
+
+def test_plan_rejects_groups(dec):
+    llr, fr, nout = _batch(400, per=2)
+    fr["group"] = 0
+    b = Bufs(dec, llr.nbytes, nout, len(fr))
+    try:
+        with pytest.raises(RuntimeError):
+            ldpcgpu.Plan(dec, fr, b.llr, b.out, b.it, b.ok)
+    finally:
+        b.free()
