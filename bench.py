@@ -430,6 +430,13 @@ def main():
     c1 = None if args.no_config1 else config1(dec)
     c4 = None if args.no_config1 else config4(dec)
     c3 = None if args.no_config1 else [config3(dec, 0.2), config3(dec, 1.0)]
+    c5 = None
+    if not args.no_config1:
+        import streaming
+
+        tabs = streaming.learned_tables()
+        c5 = [streaming.run(dec, 1.0, 1.0, False), streaming.run(dec, 1.0, 1.0, True, tabs),
+              streaming.run(dec, 0.5, 4.0, False), streaming.run(dec, 0.5, 4.0, True, tabs)]
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(batch, args.cpu_seconds)
@@ -487,6 +494,7 @@ def main():
         "config1": c1,
         "learned_vs_plain": c4,
         "s2x_high_order_graph": c3,
+        "streaming": c5,
     }
     if cpu:
         res["gpu_vs_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None
