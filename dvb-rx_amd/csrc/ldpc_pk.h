@@ -113,6 +113,21 @@ LG_PK_HD void pk_fold(PkNode<DEG>& n, int e, lg_pk x)
     n.s = n.s ^ x;
 }
 
+/* pk_fold of the lanes where skip is 0; lanes where it is all ones leave
+ * the minimums and the sign as they were (|x| | 0xffff is above any
+ * minimum).  Branch-free, so the posterior reads of all edges can be in
+ * flight together. */
+template <int DEG>
+LG_PK_HD void pk_fold_skip(PkNode<DEG>& n, int e, lg_pk x, uint32_t skip)
+{
+    const lg_pk a = pk_of(pk_bits(pk_abs(x)) | skip);
+    n.x[e] = x;
+    n.a[e] = a;
+    n.min1 = pk_minu(n.min1, pk_maxu(n.min0, a));
+    n.min0 = pk_minu(n.min0, a);
+    n.s = pk_of(pk_bits(n.s) ^ (pk_bits(x) & ~skip));
+}
+
 LG_PK_HD lg_pk pk_in(const uint16_t* post, int v, const uint32_t* mw, int e, uint32_t sel_even, uint32_t sel_odd)
 {
     const lg_pk L = pk_of(lg_perm(0, post[v], PK_UNPACK_LO));
@@ -136,9 +151,13 @@ LG_PK_HD void pk_phase_a(PkNode<DEG>& n, const uint16_t* post, const uint32_t* e
     n.min1 = pk_dup(-1);
     n.s = pk_dup(0);
 #pragma unroll
-    for (int e = 0; e < DEG; e++)
-        if (!TAIL || !((dep >> e) & 1))
-            pk_fold(n, e, pk_in(post, n.ad[e], mw, e, sel_even, sel_odd));
+    for (int e = 0; e < DEG; e++) {
+        const lg_pk x = pk_in(post, n.ad[e], mw, e, sel_even, sel_odd);
+        if (TAIL) /* dependent edges are folded by pk_step */
+            pk_fold_skip(n, e, x, 0u - ((dep >> e) & 1));
+        else
+            pk_fold(n, e, x);
+    }
     pk_fold(n, DEG, pk_in(post, n.apc, mw, DEG, sel_even, sel_odd));
     if (n.has_pp) {
         pk_fold(n, DEG + 1, pk_in(post, n.app, mw, DEG + 1, sel_even, sel_odd));
