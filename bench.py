@@ -364,8 +364,23 @@ def main():
     dec.download(it, d_it)
     dec.download(ok, d_ok)
     frame_err = 0
+    by_modcod = {}  # pls -> [frames, frames in error]
     for f in range(batch.nframes):
-        frame_err += int(not np.array_equal(batch.frame_ref(f), out[batch.layout[f][4]:batch.layout[f][4] + (batch.kldpc[f] + 7) // 8]))
+        bad = int(not np.array_equal(batch.frame_ref(f), out[batch.layout[f][4]:batch.layout[f][4] + (batch.kldpc[f] + 7) // 8]))
+        frame_err += bad
+        e = by_modcod.setdefault(int(batch.specs[f]["pls"]), [0, 0])
+        e[0] += 1
+        e[1] += bad
+    modcod_err = {}
+    for pls, (nf_, ne) in sorted(by_modcod.items()):
+        if ne:
+            m = ldpcgpu.pls_lookup(pls)
+            modcod_err[f"{pls:#04x} {'short' if m['short_frame'] else 'normal'} {m['rate']} {m['constellation']}"] = \
+                {"frames": nf_, "in_error": ne, "esn0_db": round(ldpcgpu.qef_db(pls) + 0.3, 2)}
+    # MODCODs that fail outright at this margin (more than 10 % of their
+    # frames), and the frame error rate of all the others
+    failing = [v for v in modcod_err.values() if 10 * v["in_error"] > v["frames"]]
+    fer_rest = (frame_err - sum(v["in_error"] for v in failing)) / max(1, batch.nframes - sum(v["frames"] for v in failing))
     ms_step = dt / args.steps * 1e3
     value = batch.info_bits / (dt / args.steps) / 1e9
 
@@ -463,6 +478,8 @@ def main():
             "info_bits_per_step": batch.info_bits,
             "avg_iterations": round(float(it.mean()), 3),
             "fer": frame_err / batch.nframes,
+            "fer_modcods_in_error": modcod_err,
+            "fer_without_failing_modcods": round(fer_rest, 6),
             "frames_unconverged": int((ok == 0).sum()),
         },
         "roofline": {

@@ -86,10 +86,12 @@ def main():
         seen = {}
         for did in sorted(src):
             name, lds, grid, v = src[did]
-            m = re.search(a.kernel + r"<\d+>", name)
+            # instance <class, learned>: the plain decoder's is named
+            # "<kernel><class>", the learned one's "<kernel><class, learned>"
+            m = re.search(a.kernel + r"<(\d+)(?:, *(true|false))?>", name)
             if not m:
                 continue
-            inst = m.group(0)
+            inst = f"{a.kernel}<{m.group(1)}{', learned' if m.group(2) == 'true' else ''}>"
             i = seen.get(inst, 0)
             seen[inst] = i + 1
             label = a.labels[i % len(a.labels)]

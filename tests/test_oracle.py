@@ -193,3 +193,22 @@ def test_oracle_matches_reference_every_code(short, rate):
             assert (rb[0] == bits[f]).all(), (name, f)
         ri, rb = ref_decode(R, fec, rate, llr, mi)
         assert ri == r and (rb == gbits).all(), name
+
+
+@needs_ref
+@pytest.mark.parametrize("short,rate", [(0, "1_2"), (0, "3_4"), (0, "9_10"), (1, "1_4"), (1, "8_9"), (0, "2_3_t2")])
+def test_fixed_iteration_run_matches_reference(short, rate):
+    """oracle_ldpc_run (T updates, no early stop) against the reference
+    build: well below threshold no frame converges, so the reference runs
+    exactly T updates and its output bits are the signs of the oracle's
+    posteriors after T."""
+    fec, llr, _ = gen(short, rate, esn0_for(short, rate) - 3.0, 3, 900 + len(rate))
+    o = Oracle(short, rate, fec.kldpc)
+    T = 5
+    post = o.run(llr, T)
+    R = next(iter(REFS.values()))
+    for f in range(len(llr)):
+        ri, rb = ref_decode(R, fec, rate, llr[f:f + 1], T)
+        assert ri == -1, "frame converged: the case needs a lower Es/N0"
+        mine = np.packbits((post[f, :fec.kldpc] < 0).astype(np.uint8))
+        assert (mine == rb[0]).all(), f
