@@ -450,8 +450,16 @@ def main():
         import streaming
 
         tabs = streaming.learned_tables()
-        c5 = [streaming.run(dec, 1.0, 1.0, False), streaming.run(dec, 1.0, 1.0, True, tabs),
-              streaming.run(dec, 0.5, 4.0, False), streaming.run(dec, 0.5, 4.0, True, tabs)]
+        # live scheduler with two decoder contexts in flight (per launch
+        # kind queues), and the one-batch-at-a-time scheduler for reference
+        dec2 = ldpcgpu.Decoder(0)
+        try:
+            lanes = [dec, dec2]
+            c5 = [streaming.run(dec, 1.0, 1.0, False, lanes=lanes), streaming.run(dec, 1.0, 1.0, True, tabs, lanes=lanes),
+                  streaming.run(dec, 0.5, 4.0, False, lanes=lanes), streaming.run(dec, 0.5, 4.0, True, tabs, lanes=lanes),
+                  streaming.run(dec, 1.0, 1.0, False)]
+        finally:
+            dec2.close()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(batch, args.cpu_seconds)

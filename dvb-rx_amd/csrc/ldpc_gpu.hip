@@ -800,6 +800,7 @@ struct ldpc_gpu_ctx {
     int cls_di[NLAUNCH];     /* its degree class bound */
     int timed;
     int force_scalar;        /* per-frame batches through the one-frame kernel */
+    int serial;              /* launches of a decode one after the other on the context's stream */
     /* learned tables: row 0 plain (S = 16, B = 1), then per code its rows */
     uint8_t* h_nS;
     int16_t* h_nB;
@@ -1093,6 +1094,26 @@ int ldpc_gpu_synchronize(ldpc_gpu_ctx* c)
         return LDPC_GPU_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LDPC_GPU_OK;
+}
+
+int ldpc_gpu_query(ldpc_gpu_ctx* c)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipErrorNotReady)
+        return 0;
+    HIPCHK(c, e);
+    return 1;
+}
+
+int ldpc_gpu_set_concurrent(ldpc_gpu_ctx* c, int concurrent)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    c->serial = !concurrent;
     return LDPC_GPU_OK;
 }
 
@@ -1448,7 +1469,8 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
         c->cls_frames[k] = L.nfr[k];
         c->cls_di[k] = CLASS_DI[k / 2];
     }
-    if ((rc = pk_enqueue(c, L, a, c->d_seg, c->ticket, c->msg, c->stream, c->aux, c->ev0, join, c->ev_cls, true, nms)))
+    if ((rc = pk_enqueue(c, L, a, c->d_seg, c->ticket, c->msg, c->stream, c->aux, c->ev0, join, c->ev_cls, !c->serial,
+                         nms)))
         return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timed = 1;

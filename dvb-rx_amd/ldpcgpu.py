@@ -73,6 +73,8 @@ def gpu_lib():
         L.ldpc_gpu_upload.argtypes = [vp, vp, vp, sz]
         L.ldpc_gpu_download.argtypes = [vp, vp, vp, sz]
         L.ldpc_gpu_synchronize.argtypes = [vp]
+        L.ldpc_gpu_query.argtypes = [vp]
+        L.ldpc_gpu_set_concurrent.argtypes = [vp, C.c_int]
         L.ldpc_gpu_decode_batch.argtypes = [vp, vp, i32, vp, vp, vp, vp]
         L.ldpc_gpu_decode_host.argtypes = [vp, vp, i32, vp, sz, vp, sz, vp, vp]
         L.ldpc_gpu_last_kernel_ms.argtypes = [vp]
@@ -284,6 +286,18 @@ class Decoder:
 
     def synchronize(self):
         self._check(self.L.ldpc_gpu_synchronize(self.ctx), "synchronize")
+
+    def set_concurrent(self, on):
+        """Launches of one decode on forked streams (True, the default) or
+        one after the other on the context's stream (False)."""
+        self._check(self.L.ldpc_gpu_set_concurrent(self.ctx, int(bool(on))), "set_concurrent")
+
+    def query(self):
+        """True when everything enqueued on the context has finished
+        (non-blocking)."""
+        r = self.L.ldpc_gpu_query(self.ctx)
+        self._check(min(r, 0), "query")
+        return r == 1
 
     def decode_batch(self, frames, d_llr, d_out, d_iters, d_synd):
         frames = np.ascontiguousarray(frames, FRAME_DTYPE)

@@ -13,13 +13,19 @@ arrivals):
 
 Every frame's int8 LLRs are written by the GPU channel into a device-
 resident slot when it is admitted (slots are the admission limit: a frame
-arriving while all slots are in use waits for one).  The scheduler runs
-on a simulated clock driven by measured GPU time: whenever the decoder is
-idle it takes every frame that has arrived (up to max_batch, mixed codes
-and frame sizes) as one batch, decodes it (ldpc_gpu_decode_batch, or the
-learned decoder), and the batch completes after its measured wall time
-(frame table upload, launches, synchronisation).  A frame's latency is
-its batch's completion time minus its arrival time.
+arriving while all slots are in use waits for one).  Two schedulers take
+every frame that has arrived (up to max_batch, mixed codes and frame
+sizes) as one batch whenever a decoder is idle, and decode it with
+ldpc_gpu_decode_batch (or the learned decoder):
+
+  schedule()  one batch at a time on a simulated clock driven by measured
+              GPU time: a batch completes after its measured wall time
+              (frame table upload, launches, synchronisation);
+  run_live()  in real time with several decoder contexts, each with its
+              own stream, so that batches overlap on the GPU's idle CUs;
+              completion is seen by polling the contexts.
+
+A frame's latency is its batch's completion time minus its arrival time.
 
 DVB-T2 QAM constellations are not among the DVB-S2/S2X MODCODs of the
 GPU channel; the T2 streams' LLRs come from its +-1 per bit channel at an
@@ -97,9 +103,13 @@ def schedule(arr, decode, max_batch=4096, slots=None):
     return done, batches
 
 
-def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, max_batch=4096, slots=8192):
-    """Generate the trace on the GPU and run the scheduler with real
-    decodes.  Returns latency percentiles and throughput."""
+def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, max_batch=4096, slots=8192,
+        lanes=None):
+    """Generate the trace on the GPU and decode it.  lanes=None: the
+    scheduler on the simulated clock with one decode at a time (each
+    batch's measured wall time advances the clock).  lanes=[decoders]:
+    live, in real time, with one batch in flight per decoder context
+    (run_live).  Returns latency percentiles and throughput."""
     from nms_train import DevBuf, codeword_pool
 
     arr, kind = arrivals(duration, load_scale, seed)
@@ -151,60 +161,167 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
         frames["llr_off"] = llr_off
         frames["out_off"] = out_off
         frames["max_iter"] = 50
+        decs = lanes if lanes else [dec]
         if learned:
-            for k, (pls, short, rate, fec, code, e) in enumerate(info):
-                t = (tables or {}).get(code)
-                if t is None:  # no trained table: the plain arithmetic
-                    dec.set_nms(code, np.full((1, ldpcgpu.NMS_DEGS), 16, np.uint8),
-                                np.full((1, ldpcgpu.NMS_DEGS), 16, np.int16))
-                else:
-                    dec.set_nms(code, t[0], t[1])
+            for dk in decs:
+                for k, (pls, short, rate, fec, code, e) in enumerate(info):
+                    t = (tables or {}).get(code)
+                    if t is None:  # no trained table: the plain arithmetic
+                        dk.set_nms(code, np.full((1, ldpcgpu.NMS_DEGS), 16, np.uint8),
+                                   np.full((1, ldpcgpu.NMS_DEGS), 16, np.int16))
+                    else:
+                        dk.set_nms(code, t[0], t[1])
         learned_frames = frames.copy()
         learned_frames["max_iter"] = 30
 
-        def decode(idx):
-            # idx: consecutive trace positions; results at those positions
-            i0 = int(idx[0])
-            t0 = time.perf_counter()
+        def enqueue(dk, pos):
+            # trace positions pos (ascending); iteration counts and flags
+            # land at those positions
+            i0, i1 = int(pos[0]), int(pos[-1]) + 1
+            assert i1 - i0 == len(pos)
             if learned:
-                dec.decode_batch_nms(learned_frames[idx], d_llr, d_out, d_it + 4 * i0, d_ok + i0, 50)
+                dk.decode_batch_nms(learned_frames[i0:i1], d_llr, d_out, d_it + 4 * i0, d_ok + i0, 50)
             else:
-                dec.decode_batch(frames[idx], d_llr, d_out, d_it + 4 * i0, d_ok + i0)
+                dk.decode_batch(frames[i0:i1], d_llr, d_out, d_it + 4 * i0, d_ok + i0)
+
+        def decode(idx):
+            t0 = time.perf_counter()
+            enqueue(dec, idx)
             dec.synchronize()
             return time.perf_counter() - t0
 
-        # warm the kernels
-        decode(np.arange(min(64, n)))
-        done, batches = schedule(arr, decode, max_batch, slots)
+        if lanes:
+            # one queue per launch kind (degree class x frame size), each
+            # context launching one kind at a time on its own stream
+            kkey = []
+            for k, inf in enumerate(info):
+                p = ldpcgpu.code_params(inf[4])
+                kkey.append((p["degree_class"], p["n"]))
+            key = np.array([kkey.index(kkey[k]) for k in kind])
+            # frames are renumbered so that each queue's frames (in arrival
+            # order) sit at consecutive positions
+            perm = np.concatenate([np.nonzero(key == u)[0] for u in np.unique(key)])
+            frames[:] = frames[perm]
+            learned_frames[:] = learned_frames[perm]
+            qstart = np.concatenate([[0], np.cumsum([int((key == u).sum()) for u in np.unique(key)])])
+            groups = [perm[qstart[g]:qstart[g + 1]] for g in range(len(qstart) - 1)]
+            for dk in decs:
+                dk.set_concurrent(False)
+            # grow every context's scratch to the largest batch first (a
+            # device allocation mid-run would wait for all streams): up to
+            # max_batch frames of each queue
+            for dk in decs:
+                for g in range(len(groups)):
+                    m = min(max_batch, len(groups[g]))
+                    enqueue(dk, np.arange(qstart[g], qstart[g] + m))
+                dk.synchronize()
+            done, batches = run_live(arr, decs, lambda dk, g, s0, e: enqueue(dk, np.arange(qstart[g] + s0, qstart[g] + e)),
+                                     groups, max_batch, slots)
+            # results sit at the renumbered positions
+            back = np.empty(n, np.int64)
+            back[perm] = np.arange(n)
+        else:
+            # warm the kernels
+            decode(np.arange(min(64, n)))
+            done, batches = schedule(arr, decode, max_batch, slots)
+            back = np.arange(n)
         out = np.zeros(int(((KL + 7) // 8).sum()), np.uint8)
         it = np.zeros(n, np.int32)
         dec.download(out, d_out)
         dec.download(it, d_it)
+        it = it[back]
     finally:
         dv.free()
         if learned:
-            for k, inf in enumerate(info):
-                dec.set_nms(inf[4], None, None)
+            for dk in (lanes or [dec]):
+                for k, inf in enumerate(info):
+                    dk.set_nms(inf[4], None, None)
+        for dk in (lanes or []):
+            dk.set_concurrent(True)
     lat = (done - arr) * 1e3
     errors = 0
+    nl = len(lanes) if lanes else 1
     for f in range(n):
         k = kind[f]
         ref = refs[k][rows_all[f]]
         o = out[out_off[f]:out_off[f] + len(ref)]
         errors += int(not np.array_equal(o, ref))
-    busy = sum(b[2] for b in batches)
-    span = max(done[-1], duration)
+    busy = sum(b[-1] for b in batches)
+    span = max(float(done.max()), duration)
     return {
         "decoder": "learned" if learned else "plain", "frames": int(n), "duration_s": duration,
-        "load_scale": load_scale, "device_slots": slots, "offered_gbit_s": round(float(KL.sum()) / duration / 1e9, 4),
+        "load_scale": load_scale, "device_slots": slots,
+        "scheduler": f"live, {nl} contexts in flight" if lanes else "simulated clock, one batch at a time",
+        "offered_gbit_s": round(float(KL.sum()) / duration / 1e9, 4),
         "decoded_gbit_s": round(float(KL.sum()) / span / 1e9, 4),
-        "capacity_gbit_s": round(float(KL.sum()) / busy / 1e9, 3),
-        "gpu_busy": round(busy / span, 4), "batches": len(batches),
+        "capacity_gbit_s": round(float(KL.sum()) / busy * nl / 1e9, 3),
+        "gpu_busy": round(busy / span / nl, 4), "batches": len(batches),
         "avg_batch": round(n / len(batches), 1), "avg_iterations": round(float(it.mean()), 3),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(lat.max()), 3), "mean": round(float(lat.mean()), 3)},
         "frame_errors_ldpc": errors,
     }
+
+
+def run_live(arr, decs, enqueue, groups=None, max_batch=4096, slots=None, poll_s=20e-6):
+    """Real-time scheduler with several decoder contexts.  Frames arrive at
+    arr (seconds from the start, sorted) on the wall clock; a frame is
+    admitted at its arrival, or when a device slot frees (slots frames
+    resident, as schedule()).  groups: lists of trace positions in arrival
+    order (frames of one launch kind each; default: one list of all).
+    Whenever a context is idle it takes the group whose oldest admitted
+    frame arrived first and batches all of that group's admitted frames
+    (up to max_batch): enqueue(context, group, s, e) enqueues positions
+    s..e-1 of the group without waiting.  Completion is seen by polling
+    each context (Decoder.query).  Returns per-frame completion times and
+    the batches (group, first, end, seconds from enqueue to completion)."""
+    n = len(arr)
+    groups = [np.arange(n)] if groups is None else [np.asarray(g) for g in groups]
+    done = np.full(n, np.inf)
+    batches = []
+    busy = [None] * len(decs)  # (group, s, e, enqueue time) per context
+    heads = [0] * len(groups)  # first position of each group not yet enqueued
+    finished = 0
+    t0 = time.perf_counter()
+
+    def admitted(j, now):
+        return arr[j] <= now and (slots is None or j < slots or done[j - slots] <= now)
+
+    while finished < n:
+        now = time.perf_counter() - t0
+        for k, dk in enumerate(decs):
+            b = busy[k]
+            if b is not None and dk.query():
+                now = time.perf_counter() - t0
+                done[groups[b[0]][b[1]:b[2]]] = now
+                batches.append((b[0], b[1], b[2], now - b[3]))
+                finished += b[2] - b[1]
+                busy[k] = None
+        for k, dk in enumerate(decs):
+            if busy[k] is not None:
+                continue
+            best = None
+            for g, idx in enumerate(groups):
+                h = heads[g]
+                if h < len(idx) and admitted(idx[h], now) and (best is None or arr[idx[h]] < arr[groups[best][heads[best]]]):
+                    best = g
+            if best is None:
+                break
+            idx = groups[best]
+            s0 = e = heads[best]
+            while e < len(idx) and e - s0 < max_batch and admitted(idx[e], now):
+                e += 1
+            enqueue(dk, best, s0, e)
+            busy[k] = (best, s0, e, now)
+            heads[best] = e
+        if all(b is None for b in busy):
+            pend = [arr[idx[heads[g]]] for g, idx in enumerate(groups) if heads[g] < len(idx)]
+            wait = (min(pend) if pend else 0.0) - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+        else:
+            time.sleep(poll_s)
+    return done, batches
 
 
 def learned_tables():
@@ -225,10 +342,13 @@ def main():
     ap.add_argument("--duration", type=float, default=1.0)
     ap.add_argument("--load", type=float, default=1.0)
     ap.add_argument("--learned", action="store_true")
+    ap.add_argument("--contexts", type=int, default=0, help="live scheduler with this many contexts (0: simulated clock)")
     a = ap.parse_args()
     dec = ldpcgpu.Decoder(0)
-    res = run(dec, a.duration, a.load, a.learned, learned_tables() if a.learned else None)
-    dec.close()
+    lanes = [dec] + [ldpcgpu.Decoder(0) for _ in range(a.contexts - 1)] if a.contexts > 0 else None
+    res = run(dec, a.duration, a.load, a.learned, learned_tables() if a.learned else None, lanes=lanes)
+    for d in lanes or [dec]:
+        d.close()
     print(json.dumps(res))
 
 

@@ -69,6 +69,11 @@ int ldpc_gpu_free(ldpc_gpu_ctx* ctx, void* d_ptr);
 int ldpc_gpu_upload(ldpc_gpu_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 int ldpc_gpu_download(ldpc_gpu_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 int ldpc_gpu_synchronize(ldpc_gpu_ctx* ctx);
+/* Non-blocking completion test (no counterpart in the reference, whose
+ * decode call returns when the batch is done): 1 if every call enqueued on
+ * the context has finished, 0 if work is pending, < 0 on error.  Lets one
+ * host thread keep batches in flight on several contexts at once. */
+int ldpc_gpu_query(ldpc_gpu_ctx* ctx);
 
 /*
  * Decode a batch (replaces drx_ldpc_decode, libdvbrx/src/ldpc.c:614-638).
@@ -146,6 +151,14 @@ void ldpc_gpu_plan_destroy(ldpc_gpu_plan* plan);
 #define LDPC_GPU_KERNEL_AUTO 0
 #define LDPC_GPU_KERNEL_SINGLE 1
 int ldpc_gpu_set_kernel(ldpc_gpu_ctx* ctx, int kernel);
+
+/* Launch mode of ldpc_gpu_decode_batch (no counterpart in the reference).
+ * 1 (the default): the launches of one call - one per degree class and
+ * frame size - run concurrently on forked streams.  0: one after the other
+ * on the context's stream, so that the context occupies a single hardware
+ * queue; for callers that keep several contexts in flight at once, each
+ * with batches of one launch kind. */
+int ldpc_gpu_set_concurrent(ldpc_gpu_ctx* ctx, int concurrent);
 
 /* Timing of the last ldpc_gpu_decode_batch's decode kernel on the stream it
  * ran on (HIP events around the launch), in milliseconds; < 0 if none. */

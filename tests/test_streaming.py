@@ -4,6 +4,8 @@ device-slot admission."""
 import os
 import sys
 
+import time
+
 import numpy as np
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dvb-rx_amd"))
@@ -45,3 +47,67 @@ def test_schedule_max_batch_and_slots():
     done, batches = streaming.schedule(arr, lambda idx: 1.0, max_batch=8, slots=3)
     assert [b[1] - b[0] for b in batches] == [3, 3, 3, 1]
     assert np.allclose(done, [1] * 3 + [2] * 3 + [3] * 3 + [4])
+
+
+class _FakeContext:
+    """Stand-in decoder context: a batch enqueued at time t completes at
+    t + dur on the wall clock."""
+
+    def __init__(self, dur):
+        self.dur = dur
+        self.until = 0.0
+        self.batches = []
+
+    def start(self, i0, i1):
+        import time
+
+        self.until = time.perf_counter() + self.dur
+        self.batches.append((i0, i1))
+
+    def query(self):
+        import time
+
+        return time.perf_counter() >= self.until
+
+
+def test_live_scheduler_overlaps_contexts():
+    """Two contexts, 50 ms per batch, frames every 20 ms: a frame never
+    waits for a busy context while the other is idle, so every latency is
+    about one batch time and the batches alternate between contexts."""
+    arr = np.arange(8) * 0.02
+    ctx = [_FakeContext(0.05), _FakeContext(0.05)]
+    done, batches = streaming.run_live(arr, ctx, lambda dk, g, i0, i1: dk.start(i0, i1), max_batch=64)
+    lat = done - arr
+    assert np.isfinite(done).all()
+    assert sum(b[2] - b[1] for b in batches) == 8
+    assert (lat >= 0.05 - 1e-3).all() and (lat < 0.05 + 0.045).all(), lat
+    assert ctx[0].batches and ctx[1].batches
+    seen = sorted(i for c in ctx for b in c.batches for i in range(*b))
+    assert seen == list(range(8))
+
+
+def test_live_scheduler_single_context_batches_and_slots():
+    """One context: frames that arrive while it is busy form the next
+    batch; with 2 device slots a frame waits until frame j-2 completes."""
+    arr = np.zeros(5)
+    ctx = [_FakeContext(0.02)]
+    done, batches = streaming.run_live(arr, ctx, lambda dk, g, i0, i1: dk.start(i0, i1), max_batch=64, slots=2)
+    assert [b for b in ctx[0].batches] == [(0, 2), (2, 4), (4, 5)]
+    assert (np.diff(done[[0, 2, 4]]) >= 0.02 - 1e-3).all()
+
+
+def test_live_scheduler_queues_by_kind():
+    """Two queues (even and odd frames), one context, all frames at t=0:
+    each batch holds one queue's frames, the queue with the older head
+    first, and every frame completes once."""
+    arr = np.linspace(0.0, 0.001, 6)
+    groups = [np.array([0, 2, 4]), np.array([1, 3, 5])]
+    ctx = [_FakeContext(0.01)]
+    seen = []
+    time.sleep(0.002)
+    done, batches = streaming.run_live(arr, ctx, lambda dk, g, s, e: (seen.append((g, s, e)), dk.start(s, e)),
+                                       groups, max_batch=64)
+    assert np.isfinite(done).all()
+    assert [b[0] for b in batches] == [g for g, _, _ in seen]
+    assert sorted(i for g, s, e in seen for i in groups[g][s:e]) == list(range(6))
+    assert seen[0][0] == 0
