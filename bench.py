@@ -247,6 +247,41 @@ def config3(dec, margin_db=0.2, reps=5):
     }
 
 
+def demap_record(dec, nframes=512, reps=5):
+    """GPU max-log demapper (ldpc_gpu_demap) on 512 frames of random
+    unit-variance symbols per constellation, resident in HBM: Gsym/s and
+    the HBM fraction (8 bytes in per symbol, one LLR byte out per bit)."""
+    out = []
+    for pls in (0x10, 0x30, 0x60, 0xc8, 0xcc, 0xce):
+        ns = ldpcgpu.demap_symbols(pls)
+        m = ldpcgpu.pls_lookup(pls)
+        nldpc = 16200 if m["short_frame"] else 64800
+        rng = np.random.default_rng(pls)
+        sym = rng.normal(0, 0.7, (nframes, ns, 2)).astype(np.float32)
+        n0 = np.full(nframes, 0.1, np.float32)
+        d_sym, d_n0, d_llr = dec.malloc(sym.nbytes), dec.malloc(n0.nbytes), dec.malloc(nframes * nldpc)
+        try:
+            dec.upload(d_sym, sym)
+            dec.upload(d_n0, n0)
+            ldpcgpu.demap(dec, pls, nframes, d_sym, d_n0, d_llr)
+            dec.synchronize()
+            dt = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                ldpcgpu.demap(dec, pls, nframes, d_sym, d_n0, d_llr)
+                dec.synchronize()
+                dt.append(time.perf_counter() - t)
+            sec = float(np.median(dt))
+            hbm = nframes * (8 * ns + nldpc)
+            out.append({"pls": f"{pls:#04x}", "constellation": m["constellation"], "rate": m["rate"],
+                        "frames": nframes, "ms": round(sec * 1e3, 4), "gsym_s": round(nframes * ns / sec / 1e9, 3),
+                        "hbm_gb_s": round(hbm / sec / 1e9, 1), "hbm_frac": round(hbm / sec / 1e9 / HBM_PEAK_GBS, 4)})
+        finally:
+            for p_ in (d_sym, d_n0, d_llr):
+                dec.free(p_)
+    return out
+
+
 def config4(dec, nf=8192, db=0.2, reps=3):
     """Learned vs plain decoding on the two trained MODCODs (DVB-S2 normal
     3/5 8PSK and 2/3 16APSK) at Es/N0 = QEF + db: frames from pool
@@ -445,6 +480,7 @@ def main():
     c1 = None if args.no_config1 else config1(dec)
     c4 = None if args.no_config1 else config4(dec)
     c3 = None if args.no_config1 else [config3(dec, 0.2), config3(dec, 1.0)]
+    dm = None if args.no_config1 else demap_record(dec)
     c5 = None
     if not args.no_config1:
         import streaming
@@ -520,6 +556,7 @@ def main():
         "learned_vs_plain": c4,
         "s2x_high_order_graph": c3,
         "streaming": c5,
+        "demapper": dm,
     }
     if cpu:
         res["gpu_vs_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None

@@ -850,6 +850,13 @@ static int set_err(ldpc_gpu_ctx* c, int code, const char* fmt, ...)
     return code;
 }
 
+/* for the library's other translation units (ldpc_internal.h) */
+__attribute__((visibility("hidden"))) int lg_ctx_device(ldpc_gpu_ctx* c) { return c->device; }
+__attribute__((visibility("hidden"))) int lg_ctx_fail(ldpc_gpu_ctx* c, int code, const char* msg)
+{
+    return set_err(c, code, "%s", msg);
+}
+
 #define HIPCHK(ctx, call)                                                                       \
     do {                                                                                        \
         hipError_t e_ = (call);                                                                 \

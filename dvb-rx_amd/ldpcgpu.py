@@ -75,6 +75,8 @@ def gpu_lib():
         L.ldpc_gpu_synchronize.argtypes = [vp]
         L.ldpc_gpu_query.argtypes = [vp]
         L.ldpc_gpu_set_concurrent.argtypes = [vp, C.c_int]
+        L.ldpc_gpu_demap_symbols.argtypes = [C.c_int]
+        L.ldpc_gpu_demap.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.ldpc_gpu_decode_batch.argtypes = [vp, vp, i32, vp, vp, vp, vp]
         L.ldpc_gpu_decode_host.argtypes = [vp, vp, i32, vp, sz, vp, sz, vp, vp]
         L.ldpc_gpu_last_kernel_ms.argtypes = [vp]
@@ -443,6 +445,16 @@ def channel(pls, n, d_cw, d_row, nframes, d_esn0_db, seed, d_llr, stream=None):
     rc = gpu_lib().ldpc_gpu_channel(pls, n, d_cw, d_row, nframes, d_esn0_db, seed, d_llr, stream)
     if rc:
         raise RuntimeError(f"ldpc_gpu_channel: error {rc}")
+
+
+def demap_symbols(pls):
+    """Symbols per frame of a MODCOD (ldpc_gpu_demap_symbols), -1 unknown."""
+    return gpu_lib().ldpc_gpu_demap_symbols(pls)
+
+
+def demap(dec, pls, nframes, d_sym, d_n0, d_llr):
+    """ldpc_gpu_demap on the decoder context's stream (enqueue only)."""
+    dec._check(gpu_lib().ldpc_gpu_demap(dec.ctx, pls, nframes, d_sym, d_n0, d_llr), "demap")
 
 
 class DrxLdpc:

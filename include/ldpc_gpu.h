@@ -144,6 +144,21 @@ int ldpc_gpu_plan_info(ldpc_gpu_plan* plan, int* variant, float* scale, int* con
 int ldpc_gpu_plan_select(ldpc_gpu_plan* plan, int variant);
 void ldpc_gpu_plan_destroy(ldpc_gpu_plan* plan);
 
+/*
+ * Max-log demapping of received symbols into the decoder's input (replaces
+ * drx_demap_frame, libdvbrx/src/demap.c:491-532, which demaps one frame
+ * per call; drx_demap_new's constellation lookup, :164-203, is by PLS code
+ * here).  nframes frames of MODCOD pls (PLS code, pilot bit ignored):
+ *   d_sym  nframes x ldpc_gpu_demap_symbols(pls) complex symbols as (re, im)
+ *          float pairs, at unit average energy (the AGC's output);
+ *   d_n0   nframes noise variances;
+ *   d_llr  nframes x nldpc int8 LLRs (64800 or 16200 each, back to back),
+ *          codeword order (bit interleaver undone), positive for a 0 bit.
+ * Identical LLRs to the reference's AVX2/FMA build.  Enqueues only.
+ */
+int ldpc_gpu_demap_symbols(int pls);
+int ldpc_gpu_demap(ldpc_gpu_ctx* ctx, int pls, int nframes, const float* d_sym, const float* d_n0, int8_t* d_llr);
+
 /* Kernel choice for frames with their own early stop (group -1):
  * LDPC_GPU_KERNEL_AUTO runs two frames of one code per workgroup in packed
  * 16-bit arithmetic; LDPC_GPU_KERNEL_SINGLE one frame per workgroup (the

@@ -636,3 +636,167 @@ out:
     free(cw);
     return loss;
 }
+
+/* ---- max-log demapper ------------------------------------------------------
+ *
+ * Restates drx_demap_new / drx_demap_frame (libdvbrx/src/demap.c:164-203,
+ * :491-532) as the reference's AVX2/FMA build computes them:
+ *   points normalised to unit average energy (the normalisation in double),
+ *   metric t_p = x * ar_p + (y * ai_p + e_p) with ar = -2 re, ai = -2 im,
+ *   e = |p|^2, both multiply-adds fused (minima_avx / minima_grp, :318-425);
+ *   for the 8-ring x 32-phase constellations the ring metric of
+ *   minima_ring (:457-489): c_j = x rc_j + y rs_j (one fused step), ring
+ *   metric c_j * (-2 R_r) + R_r^2 (fused), phase minima over rings, ring
+ *   minima at the largest projection;
+ *   LLR of label bit b (MSB first) = rint(clamp(k (m1_b - m0_b), +-127)),
+ *   k = 2 / n0, stored at order[b] * nsym + j (bit interleaver), positions
+ *   past nldpc dropped (128APSK);
+ *   QPSK (no interleaver): LLR = rint(clamp(k * 4 * sqrt(1/2) * x)) per
+ *   axis (:495-500).
+ */
+typedef struct {
+    int bits, np, nsym, nldpc, order[8], ring;
+    float re[256], im[256], ar[256], ai[256], e[256];
+    float rc[32], rs[32], rm2[8], rr2[8];
+} odemap;
+
+static int odemap_init(odemap* d, int pls)
+{
+    const struct lg_modcod_row* row = NULL;
+    for (size_t i = 0; i < sizeof(LG_MODCODS) / sizeof(*LG_MODCODS); i++)
+        if (LG_MODCODS[i].pls == (pls & 0xFE))
+            row = &LG_MODCODS[i];
+    if (!row)
+        return -1;
+    d->bits = row->bits;
+    d->np = 1 << row->bits;
+    d->nldpc = row->short_frame ? 16200 : 64800;
+    d->nsym = (d->nldpc + d->bits - 1) / d->bits;
+    memcpy(d->order, row->order, sizeof(d->order));
+    const float* pt = LG_POINTS + 2 * row->point_off;
+    double en = 0;
+    for (int p = 0; p < d->np; p++) {
+        const float a = pt[2 * p] * pt[2 * p], b = pt[2 * p + 1] * pt[2 * p + 1];
+        en += (double)(a + b);
+    }
+    const float g = (float)(1.0 / sqrt(en / d->np));
+    float* re = d->re;
+    float* im = d->im;
+    for (int p = 0; p < d->np; p++) {
+        re[p] = pt[2 * p] * g;
+        im[p] = pt[2 * p + 1] * g;
+        d->ar[p] = -2 * re[p];
+        d->ai[p] = -2 * im[p];
+        d->e[p] = fmaf(re[p], re[p], im[p] * im[p]);
+    }
+    /* 8 rings of the same 32 phases: point p = R[p >> 5] (rc, rs)[p & 31] */
+    d->ring = 0;
+    if (d->bits == 8) {
+        for (int r = 0; r < 8; r++) {
+            const float R = hypotf(re[r << 5], im[r << 5]);
+            d->rm2[r] = -2 * R;
+            d->rr2[r] = R * R;
+        }
+        const float R0 = hypotf(re[0], im[0]);
+        for (int j = 0; j < 32; j++) {
+            d->rc[j] = re[j] / R0;
+            d->rs[j] = im[j] / R0;
+        }
+        d->ring = 1;
+        for (int p = 0; p < d->np && d->ring; p++) {
+            const float R = -0.5f * d->rm2[p >> 5];
+            if (fabsf(R * d->rc[p & 31] - re[p]) > 1e-5f || fabsf(R * d->rs[p & 31] - im[p]) > 1e-5f)
+                d->ring = 0;
+        }
+    }
+    return 0;
+}
+
+static int8_t odemap_q(float v)
+{
+    v = v > 127.0f ? 127.0f : v < -127.0f ? -127.0f : v;
+    return (int8_t)lrintf(v);
+}
+
+int oracle_demap_symbols(int pls)
+{
+    odemap d;
+    return odemap_init(&d, pls) ? -1 : d.nsym;
+}
+
+int oracle_demap_points(int pls, float* re, float* im)
+{
+    odemap d;
+    if (odemap_init(&d, pls))
+        return -1;
+    memcpy(re, d.re, sizeof(float) * (size_t)d.np);
+    memcpy(im, d.im, sizeof(float) * (size_t)d.np);
+    return d.np;
+}
+
+int oracle_demap_ring(int pls)
+{
+    odemap d;
+    return odemap_init(&d, pls) ? -1 : d.ring;
+}
+
+int oracle_demap_frame(int pls, const float* sym, float n0, int8_t* llr)
+{
+    odemap d;
+    if (odemap_init(&d, pls))
+        return -1;
+    const float k = 2.0f / n0;
+    if (d.order[0] < 0) {
+        const float kq = k * 4.0f * 0.70710678118654752440f;
+        for (int i = 0; i < 2 * d.nsym; i++)
+            llr[i] = odemap_q(kq * sym[i]);
+        return 0;
+    }
+    const int nb = d.bits;
+    for (int j = 0; j < d.nsym; j++) {
+        const float x = sym[2 * j], y = sym[2 * j + 1];
+        float m0[8], m1[8];
+        for (int b = 0; b < nb; b++)
+            m0[b] = m1[b] = INFINITY;
+        if (d.ring) {
+            float c[32], cmax = -INFINITY;
+            for (int q = 0; q < 32; q++) {
+                c[q] = fmaf(x, d.rc[q], y * d.rs[q]);
+                cmax = c[q] > cmax ? c[q] : cmax;
+            }
+            for (int q = 0; q < 32; q++) {
+                float pm = INFINITY;
+                for (int r = 0; r < 8; r++) {
+                    const float t = fmaf(c[q], d.rm2[r], d.rr2[r]);
+                    pm = t < pm ? t : pm;
+                }
+                /* phase index bit L is label bit 7 - L */
+                for (int L = 0; L < 5; L++) {
+                    float* m = (q >> L) & 1 ? m1 : m0;
+                    m[7 - L] = pm < m[7 - L] ? pm : m[7 - L];
+                }
+            }
+            for (int r = 0; r < 8; r++) {
+                const float t = fmaf(cmax, d.rm2[r], d.rr2[r]);
+                for (int L = 0; L < 3; L++) {
+                    float* m = (r >> L) & 1 ? m1 : m0;
+                    m[2 - L] = t < m[2 - L] ? t : m[2 - L];
+                }
+            }
+        } else {
+            for (int p = 0; p < d.np; p++) {
+                const float t = fmaf(x, d.ar[p], fmaf(y, d.ai[p], d.e[p]));
+                for (int b = 0; b < nb; b++) {
+                    float* m = (p >> (nb - 1 - b)) & 1 ? m1 : m0;
+                    m[b] = t < m[b] ? t : m[b];
+                }
+            }
+        }
+        for (int b = 0; b < nb; b++) {
+            const int pos = d.order[b] * d.nsym + j;
+            if (pos < d.nldpc)
+                llr[pos] = odemap_q(k * (m1[b] - m0[b]));
+        }
+    }
+    return 0;
+}

@@ -93,4 +93,20 @@ double oracle_nms_train(oracle_ldpc* d, int mode, const float* llr, const uint8_
                         const float* alpha, const float* beta, const float* w, float temp, float* galpha,
                         float* gbeta, float* post);
 
+/* ---- max-log demapper (libdvbrx/src/demap.c:164-203, :491-532) --------
+ *
+ * One frame of MODCOD pls (PLS code, pilot bit ignored): sym holds the
+ * frame's symbols as (re, im) float pairs at unit average energy,
+ * oracle_demap_symbols(pls) of them; n0 the noise variance; llr receives
+ * nldpc int8 LLRs in codeword order (positive for a 0 bit), as the
+ * reference's AVX2/FMA build computes them.  Returns 0, or -1 for an
+ * unknown pls.  oracle_demap_ring: 1 if the constellation takes the
+ * ring metric (8 rings of 32 phases), 0 if not, -1 unknown. */
+int oracle_demap_symbols(int pls);
+int oracle_demap_ring(int pls);
+/* the constellation at unit average energy, by label: returns the point
+ * count (re, im hold up to 256), -1 unknown */
+int oracle_demap_points(int pls, float* re, float* im);
+int oracle_demap_frame(int pls, const float* sym, float n0, int8_t* llr);
+
 #endif

@@ -65,8 +65,103 @@ def oracle():
         L.oracle_nms_decode_frames.argtypes = [vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
         L.oracle_nms_train.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp, C.c_float, vp, vp, vp]
         L.oracle_nms_train.restype = C.c_double
+        L.oracle_demap_symbols.argtypes = [C.c_int]
+        L.oracle_demap_ring.argtypes = [C.c_int]
+        L.oracle_demap_points.argtypes = [C.c_int, vp, vp]
+        L.oracle_demap_frame.argtypes = [C.c_int, vp, C.c_float, vp]
         _oracle = L
     return _oracle
+
+
+def demap_pls_codes():
+    """Every PLS code (pilot bit clear) with a constellation."""
+    out = []
+    for pls in range(0, 0x100, 2):
+        if oracle().oracle_demap_symbols(pls) > 0:
+            out.append(pls)
+    return out
+
+
+def demap_points(pls):
+    """Unit-energy constellation of a MODCOD by label: complex array."""
+    re = np.zeros(256, np.float32)
+    im = np.zeros(256, np.float32)
+    n = oracle().oracle_demap_points(pls, re.ctypes.data, im.ctypes.data)
+    assert n > 0, hex(pls)
+    return re[:n] + 1j * im[:n]
+
+
+def demap_order(pls):
+    """The bit interleaver's column per label bit (MSB first), read back
+    from the oracle demapper: moving symbol 0 from the point labelled 0 to
+    the one whose label has bit b alone set flips the sign of one LLR, at
+    order[b] * nsym; None for QPSK (no interleaver: bits 2j, 2j+1)."""
+    if ldpcgpu.pls_lookup(pls)["constellation"] == "QPSK":
+        return None
+    pts = demap_points(pls)
+    nb = int(np.log2(len(pts)))
+    ns = oracle().oracle_demap_symbols(pls)
+    base = np.zeros((ns, 2), np.float32)
+    base[:] = [pts[0].real, pts[0].imag]
+    ref = oracle_demap(pls, base, 1e-3)
+    order = []
+    for b in range(nb):
+        s = base.copy()
+        p = pts[1 << (nb - 1 - b)]
+        s[0] = [p.real, p.imag]
+        diff = np.nonzero((oracle_demap(pls, s, 1e-3) < 0) != (ref < 0))[0]
+        flips = [int(d) for d in diff if d % ns == 0]
+        assert len(flips) == 1, (hex(pls), b)
+        order.append(flips[0] // ns)
+    return order
+
+
+def modulate(pls, cw, esn0_db, rng):
+    """Codeword bits (N,) -> noisy unit-energy symbols (nsym, 2) float32 and
+    the noise variance: label bit b of symbol j is codeword bit
+    order[b] * nsym + j (zero past the codeword), AWGN of variance n0."""
+    pts = demap_points(pls)
+    nb = int(np.log2(len(pts)))
+    ns = oracle().oracle_demap_symbols(pls)
+    order = demap_order(pls)
+    N = len(cw)
+    lab = np.zeros(ns, np.int64)
+    for b in range(nb):
+        pos = (2 * np.arange(ns) + b) if order is None else order[b] * ns + np.arange(ns)
+        bit = np.where(pos < N, cw[np.minimum(pos, N - 1)], 0)
+        lab = lab * 2 + bit
+    n0 = 10 ** (-esn0_db / 10)
+    y = pts[lab] + np.sqrt(n0 / 2) * (rng.normal(size=ns) + 1j * rng.normal(size=ns))
+    return np.stack([y.real, y.imag], 1).astype(np.float32), np.float32(n0)
+
+
+def oracle_demap(pls, sym, n0):
+    """Oracle LLRs of one frame: sym (nsym, 2) float32 (re, im)."""
+    sym = np.ascontiguousarray(sym, np.float32)
+    nldpc = 16200 if ldpcgpu.pls_lookup(pls)["short_frame"] else 64800
+    llr = np.zeros(nldpc, np.int8)
+    assert oracle().oracle_demap_frame(pls, sym.ctypes.data, np.float32(n0), llr.ctypes.data) == 0
+    return llr
+
+
+def ref_demap(R, pls, sym, n0):
+    """drx_demap_frame of a reference build on one frame."""
+    m = ldpcgpu.pls_lookup(pls)
+    R.drx_demap_new.restype = C.c_void_p
+    R.drx_demap_new.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+    R.drx_demap_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]
+    R.drx_demap_symbols.argtypes = [C.c_void_p]
+    R.drx_demap_free.argtypes = [C.c_void_p]
+    d = R.drx_demap_new(m["constellation"].encode(), m["rate"].encode(), m["short_frame"])
+    assert d, hex(pls)
+    try:
+        sym = np.ascontiguousarray(sym, np.float32)
+        assert R.drx_demap_symbols(d) == len(sym)
+        llr = np.zeros(16200 if m["short_frame"] else 64800, np.int8)
+        R.drx_demap_frame(d, sym.ctypes.data, np.float32(n0), llr.ctypes.data)
+        return llr
+    finally:
+        R.drx_demap_free(d)
 
 
 class Oracle:
