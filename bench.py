@@ -282,6 +282,44 @@ def demap_record(dec, nframes=512, reps=5):
     return out
 
 
+def bch_record(dec, nframes=8192, reps=5):
+    """GPU BCH decoder (ldpc_gpu_bch_decode) on 8192 codewords of DVB-S2
+    normal 1/2 (t = 12) resident in HBM: all clean (the usual case after
+    LDPC decoding), and every codeword with t errors (the Berlekamp-Massey
+    and Chien search path on all of them)."""
+    fec = ldpcgpu.fec_lookup(0, "1_2")
+    # one BCH codeword of the frame source (noiseless frame: its reference bits)
+    _, ref, _ = ldpcgpu.generate([dict(short_frame=0, rate="1_2", pls=-1, esn0_db=1000.0, stream=0)], 5)
+    cw = np.asarray(ref, np.uint8).reshape(-1)
+    nb = len(cw)
+    rng = np.random.default_rng(5)
+    out = []
+    for name, w in (("clean", 0), (f"{fec.t} errors each", fec.t)):
+        buf = np.tile(cw, (nframes, 1))
+        for f in range(nframes if w else 0):
+            for bit in rng.choice(fec.kldpc, w, replace=False):
+                buf[f, bit // 8] ^= 0x80 >> (bit % 8)
+        d_cw, d_res = dec.malloc(buf.nbytes), dec.malloc(4 * nframes)
+        try:
+            dt = []
+            for _ in range(reps + 1):
+                dec.upload(d_cw, buf)
+                t = time.perf_counter()
+                ldpcgpu.bch_decode(dec, 0, "1_2", nframes, d_cw, nb, d_res)
+                dec.synchronize()
+                dt.append(time.perf_counter() - t)
+            res = np.zeros(nframes, np.int32)
+            dec.download(res, d_res)
+            sec = float(np.median(dt[1:]))
+            out.append({"codewords": name, "frames": nframes, "ms": round(sec * 1e3, 3),
+                        "mframes_s": round(nframes / sec / 1e6, 3), "info_gbit_s": round(nframes * fec.kbch / sec / 1e9, 2),
+                        "corrected": int((res == w).sum())})
+        finally:
+            dec.free(d_cw)
+            dec.free(d_res)
+    return out
+
+
 def config4(dec, nf=8192, db=0.2, reps=3):
     """Learned vs plain decoding on the two trained MODCODs (DVB-S2 normal
     3/5 8PSK and 2/3 16APSK) at Es/N0 = QEF + db: frames from pool
@@ -481,6 +519,7 @@ def main():
     c4 = None if args.no_config1 else config4(dec)
     c3 = None if args.no_config1 else [config3(dec, 0.2), config3(dec, 1.0)]
     dm = None if args.no_config1 else demap_record(dec)
+    bch = None if args.no_config1 else bch_record(dec)
     c5 = None
     if not args.no_config1:
         import streaming
@@ -557,6 +596,7 @@ def main():
         "s2x_high_order_graph": c3,
         "streaming": c5,
         "demapper": dm,
+        "bch": bch,
     }
     if cpu:
         res["gpu_vs_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None

@@ -77,6 +77,7 @@ def gpu_lib():
         L.ldpc_gpu_set_concurrent.argtypes = [vp, C.c_int]
         L.ldpc_gpu_demap_symbols.argtypes = [C.c_int]
         L.ldpc_gpu_demap.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+        L.ldpc_gpu_bch_decode.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, vp, C.c_size_t, vp]
         L.ldpc_gpu_decode_batch.argtypes = [vp, vp, i32, vp, vp, vp, vp]
         L.ldpc_gpu_decode_host.argtypes = [vp, vp, i32, vp, sz, vp, sz, vp, vp]
         L.ldpc_gpu_last_kernel_ms.argtypes = [vp]
@@ -455,6 +456,12 @@ def demap_symbols(pls):
 def demap(dec, pls, nframes, d_sym, d_n0, d_llr):
     """ldpc_gpu_demap on the decoder context's stream (enqueue only)."""
     dec._check(gpu_lib().ldpc_gpu_demap(dec.ctx, pls, nframes, d_sym, d_n0, d_llr), "demap")
+
+
+def bch_decode(dec, short_frame, rate, nframes, d_cw, stride, d_result):
+    """ldpc_gpu_bch_decode on the decoder context's stream (enqueue only)."""
+    dec._check(gpu_lib().ldpc_gpu_bch_decode(dec.ctx, short_frame, rate.encode(), nframes, d_cw, stride, d_result),
+               "bch_decode")
 
 
 class DrxLdpc:

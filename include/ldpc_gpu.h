@@ -159,6 +159,19 @@ void ldpc_gpu_plan_destroy(ldpc_gpu_plan* plan);
 int ldpc_gpu_demap_symbols(int pls);
 int ldpc_gpu_demap(ldpc_gpu_ctx* ctx, int pls, int nframes, const float* d_sym, const float* d_n0, int8_t* d_llr);
 
+/*
+ * BCH decoding of LDPC decoder output (replaces drx_bch_decode,
+ * libdvbrx/src/bch.c:189-290, one codeword per call there; drx_bch_new's
+ * per-FEC set-up, :106-172, is by (frame size, rate) here, rate names as
+ * drx_fec_lookup).  nframes codewords, each nbch (= kldpc) bits packed MSB
+ * first at d_cw + f * stride (stride >= nbch / 8 bytes: the decoder's
+ * output with nbits = kldpc), corrected in place.  d_result[f]: the number
+ * of corrected bits, 0 for a codeword, -1 if uncorrectable (the codeword is
+ * left as it was), as the reference returns.  Enqueues only.
+ */
+int ldpc_gpu_bch_decode(ldpc_gpu_ctx* ctx, int short_frame, const char* rate, int nframes, uint8_t* d_cw,
+                        size_t stride, int32_t* d_result);
+
 /* Kernel choice for frames with their own early stop (group -1):
  * LDPC_GPU_KERNEL_AUTO runs two frames of one code per workgroup in packed
  * 16-bit arithmetic; LDPC_GPU_KERNEL_SINGLE one frame per workgroup (the

@@ -800,3 +800,150 @@ int oracle_demap_frame(int pls, const float* sym, float n0, int8_t* llr)
     }
     return 0;
 }
+
+/* ---- BCH decoder -----------------------------------------------------------
+ *
+ * Restates drx_bch_new / drx_bch_decode (libdvbrx/src/bch.c:106-172,
+ * :189-290): GF(2^16) (normal frames, x^16+x^5+x^3+x^2+1) or GF(2^14)
+ * (short, x^14+x^5+x^3+x+1); the codeword bit at index i (MSB first) has
+ * degree nbch - 1 - i.  The reference tests g(x) | c(x) by a remainder and
+ * derives the odd syndromes from it; here the syndromes S_j = c(alpha^j),
+ * j = 1 .. 2t, come straight from the bits (all zero exactly when g divides
+ * c, g being the product of the minimal polynomials of alpha^1, alpha^3,
+ * ..., alpha^(2t-1)).  Then Berlekamp-Massey; more than t errors -> -1; a
+ * Chien search over the nbch positions (root at alpha^-p: error at degree
+ * p); a root count different from the locator's degree -> -1; else the
+ * bits are flipped and their count returned.
+ */
+typedef struct {
+    int m, q, t, nbch;
+    uint16_t* exp; /* 2q entries */
+    int* log;
+} obch;
+
+static int obch_init(obch* b, int short_frame, const char* rate)
+{
+    const struct lg_fec_row* row = NULL;
+    const size_t n = strlen(rate);
+    for (size_t i = 0; i < sizeof(LG_FEC) / sizeof(*LG_FEC); i++) {
+        const char* r = LG_FEC[i].rate;
+        const int t2 = n > 3 && !strcmp(rate + n - 3, "_t2");
+        if (LG_FEC[i].short_frame == (short_frame != 0) &&
+            (t2 ? strlen(r) == n - 3 && !strncmp(r, rate, n - 3) : !strcmp(r, rate)))
+            row = &LG_FEC[i];
+    }
+    if (!row)
+        return -1;
+    b->m = short_frame ? 14 : 16;
+    b->q = (1 << b->m) - 1;
+    b->t = row->t;
+    b->nbch = row->nbch;
+    const unsigned prim = short_frame ? 0x402Bu : 0x1002Du;
+    b->exp = malloc(sizeof(uint16_t) * 2 * (size_t)b->q);
+    b->log = malloc(sizeof(int) * (size_t)(b->q + 1));
+    if (!b->exp || !b->log) {
+        free(b->exp);
+        free(b->log);
+        return -1;
+    }
+    unsigned x = 1;
+    for (int i = 0; i < b->q; i++) {
+        b->exp[i] = b->exp[i + b->q] = (uint16_t)x;
+        b->log[x] = i;
+        x <<= 1;
+        if (x >> b->m)
+            x ^= prim;
+    }
+    b->log[0] = -1;
+    return 0;
+}
+
+static uint16_t obch_mul(const obch* b, uint16_t x, uint16_t y)
+{
+    return x && y ? b->exp[b->log[x] + b->log[y]] : 0;
+}
+
+int oracle_bch_nbch(int short_frame, const char* rate)
+{
+    obch b;
+    if (obch_init(&b, short_frame, rate))
+        return -1;
+    free(b.exp);
+    free(b.log);
+    return b.nbch;
+}
+
+int oracle_bch_decode(int short_frame, const char* rate, uint8_t* cw)
+{
+    obch b;
+    if (obch_init(&b, short_frame, rate))
+        return -2;
+    const int t = b.t, q = b.q;
+    uint16_t S[2 * 12 + 1] = { 0 };
+    for (int i = 0; i < b.nbch; i++) {
+        if (!((cw[i >> 3] >> (7 - (i & 7))) & 1))
+            continue;
+        const long long p = b.nbch - 1 - i;
+        for (int j = 1; j <= 2 * t; j++)
+            S[j] ^= b.exp[(int)((j * p) % q)];
+    }
+    int any = 0;
+    for (int j = 1; j <= 2 * t; j++)
+        any |= S[j];
+    int res = 0;
+    if (any) {
+        /* Berlekamp-Massey: L(x) the shortest connection polynomial */
+        uint16_t L[14] = { 1 }, B[14] = { 1 }, T[14];
+        int len = 0, shift = 1;
+        uint16_t bd = 1;
+        for (int k = 0; k < 2 * t; k++) {
+            uint16_t d = S[k + 1];
+            for (int i = 1; i <= len; i++)
+                d ^= obch_mul(&b, L[i], S[k + 1 - i]);
+            if (!d) {
+                shift++;
+                continue;
+            }
+            const uint16_t c = obch_mul(&b, d, b.exp[q - b.log[bd]]);
+            memcpy(T, L, sizeof(T));
+            for (int i = 0; i + shift <= t + 1; i++)
+                L[i + shift] ^= obch_mul(&b, c, B[i]);
+            if (2 * len <= k) {
+                len = k + 1 - len;
+                memcpy(B, T, sizeof(B));
+                bd = d;
+                shift = 1;
+            } else {
+                shift++;
+            }
+        }
+        if (len > t) {
+            res = -1;
+        } else {
+            int pos[13], nroots = 0;
+            for (int p = 0; p < b.nbch; p++) {
+                uint16_t sum = L[0];
+                for (int k = 1; k <= len; k++)
+                    if (L[k])
+                        sum ^= b.exp[(int)((b.log[L[k]] + (long long)(q - k) * p) % q)];
+                if (!sum) {
+                    if (nroots < 13)
+                        pos[nroots] = p;
+                    nroots++;
+                }
+            }
+            if (nroots != len) {
+                res = -1;
+            } else {
+                for (int k = 0; k < nroots; k++) {
+                    const int bit = b.nbch - 1 - pos[k];
+                    cw[bit >> 3] ^= (uint8_t)(0x80 >> (bit & 7));
+                }
+                res = nroots;
+            }
+        }
+    }
+    free(b.exp);
+    free(b.log);
+    return res;
+}

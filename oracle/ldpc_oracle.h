@@ -109,4 +109,14 @@ int oracle_demap_ring(int pls);
 int oracle_demap_points(int pls, float* re, float* im);
 int oracle_demap_frame(int pls, const float* sym, float n0, int8_t* llr);
 
+/* ---- BCH decoder (libdvbrx/src/bch.c:106-172, :189-290) ---------------
+ *
+ * One codeword of the FEC (short_frame, rate as drx_fec_lookup): nbch bits
+ * packed MSB first (the LDPC decoder's output), corrected in place.
+ * Returns the number of corrected bits, 0 for a codeword, -1 if more
+ * errors than the code corrects were found (cw unchanged), -2 for an
+ * unknown rate.  oracle_bch_nbch: nbch, -1 unknown. */
+int oracle_bch_nbch(int short_frame, const char* rate);
+int oracle_bch_decode(int short_frame, const char* rate, uint8_t* cw);
+
 #endif

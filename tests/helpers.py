@@ -69,6 +69,8 @@ def oracle():
         L.oracle_demap_ring.argtypes = [C.c_int]
         L.oracle_demap_points.argtypes = [C.c_int, vp, vp]
         L.oracle_demap_frame.argtypes = [C.c_int, vp, C.c_float, vp]
+        L.oracle_bch_nbch.argtypes = [C.c_int, C.c_char_p]
+        L.oracle_bch_decode.argtypes = [C.c_int, C.c_char_p, vp]
         _oracle = L
     return _oracle
 
@@ -348,4 +350,30 @@ def codewords(short_frame, rate, ref):
     for f in range(ref.shape[0]):
         info = np.ascontiguousarray(np.unpackbits(ref[f])[:fec.kldpc], np.uint8)
         assert L.sim_ldpc_encode(short_frame, rate.encode(), info.ctypes.data, out[f].ctypes.data) == 0
+    return out
+
+
+def oracle_bch(short_frame, rate, cw):
+    """Oracle BCH decode of one packed codeword (copy): (result, bytes)."""
+    cw = np.array(cw, np.uint8, copy=True)
+    r = oracle().oracle_bch_decode(short_frame, rate.encode(), cw.ctypes.data)
+    return r, cw
+
+
+def ref_bch(R, fec, rate, cw):
+    """drx_bch_decode of a reference build on one packed codeword (copy)."""
+    b = R.drx_bch_new(C.byref(fec.as_struct()))
+    assert b, rate
+    try:
+        cw = np.array(cw, np.uint8, copy=True)
+        return R.drx_bch_decode(b, cw.ctypes.data), cw
+    finally:
+        R.drx_bch_free(b)
+
+
+def flip(cw, bits):
+    """Packed codeword with the given bit indexes (MSB first) flipped."""
+    out = np.array(cw, np.uint8, copy=True)
+    for bit in bits:
+        out[bit // 8] ^= 0x80 >> (bit % 8)
     return out
