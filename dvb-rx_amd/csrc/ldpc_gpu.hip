@@ -49,7 +49,11 @@
 #include "ldpc_node.h"
 #include "ldpc_pk.h"
 
-#define NT 384 /* threads per workgroup: 360 check nodes of a layer */
+#define NT 384 /* threads per workgroup of the one-frame kernel: 360 check nodes of a layer */
+/* the two-slot kernel: exactly one thread per check node, so no thread of
+ * a layer is idle and the node code needs no lane test (the sixth wave has
+ * 40 lanes) */
+#define PK_NT 360
 #define NCLASS 2 /* degree classes: one kernel instance each (registers grow with the degree) */
 #define NLAUNCH 4 /* launches of one decode: degree class x frame size */
 static const int CLASS_DI[NCLASS] = { 12, 28 }; /* largest layer degree of each class */
@@ -570,7 +574,7 @@ __device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, 
  * workgroup moves to another segment with frames left, or exits.
  */
 template <int CLS, bool NMS>
-__global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
+__global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
 {
     extern __shared__ __align__(16) uint16_t sm16[];
     __shared__ unsigned s_vote[3];
@@ -647,13 +651,13 @@ __global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
             const DFrame fr = a.frames[f];
             const uint32_t* src = (const uint32_t*)(a.llr + fr.llr_off);
             uint8_t* sb = (uint8_t*)sm16 + s;
-            for (int v4 = tid; v4 < K / 4; v4 += NT) {
+            for (int v4 = tid; v4 < K / 4; v4 += PK_NT) {
                 const uint32_t w = src[v4];
                 for (int b = 0; b < 4; b++)
                     sb[2 * (4 * v4 + b)] = (uint8_t)(w >> (8 * b));
             }
             const uint8_t* par = (const uint8_t*)(a.llr + fr.llr_off + K);
-            for (int t = tid; t < R; t += NT) {
+            for (int t = tid; t < R; t += PK_NT) {
                 const int jj = t / q, ii = t - jj * q;
                 sb[2 * (K + 360 * ii + jj)] = par[t];
             }
@@ -698,7 +702,7 @@ __global__ void __launch_bounds__(NT) ldpc_pk_kernel(PkArgs a)
                     const int nbytes = (fr.nbits + 7) / 8;
                     uint8_t* o = a.out + fr.out_off;
                     const uint8_t* sb = (const uint8_t*)sm16 + s;
-                    for (int b4 = 4 * tid; b4 < nbytes; b4 += 4 * NT) {
+                    for (int b4 = 4 * tid; b4 < nbytes; b4 += 4 * PK_NT) {
                         uint32_t v = 0;
                         for (int k = 0; k < 32; k++) {
                             const int bit = 8 * b4 + k;
@@ -1302,7 +1306,7 @@ template <int CLS, bool NMS>
 static int pk_grid(ldpc_gpu_ctx* c, size_t lds, int nframes, int* grid)
 {
     int per_cu = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_pk_kernel<CLS, NMS>, NT, lds));
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_pk_kernel<CLS, NMS>, PK_NT, lds));
     if (per_cu < 1)
         return set_err(c, LDPC_GPU_EDEVICE, "two-slot kernel does not fit (LDS %zu)", lds);
     *grid = std::min((nframes + 1) / 2, per_cu * c->ncu);
@@ -1404,14 +1408,14 @@ static int pk_enqueue(ldpc_gpu_ctx* c, const PkLayout& L, PkArgs a, const int2* 
             HIPCHK(c, hipEventRecord(tev[2 * k], st));
         if (nms) {
             if (k < 2)
-                hipLaunchKernelGGL((ldpc_pk_kernel<0, true>), dim3(L.grid[k]), dim3(NT), lds, st, a);
+                hipLaunchKernelGGL((ldpc_pk_kernel<0, true>), dim3(L.grid[k]), dim3(PK_NT), lds, st, a);
             else
-                hipLaunchKernelGGL((ldpc_pk_kernel<1, true>), dim3(L.grid[k]), dim3(NT), lds, st, a);
+                hipLaunchKernelGGL((ldpc_pk_kernel<1, true>), dim3(L.grid[k]), dim3(PK_NT), lds, st, a);
         } else {
             if (k < 2)
-                hipLaunchKernelGGL((ldpc_pk_kernel<0, false>), dim3(L.grid[k]), dim3(NT), lds, st, a);
+                hipLaunchKernelGGL((ldpc_pk_kernel<0, false>), dim3(L.grid[k]), dim3(PK_NT), lds, st, a);
             else
-                hipLaunchKernelGGL((ldpc_pk_kernel<1, false>), dim3(L.grid[k]), dim3(NT), lds, st, a);
+                hipLaunchKernelGGL((ldpc_pk_kernel<1, false>), dim3(L.grid[k]), dim3(PK_NT), lds, st, a);
         }
         HIPCHK(c, hipGetLastError());
         if (tev)
