@@ -397,8 +397,9 @@ def main():
                     help="skip the side measurements (64-frame rate 1/2 batch, learned vs plain)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
-    ap.add_argument("--only", choices=["demapper", "bch"], default=None,
-                    help="only run that side record (the demapper's or the BCH decoder's throughput) and print it")
+    ap.add_argument("--only", choices=["demapper", "bch", "learned"], default=None,
+                    help="only run that side record (the demapper's or the BCH decoder's throughput, learned vs "
+                         "plain decoding) and print it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: frames are independent, so N GPUs would each decode their "
@@ -409,7 +410,7 @@ def main():
         return
     if args.only:
         dec = ldpcgpu.Decoder(0)
-        rec = demap_record(dec) if args.only == "demapper" else bch_record(dec)
+        rec = {"demapper": demap_record, "bch": bch_record, "learned": config4}[args.only](dec)
         dec.close()
         print(json.dumps({args.only: rec}), flush=True)
         return
