@@ -20,9 +20,10 @@
  *   - QPSK has no interleaver: LLR = rint(clamp(k 4 sqrt(1/2) x)) per axis.
  *
  * Work layout: one thread per symbol, a block of 256 symbols of one frame;
- * the MODCOD's point tables (3 KB) are staged in LDS per block.  The
- * minima are exact, so only the metric's arithmetic (written with explicit
- * fused multiply-adds) has to follow the reference.
+ * the MODCOD's point tables (3 KB) are staged in LDS per block; the full
+ * search goes by blocks of 16 points (minima_full).  The minima are exact,
+ * so only the metric's arithmetic (written with explicit fused
+ * multiply-adds) has to follow the reference.
  */
 #include <hip/hip_runtime.h>
 
@@ -141,7 +142,11 @@ __device__ __forceinline__ int8_t quantise(float v)
     return (int8_t)__builtin_rintf(v);
 }
 
-/* per-bit minima of the full search over NB-bit labels */
+/* per-bit minima of the full search over NB-bit labels.  From 16 points
+ * on, blocks of 16 consecutive labels: the 4 low label bits are resolved
+ * inside a block by a fixed tree of minima (pairs, quads, halves), the
+ * high bits by the block's minimum; minima are exact, so the grouping does
+ * not change them. */
 template <int NB>
 __device__ __forceinline__ void minima_full(const float* ar, const float* ai, const float* e, float x, float y,
                                             float* m0, float* m1)
@@ -149,15 +154,62 @@ __device__ __forceinline__ void minima_full(const float* ar, const float* ai, co
 #pragma unroll
     for (int b = 0; b < NB; b++)
         m0[b] = m1[b] = INFINITY;
-#pragma unroll 4
-    for (int p = 0; p < (1 << NB); p++) {
-        const float t = fmaf(x, ar[p], fmaf(y, ai[p], e[p]));
+    if constexpr (NB < 4) {
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
-            if ((p >> (NB - 1 - b)) & 1)
-                m1[b] = fminf(m1[b], t);
-            else
-                m0[b] = fminf(m0[b], t);
+        for (int p = 0; p < (1 << NB); p++) {
+            const float t = fmaf(x, ar[p], fmaf(y, ai[p], e[p]));
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                if ((p >> (NB - 1 - b)) & 1)
+                    m1[b] = fminf(m1[b], t);
+                else
+                    m0[b] = fminf(m0[b], t);
+            }
+        }
+    } else {
+#pragma unroll 1
+        for (int g = 0; g < (1 << (NB - 4)); g++) {
+            float v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int p = 16 * g + u;
+                v[u] = fmaf(x, ar[p], fmaf(y, ai[p], e[p]));
+            }
+            /* label bit NB-1-L is bit L of the index within the block */
+            float pr[8], qd[4];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                pr[k] = fminf(v[2 * k], v[2 * k + 1]);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                qd[k] = fminf(pr[2 * k], pr[2 * k + 1]);
+            float l0 = v[0], h0 = v[1];
+#pragma unroll
+            for (int k = 1; k < 8; k++) {
+                l0 = fminf(l0, v[2 * k]);
+                h0 = fminf(h0, v[2 * k + 1]);
+            }
+            const float l1 = fminf(fminf(pr[0], pr[2]), fminf(pr[4], pr[6]));
+            const float h1 = fminf(fminf(pr[1], pr[3]), fminf(pr[5], pr[7]));
+            const float l2 = fminf(qd[0], qd[2]), h2 = fminf(qd[1], qd[3]);
+            const float l3 = fminf(qd[0], qd[1]), h3 = fminf(qd[2], qd[3]);
+            m0[NB - 1] = fminf(m0[NB - 1], l0);
+            m1[NB - 1] = fminf(m1[NB - 1], h0);
+            m0[NB - 2] = fminf(m0[NB - 2], l1);
+            m1[NB - 2] = fminf(m1[NB - 2], h1);
+            m0[NB - 3] = fminf(m0[NB - 3], l2);
+            m1[NB - 3] = fminf(m1[NB - 3], h2);
+            m0[NB - 4] = fminf(m0[NB - 4], l3);
+            m1[NB - 4] = fminf(m1[NB - 4], h3);
+            const float gmin = fminf(l3, h3);
+            /* high label bits NB-1-L, L >= 4: bit L - 4 of the block index */
+#pragma unroll
+            for (int L = 4; L < NB; L++) {
+                if ((g >> (L - 4)) & 1)
+                    m1[NB - 1 - L] = fminf(m1[NB - 1 - L], gmin);
+                else
+                    m0[NB - 1 - L] = fminf(m0[NB - 1 - L], gmin);
+            }
         }
     }
 }
