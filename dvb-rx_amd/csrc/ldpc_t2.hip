@@ -1,0 +1,507 @@
+/*
+ * DVB-T2 FEC blocks on the device: the front end of the reference's FEC
+ * thread (libdvbrx/src/dvbt2.c decode_job, :2188-2400, with the maps of
+ * setup_plp_fec, :1953-2082) for a whole PLP at once, then the LDPC and BCH
+ * decoders of this library.
+ *
+ * t2_demap_kernel: one workgroup per FEC block.
+ *   1. Each block's cyclic shift P of the cell deinterleaver (the
+ *      bit-reversed counter of its TI-block), by one thread.
+ *   2. Cell deinterleaving, derotation and per-axis noise weights of every
+ *      cell, and the MER sums.  The sums are float and must come out as
+ *      the reference's: it accumulates cell i in lane i % 8 of an 8-wide
+ *      vector, then adds the 8 lanes in order.  The workgroup computes the
+ *      terms of 2048 cells into LDS, then 8 threads (16 with the signal
+ *      sum) add their lane's terms in cell order.
+ *   3. The noise estimate n0 = err / ncells + 1e-6 sets the soft-value
+ *      scale k = 8 / n0; every cell is recomputed (its loads hit L2) and
+ *      its m soft values go to LDS by (cell, plane).
+ *   4. The bit deinterleaver as a gather: codeword bit n takes LDS entry
+ *      src[n] (a per-PLP-configuration table of uint16), written to HBM in
+ *      coalesced 32-bit words.
+ * Floating point follows the reference's x86-64 FMA build exactly: the
+ * fused products are fmaf() calls and everything else is kept unfused
+ * (contraction off).
+ *
+ * HBM per block: 12 bytes a cell in (twice; the second pass mostly from
+ * L2), 4 bytes a cell of the permutation, 3 bytes a codeword bit (table and
+ * output).  The kernel is latency-bound on the ordered sums, not on HBM.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "ldpc_codes.h"
+#include "ldpc_gpu.h"
+#include "ldpc_internal.h"
+
+namespace {
+
+constexpr int T2_MAX_DEV = 16;
+constexpr int T2_BLOCK = 256;
+constexpr int T2_CHUNK = 2048; /* cells per pass of the ordered sums */
+constexpr int T2_LDS = 64800;  /* one block's soft values (the sums' terms alias it) */
+static_assert(2 * T2_CHUNK * 4 <= T2_LDS, "sum terms fit the soft-value area");
+
+/* Column twist (EN 302 755 table 8; libdvbrx/src/dvbt2.c:188-193),
+ * 16/64/256-QAM, normal then short frames. */
+const int TW_N[3][16] = { { 0, 0, 2, 4, 4, 5, 7, 7 },
+                          { 0, 0, 2, 2, 3, 4, 4, 5, 5, 7, 8, 9 },
+                          { 0, 2, 2, 2, 2, 3, 7, 15, 16, 20, 22, 22, 27, 27, 28, 32 } };
+const int TW_S[3][16] = { { 0, 0, 0, 1, 7, 20, 20, 21 },
+                          { 0, 0, 0, 2, 2, 2, 3, 3, 3, 6, 7, 7 },
+                          { 0, 0, 0, 1, 7, 20, 20, 21 } };
+/* Demultiplexer: bit interleaver column c carries word bit mux[c]
+ * (EN 302 755 tables 13a/13b; libdvbrx/src/dvbt2.c:194-200). */
+const int MUX16[8] = { 7, 1, 4, 2, 5, 3, 6, 0 };
+const int MUX64[12] = { 11, 7, 3, 10, 6, 2, 9, 5, 1, 8, 4, 0 };
+const int MUX256[16] = { 15, 1, 13, 3, 8, 11, 9, 5, 10, 6, 4, 7, 12, 2, 14, 0 };
+const int MUX16_35[8] = { 0, 5, 1, 2, 4, 7, 3, 6 };
+const int MUX64_35[12] = { 2, 7, 6, 9, 0, 3, 1, 8, 4, 11, 5, 10 };
+const int MUX256_35[16] = { 2, 11, 3, 4, 0, 9, 1, 8, 10, 13, 7, 14, 6, 15, 5, 12 };
+const int MUX256_23[16] = { 7, 2, 9, 0, 4, 6, 13, 3, 14, 10, 15, 5, 8, 12, 11, 1 };
+const int MUX256S[8] = { 7, 3, 1, 5, 2, 6, 4, 0 };
+/* constellation rotation in degrees (EN 302 755 table 9) */
+const double ROT_DEG[4] = { 29.0, 16.8, 8.6, 3.576334375 };
+const char* const RATES[6] = { "1_2", "3_5", "2_3", "3_4", "4_5", "5_6" };
+
+int nbits_for(int ncells)
+{
+    int nd = 0;
+    while ((1 << nd) < ncells)
+        nd++;
+    return nd;
+}
+
+/* L0, the cell interleaver's basic permutation (EN 302 755 clause 6.5;
+ * dvbt2.c:2052-2081): an nd-bit PRBS with a toggling top bit, values at or
+ * above ncells skipped. */
+bool build_cellperm(int ncells, std::vector<int32_t>& perm)
+{
+    /* feedback taps of R' for nd = 11..15 */
+    static const int taps[5][6] = { { 0, 3 }, { 0, 2 }, { 0, 1, 4, 6 }, { 0, 1, 4, 5, 9, 11 }, { 0, 1, 2, 12 } };
+    static const int ntaps[5] = { 2, 2, 4, 6, 4 };
+    const int nd = nbits_for(ncells);
+    if (nd < 11 || nd > 15)
+        return false;
+    perm.assign(ncells, 0);
+    int r = 0, q = 0;
+    for (int i = 0; i < (1 << nd) && q < ncells; i++) {
+        if (i >= 3) {
+            int fb = 0;
+            for (int t = 0; t < ntaps[nd - 11]; t++)
+                fb ^= (r >> taps[nd - 11][t]) & 1;
+            r = (r >> 1) | (fb << (nd - 2));
+        } else {
+            r = i == 2;
+        }
+        const int v = r | ((i & 1) << (nd - 1));
+        if (v < ncells)
+            perm[q++] = v;
+    }
+    return q == ncells;
+}
+
+/* Codeword bit n -> (cell, plane) as cell * m + plane, plane 2j = bit j of
+ * I, 2j + 1 = bit j of Q: parity interleaving, column twist and the
+ * demultiplexer undone (dvbt2.c:1969-2050, :2364-2385).  QPSK has no bit
+ * interleaver (bit n is plane n % 2 of cell n / 2). */
+bool build_bitsrc(int cod, int mod, bool sf, int N, int K, std::vector<uint16_t>& src)
+{
+    const int m = 2 * (mod + 1);
+    src.assign(N, 0);
+    if (m == 2) {
+        for (int n = 0; n < N; n++)
+            src[n] = (uint16_t)n;
+        return true;
+    }
+    const int Q = (N - K) / 360;
+    const int nc = (m == 8 && sf) ? 8 : 2 * m, nr = N / nc;
+    const int* tw = sf ? TW_S[mod - 1] : TW_N[mod - 1];
+    const int* mux = m == 4   ? (cod == 1 && !sf ? MUX16_35 : MUX16)
+                     : m == 6 ? (cod == 1 && !sf ? MUX64_35 : MUX64)
+                              : (sf ? MUX256S : cod == 1 ? MUX256_35 : cod == 2 ? MUX256_23 : MUX256);
+    const bool pair = !(m == 8 && sf); /* two cells per demultiplexer word */
+    for (int n = 0; n < N; n++) {
+        const int x = n < K ? n : K + 360 * ((n - K) % Q) + (n - K) / Q;
+        const int c = x / nr, row = (tw[c] + x % nr) % nr;
+        const int cell = pair ? 2 * row + mux[c] / m : row;
+        src[n] = (uint16_t)(cell * m + mux[c] % m);
+    }
+    return true;
+}
+
+struct T2Maps {
+    int32_t* d_perm[2][4] = {};     /* by (fec_type, mod): ncells = N / m */
+    uint16_t* d_src[2][6][4] = {};  /* by (fec_type, cod, mod) */
+};
+T2Maps g_maps[T2_MAX_DEV];
+std::mutex g_maps_lock;
+
+/* The PLP configuration's device tables, built at first use per device. */
+int get_maps(int dev, int cod, int mod, int fec_type, int N, int K, const int32_t** perm, const uint16_t** src)
+{
+    std::lock_guard<std::mutex> lk(g_maps_lock);
+    T2Maps& t = g_maps[dev];
+    int32_t*& dp = t.d_perm[fec_type][mod];
+    uint16_t*& ds = t.d_src[fec_type][cod][mod];
+    if (!dp) {
+        std::vector<int32_t> h;
+        if (!build_cellperm(N / (2 * (mod + 1)), h))
+            return LDPC_GPU_EINVAL;
+        int32_t* p = nullptr;
+        if (hipMalloc((void**)&p, sizeof(int32_t) * h.size()) != hipSuccess)
+            return LDPC_GPU_ENOMEM;
+        if (hipMemcpy(p, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(p);
+            return LDPC_GPU_EDEVICE;
+        }
+        dp = p;
+    }
+    if (!ds) {
+        std::vector<uint16_t> h;
+        if (!build_bitsrc(cod, mod, fec_type == 0, N, K, h))
+            return LDPC_GPU_EINVAL;
+        uint16_t* p = nullptr;
+        if (hipMalloc((void**)&p, sizeof(uint16_t) * h.size()) != hipSuccess)
+            return LDPC_GPU_ENOMEM;
+        if (hipMemcpy(p, h.data(), sizeof(uint16_t) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(p);
+            return LDPC_GPU_EDEVICE;
+        }
+        ds = p;
+    }
+    *perm = dp;
+    *src = ds;
+    return LDPC_GPU_OK;
+}
+
+struct T2Args {
+    const float2* cells;
+    const float* csi;
+    const int32_t* perm;
+    const uint16_t* src;
+    int8_t* llr;
+    float* err;
+    float* sig;
+    int ncells, N, nblocks, nti, nd, rot;
+    float ct, sn, norm, ct2, sn2, s2, amax;
+};
+
+/* Shift P of block b (dvbt2.c:2230-2254): blocks are dealt to the nti
+ * TI-blocks in order, the last nblocks % nti taking one more; the r-th
+ * block of a TI-block takes the r-th nd-bit bit-reversed counter value
+ * below ncells. */
+__device__ int block_shift(int b, int nblocks, int nti, int ncells, int nd)
+{
+    if (!nti)
+        return 0;
+    int cum = 0;
+    for (int s = 0; s < nti; s++) {
+        const int ns = nblocks / nti + (s >= nti - nblocks % nti);
+        if (b < cum + ns)
+            break;
+        cum += ns;
+    }
+    const int r = b - cum;
+    int seen = 0;
+    for (unsigned k = 0; k < (1u << nd); k++) {
+        const int v = (int)(__brev(k) >> (32 - nd));
+        if (v < ncells) {
+            if (seen == r)
+                return v;
+            seen++;
+        }
+    }
+    return 0; /* not reached: the host bounds blocks per TI-block below ncells */
+}
+
+/* Derotated point of cell i and its per-axis weights (dvbt2.c:2257-2292):
+ * I from deinterleaved cell i, Q from cell i + 1 (cyclically) when the
+ * constellation is rotated. */
+__device__ __forceinline__ void t2_point(const T2Args& a, const float2* in, const float* ic, int P, int i, float& wr,
+                                         float& wi, float& kr, float& ki)
+{
+#pragma clang fp contract(off)
+    int s = a.perm[i] + P;
+    s -= s >= a.ncells ? a.ncells : 0;
+    const float2 x = in[s];
+    const float c1 = ic[s] + 1e-12f;
+    float v = x.y, c2 = c1;
+    if (a.rot) {
+        const int i1 = i + 1 == a.ncells ? 0 : i + 1;
+        int s1 = a.perm[i1] + P;
+        s1 -= s1 >= a.ncells ? a.ncells : 0;
+        v = in[s1].y;
+        c2 = ic[s1] + 1e-12f;
+    }
+    const float u = x.x;
+    wr = fmaf(a.ct, u, a.sn * v) * a.norm;
+    wi = fmaf(a.ct, v, -(a.sn * u)) * a.norm;
+    kr = ki = c1;
+    if (a.rot) {
+        const float c12 = c1 * c2;
+        kr = c12 / fmaf(a.sn2, c1, a.ct2 * c2);
+        ki = c12 / fmaf(a.ct2, c1, a.sn2 * c2);
+    }
+}
+
+__device__ __forceinline__ int8_t t2_llr(float v)
+{
+    v = fminf(fmaxf(v, -127.0f), 127.0f);
+    return (int8_t)(int)rintf(v);
+}
+
+template <int M>
+__global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
+{
+#pragma clang fp contract(off)
+    constexpr int NB = M / 2;
+    __shared__ __attribute__((aligned(16))) int8_t soft[T2_LDS];
+    __shared__ float lane_sum[16];
+    __shared__ int s_P;
+    __shared__ float s_k;
+    float* tE = (float*)soft;
+    float* tS = tE + T2_CHUNK;
+    const int tid = threadIdx.x, b = blockIdx.x;
+    if (tid == 0)
+        s_P = block_shift(b, a.nblocks, a.nti, a.ncells, a.nd);
+    __syncthreads();
+    const int P = s_P;
+    const float2* in = a.cells + (size_t)b * a.ncells;
+    const float* ic = a.csi + (size_t)b * a.ncells;
+
+    /* MER sums in the reference's order: thread l < 8 holds err lane l,
+     * thread 8 + l sig lane l */
+    float acc = 0.0f;
+    for (int base = 0; base < a.ncells; base += T2_CHUNK) {
+        for (int j = tid; j < T2_CHUNK; j += T2_BLOCK) {
+            const int i = base + j;
+            float te = 0.0f, ts = 0.0f;
+            if (i < a.ncells) {
+                float wr, wi, kr, ki;
+                t2_point(a, in, ic, P, i, wr, wi, kr, ki);
+                const float awr = fabsf(wr), awi = fabsf(wi);
+                const float ar = 2.0f * truncf(fminf(awr * 0.5f, a.amax)) + 1.0f;
+                const float ai = 2.0f * truncf(fminf(awi * 0.5f, a.amax)) + 1.0f;
+                const float er = awr - ar, ei = awi - ai;
+                te = fmaf(er * er, kr, ei * ei * ki);
+                ts = a.s2 * (kr + ki);
+            }
+            tE[j] = te;
+            tS[j] = ts;
+        }
+        __syncthreads();
+        if (tid < 16) {
+            const float* t = (tid < 8 ? tE : tS) + (tid & 7);
+            const int cnt = min(T2_CHUNK, a.ncells - base);
+#pragma unroll 8
+            for (int q = 0; 8 * q < cnt; q++)
+                acc = acc + t[8 * q];
+        }
+        __syncthreads();
+    }
+    if (tid < 16)
+        lane_sum[tid] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        float err = 0.0f, sig = 0.0f;
+        for (int l = 0; l < 8; l++) {
+            err = err + lane_sum[l];
+            sig = sig + lane_sum[8 + l];
+        }
+        a.err[b] = err;
+        a.sig[b] = sig;
+        const float n0u = err / (float)a.ncells + 1e-6f;
+        s_k = 8.0f / n0u;
+    }
+    __syncthreads();
+    const float k = s_k;
+
+    /* soft values by (cell, plane) (dvbt2.c:2333-2359) */
+    for (int i = tid; i < a.ncells; i += T2_BLOCK) {
+        float ur, ui, kr, ki;
+        t2_point(a, in, ic, P, i, ur, ui, kr, ki);
+        int8_t v[M];
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            if (j) {
+                const float t = (float)(1 << (NB - j));
+                ur = fabsf(ur) - t;
+                ui = fabsf(ui) - t;
+            }
+            v[2 * j] = t2_llr((ur * kr) * k);
+            v[2 * j + 1] = t2_llr((ui * ki) * k);
+        }
+#pragma unroll
+        for (int p = 0; p < M; p++)
+            soft[i * M + p] = v[p];
+    }
+    __syncthreads();
+
+    /* bit deinterleaving as a gather, 4 codeword bits a thread and step */
+    uint32_t* out = (uint32_t*)(a.llr + (size_t)b * a.N);
+    const uint2* src = (const uint2*)a.src;
+    for (int w = tid; w < a.N / 4; w += T2_BLOCK) {
+        const uint2 s = src[w];
+        const uint32_t b0 = (uint8_t)soft[s.x & 0xFFFF], b1 = (uint8_t)soft[s.x >> 16];
+        const uint32_t b2 = (uint8_t)soft[s.y & 0xFFFF], b3 = (uint8_t)soft[s.y >> 16];
+        out[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+    }
+}
+
+struct PlpGeom {
+    int N, K, kbch, ncells, m, code;
+    char rate[16];
+};
+
+int plp_geom(const ldpc_gpu_t2_plp* q, PlpGeom& g)
+{
+    if (!q || q->cod < 0 || q->cod > 5 || q->mod < 0 || q->mod > 3 || (q->fec_type != 0 && q->fec_type != 1))
+        return LDPC_GPU_EINVAL;
+    const bool sf = q->fec_type == 0;
+    /* dvbt2.c:1966: T2's own 2/3 normal and 3/5 short codes */
+    const char* name = !sf && q->cod == 2 ? "2_3_t2" : sf && q->cod == 1 ? "3_5_t2" : RATES[q->cod];
+    std::strncpy(g.rate, name, sizeof g.rate - 1);
+    g.rate[sizeof g.rate - 1] = 0;
+    g.code = lg_code_lookup(sf, name);
+    int nbch = 0, t = 0;
+    if (g.code < 0 || lg_fec_lookup(sf, RATES[q->cod], &g.kbch, &nbch, &t))
+        return LDPC_GPU_EINVAL;
+    int qq, links, steps;
+    if (ldpc_gpu_code_params(g.code, &g.N, &g.K, &qq, &links, &steps))
+        return LDPC_GPU_EINVAL;
+    g.m = 2 * (q->mod + 1);
+    g.ncells = g.N / g.m;
+    return LDPC_GPU_OK;
+}
+
+} // namespace
+
+extern "C" int ldpc_gpu_t2_geometry(const ldpc_gpu_t2_plp* plp, int* nldpc, int* kldpc, int* kbch, int* ncells)
+{
+    PlpGeom g;
+    if (plp_geom(plp, g))
+        return LDPC_GPU_EINVAL;
+    if (nldpc)
+        *nldpc = g.N;
+    if (kldpc)
+        *kldpc = g.K;
+    if (kbch)
+        *kbch = g.kbch;
+    if (ncells)
+        *ncells = g.ncells;
+    return LDPC_GPU_OK;
+}
+
+extern "C" int ldpc_gpu_t2_demap(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, int nblocks, const float* d_cells,
+                                 const float* d_csi, int8_t* d_llr, float* d_err, float* d_sig)
+{
+    if (!ctx)
+        return LDPC_GPU_EINVAL;
+    PlpGeom g;
+    if (plp_geom(plp, g))
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_demap: unsupported PLP (cod 0..5, mod 0..3, fec_type 0/1)");
+    if (nblocks < 0 || (nblocks > 0 && (!d_cells || !d_csi || !d_llr || !d_err || !d_sig)) ||
+        plp->til_length < 0 || plp->til_length > 255)
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_demap: bad arguments");
+    if (nblocks == 0)
+        return LDPC_GPU_OK;
+    if (nblocks > 65535)
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_demap: at most 65535 blocks per call");
+    if (((uintptr_t)d_cells & 7) || ((uintptr_t)d_llr & 3))
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_demap: d_cells must be 8-byte and d_llr 4-byte aligned");
+    const int nti = plp->til_length && plp->til_type == 0 ? plp->til_length : 0;
+    /* the counter of a TI-block runs up to twice its block count */
+    if (nti && (nblocks / nti + 1) * 2 > (1 << nbits_for(g.ncells)))
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_demap: more blocks per TI-block than the cell interleaver has shifts");
+    const int dev = lg_ctx_device(ctx);
+    if (dev < 0 || dev >= T2_MAX_DEV)
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_demap: device index");
+    if (hipSetDevice(dev) != hipSuccess)
+        return lg_ctx_fail(ctx, LDPC_GPU_EDEVICE, "t2_demap: hipSetDevice");
+    const int32_t* perm = nullptr;
+    const uint16_t* src = nullptr;
+    int rc = get_maps(dev, plp->cod, plp->mod, plp->fec_type, g.N, g.K, &perm, &src);
+    if (rc)
+        return lg_ctx_fail(ctx, rc, "t2_demap: deinterleaver tables");
+    T2Args a;
+    a.cells = (const float2*)d_cells;
+    a.csi = d_csi;
+    a.perm = perm;
+    a.src = src;
+    a.llr = d_llr;
+    a.err = d_err;
+    a.sig = d_sig;
+    a.ncells = g.ncells;
+    a.N = g.N;
+    a.nblocks = nblocks;
+    a.nti = nti;
+    a.nd = nbits_for(g.ncells);
+    a.rot = plp->rotation != 0;
+    /* constants as dvbt2.c:2215-2230 forms them, in float */
+    const double th = a.rot ? ROT_DEG[plp->mod] * M_PI / 180 : 0;
+    a.ct = (float)std::cos(th);
+    a.sn = (float)std::sin(th);
+    a.norm = g.m == 2 ? (float)M_SQRT2 : g.m == 4 ? std::sqrt(10.0f) : g.m == 6 ? std::sqrt(42.0f) : std::sqrt(170.0f);
+    a.ct2 = a.ct * a.ct;
+    a.sn2 = a.sn * a.sn;
+    a.s2 = a.norm * a.norm * 0.5f;
+    a.amax = (float)((1 << (g.m / 2 - 1)) - 1);
+    const hipStream_t st = (hipStream_t)ldpc_gpu_stream(ctx);
+    switch (g.m) {
+    case 2:
+        hipLaunchKernelGGL(t2_demap_kernel<2>, dim3(nblocks), dim3(T2_BLOCK), 0, st, a);
+        break;
+    case 4:
+        hipLaunchKernelGGL(t2_demap_kernel<4>, dim3(nblocks), dim3(T2_BLOCK), 0, st, a);
+        break;
+    case 6:
+        hipLaunchKernelGGL(t2_demap_kernel<6>, dim3(nblocks), dim3(T2_BLOCK), 0, st, a);
+        break;
+    default:
+        hipLaunchKernelGGL(t2_demap_kernel<8>, dim3(nblocks), dim3(T2_BLOCK), 0, st, a);
+        break;
+    }
+    if (hipGetLastError() != hipSuccess)
+        return lg_ctx_fail(ctx, LDPC_GPU_EDEVICE, "t2_demap: launch failed");
+    return LDPC_GPU_OK;
+}
+
+extern "C" int ldpc_gpu_t2_decode(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, int nblocks, const float* d_cells,
+                                  const float* d_csi, int max_iter, int lanes, int8_t* d_llr, uint8_t* d_cw,
+                                  int32_t* d_iters, uint8_t* d_synd, int32_t* d_bch, float* d_err, float* d_sig)
+{
+    if (!ctx)
+        return LDPC_GPU_EINVAL;
+    PlpGeom g;
+    if (plp_geom(plp, g))
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_decode: unsupported PLP (cod 0..5, mod 0..3, fec_type 0/1)");
+    if (max_iter < 0 || (nblocks > 0 && (!d_cw || !d_iters || !d_synd || !d_bch)))
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_decode: bad arguments");
+    if (((uintptr_t)d_llr & 7))
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_decode: d_llr must be 8-byte aligned");
+    int rc = ldpc_gpu_t2_demap(ctx, plp, nblocks, d_cells, d_csi, d_llr, d_err, d_sig);
+    if (rc || nblocks == 0)
+        return rc;
+    const size_t stride = (size_t)g.K / 8;
+    std::vector<ldpc_gpu_frame> fr(nblocks);
+    for (int b = 0; b < nblocks; b++) {
+        fr[b].code = g.code;
+        fr[b].max_iter = max_iter;
+        fr[b].nbits = g.K;
+        /* lanes > 0: the blocks of one reference decoder call stop
+         * together (dvbt2.c:2387-2396 decodes lanes blocks per call) */
+        fr[b].group = lanes > 0 ? b / lanes : -1;
+        fr[b].llr_off = (uint64_t)b * g.N;
+        fr[b].out_off = (uint64_t)b * stride;
+    }
+    rc = ldpc_gpu_decode_batch(ctx, fr.data(), nblocks, d_llr, d_cw, d_iters, d_synd);
+    if (rc)
+        return rc;
+    return ldpc_gpu_bch_decode(ctx, plp->fec_type == 0, RATES[plp->cod], nblocks, d_cw, stride, d_bch);
+}

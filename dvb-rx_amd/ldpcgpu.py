@@ -78,6 +78,9 @@ def gpu_lib():
         L.ldpc_gpu_demap_symbols.argtypes = [C.c_int]
         L.ldpc_gpu_demap.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.ldpc_gpu_bch_decode.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, vp, C.c_size_t, vp]
+        L.ldpc_gpu_t2_geometry.argtypes = [vp, ip, ip, ip, ip]
+        L.ldpc_gpu_t2_demap.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
+        L.ldpc_gpu_t2_decode.argtypes = [vp, vp, i32, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp]
         L.ldpc_gpu_decode_batch.argtypes = [vp, vp, i32, vp, vp, vp, vp]
         L.ldpc_gpu_decode_host.argtypes = [vp, vp, i32, vp, sz, vp, sz, vp, vp]
         L.ldpc_gpu_last_kernel_ms.argtypes = [vp]
@@ -462,6 +465,34 @@ def bch_decode(dec, short_frame, rate, nframes, d_cw, stride, d_result):
     """ldpc_gpu_bch_decode on the decoder context's stream (enqueue only)."""
     dec._check(gpu_lib().ldpc_gpu_bch_decode(dec.ctx, short_frame, rate.encode(), nframes, d_cw, stride, d_result),
                "bch_decode")
+
+
+class T2Plp(C.Structure):
+    """ldpc_gpu_t2_plp: a DVB-T2 PLP's FEC parameters as L1-post signals
+    them (cod 0..5, mod 0..3, fec_type 1 normal / 0 short, rotation, time
+    interleaver length and type)."""
+    _fields_ = [(n, C.c_int32) for n in ("cod", "mod", "fec_type", "rotation", "til_length", "til_type")]
+
+
+def t2_geometry(plp):
+    """(nldpc, kldpc, kbch, ncells) of a PLP configuration; ValueError if
+    unsupported."""
+    v = [C.c_int() for _ in range(4)]
+    if gpu_lib().ldpc_gpu_t2_geometry(C.byref(plp), *[C.byref(x) for x in v]):
+        raise ValueError(f"unsupported DVB-T2 PLP {[getattr(plp, f) for f, _ in plp._fields_]}")
+    return tuple(x.value for x in v)
+
+
+def t2_demap(dec, plp, nblocks, d_cells, d_csi, d_llr, d_err, d_sig):
+    """ldpc_gpu_t2_demap on the decoder context's stream (enqueue only)."""
+    dec._check(gpu_lib().ldpc_gpu_t2_demap(dec.ctx, C.byref(plp), nblocks, d_cells, d_csi, d_llr, d_err, d_sig),
+               "t2_demap")
+
+
+def t2_decode(dec, plp, nblocks, d_cells, d_csi, max_iter, lanes, d_llr, d_cw, d_iters, d_synd, d_bch, d_err, d_sig):
+    """ldpc_gpu_t2_decode: front end, LDPC and BCH of nblocks FEC blocks."""
+    dec._check(gpu_lib().ldpc_gpu_t2_decode(dec.ctx, C.byref(plp), nblocks, d_cells, d_csi, max_iter, lanes, d_llr,
+                                            d_cw, d_iters, d_synd, d_bch, d_err, d_sig), "t2_decode")
 
 
 class DrxLdpc:

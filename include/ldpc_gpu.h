@@ -172,6 +172,53 @@ int ldpc_gpu_demap(ldpc_gpu_ctx* ctx, int pls, int nframes, const float* d_sym, 
 int ldpc_gpu_bch_decode(ldpc_gpu_ctx* ctx, int short_frame, const char* rate, int nframes, uint8_t* d_cw,
                         size_t stride, int32_t* d_result);
 
+/*
+ * DVB-T2 FEC blocks of one PLP (the reference's FEC thread, decode_job,
+ * libdvbrx/src/dvbt2.c:2188-2400, and the per-PLP maps of setup_plp_fec,
+ * :1953-2082; there one frame's blocks per call from a pthread, here a
+ * whole batch per launch).  The PLP: cod 0..5 (1/2 3/5 2/3 3/4 4/5 5/6),
+ * mod 0..3 (QPSK, 16-, 64-, 256-QAM), fec_type 1 normal / 0 short,
+ * rotation 0/1, and the time interleaver's length and type (with type 0
+ * the cell deinterleaver's cyclic shift restarts with each TI-block), as
+ * in L1-post (struct plp, dvbt2.c:216-221).
+ */
+typedef struct {
+    int32_t cod, mod, fec_type, rotation, til_length, til_type;
+} ldpc_gpu_t2_plp;
+
+/* nldpc, kldpc, kbch and cells per block; LDPC_GPU_EINVAL if unsupported
+ * (the checks of setup_plp_fec, dvbt2.c:1963). */
+int ldpc_gpu_t2_geometry(const ldpc_gpu_t2_plp* plp, int* nldpc, int* kldpc, int* kbch, int* ncells);
+
+/*
+ * Front end (dvbt2.c:2213-2386): nblocks blocks of ncells time-
+ * deinterleaved cells, d_cells as (re, im) float pairs (8-byte aligned)
+ * and d_csi their channel state -> d_llr nblocks x nldpc int8 soft values
+ * in codeword order (4-byte aligned; positive for a 0 bit), d_err/d_sig
+ * the MER error and signal sums of each block.  Cell deinterleaving,
+ * derotation with per-axis noise weights, the noise estimate, max-log
+ * soft values and bit deinterleaving: identical to the reference's x86-64
+ * FMA build.  Enqueues only.
+ */
+int ldpc_gpu_t2_demap(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, int nblocks, const float* d_cells,
+                      const float* d_csi, int8_t* d_llr, float* d_err, float* d_sig);
+
+/*
+ * The whole job: the front end into d_llr (nblocks x nldpc, 8-byte
+ * aligned), LDPC decoding with max_iter (the reference's FEC thread uses
+ * 30, 15 when frames queue up, dvbt2.c:127-129, :2457) into d_cw (kldpc / 8
+ * bytes a block, packed MSB first; d_iters and d_synd per block as
+ * ldpc_gpu_decode_batch), then BCH in place: d_bch[b] >= 0 is the
+ * reference's job->ok, and the block's BBFRAME is its first kbch / 8
+ * bytes.  lanes > 0 decodes blocks b / lanes together as one call of the
+ * reference's decoder does (32 in its AVX2 build, 64 with AVX-512BW), for
+ * bit-identical output; lanes <= 0 gives every block its own early stop.
+ * Returns after decoding when lanes > 0, else enqueues.
+ */
+int ldpc_gpu_t2_decode(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, int nblocks, const float* d_cells,
+                       const float* d_csi, int max_iter, int lanes, int8_t* d_llr, uint8_t* d_cw, int32_t* d_iters,
+                       uint8_t* d_synd, int32_t* d_bch, float* d_err, float* d_sig);
+
 /* Kernel choice for frames with their own early stop (group -1):
  * LDPC_GPU_KERNEL_AUTO runs two frames of one code per workgroup in packed
  * 16-bit arithmetic; LDPC_GPU_KERNEL_SINGLE one frame per workgroup (the

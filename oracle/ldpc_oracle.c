@@ -947,3 +947,271 @@ int oracle_bch_decode(int short_frame, const char* rate, uint8_t* cw)
     free(b.log);
     return res;
 }
+
+/* ---- DVB-T2 FEC block front end --------------------------------------- */
+
+/* Column twist t_c of the bit interleaver (EN 302 755 table 8), as
+ * dvbt2.c:188-193 lists it: normal frames 16/64/256-QAM, then short. */
+static const int T2_TW_N[3][16] = { { 0, 0, 2, 4, 4, 5, 7, 7 },
+                                    { 0, 0, 2, 2, 3, 4, 4, 5, 5, 7, 8, 9 },
+                                    { 0, 2, 2, 2, 2, 3, 7, 15, 16, 20, 22, 22, 27, 27, 28, 32 } };
+static const int T2_TW_S[3][16] = { { 0, 0, 0, 1, 7, 20, 20, 21 },
+                                    { 0, 0, 0, 2, 2, 2, 3, 3, 3, 6, 7, 7 },
+                                    { 0, 0, 0, 1, 7, 20, 20, 21 } };
+/* Demultiplexer: interleaver column c carries word bit mux[c]
+ * (dvbt2.c:194-200, chosen as dvbt2.c:1940-1944 does). */
+static const int T2_MUX16[8] = { 7, 1, 4, 2, 5, 3, 6, 0 };
+static const int T2_MUX64[12] = { 11, 7, 3, 10, 6, 2, 9, 5, 1, 8, 4, 0 };
+static const int T2_MUX256[16] = { 15, 1, 13, 3, 8, 11, 9, 5, 10, 6, 4, 7, 12, 2, 14, 0 };
+static const int T2_MUX16_35[8] = { 0, 5, 1, 2, 4, 7, 3, 6 };
+static const int T2_MUX64_35[12] = { 2, 7, 6, 9, 0, 3, 1, 8, 4, 11, 5, 10 };
+static const int T2_MUX256_35[16] = { 2, 11, 3, 4, 0, 9, 1, 8, 10, 13, 7, 14, 6, 15, 5, 12 };
+static const int T2_MUX256_23[16] = { 7, 2, 9, 0, 4, 6, 13, 3, 14, 10, 15, 5, 8, 12, 11, 1 };
+static const int T2_MUX256S[8] = { 7, 3, 1, 5, 2, 6, 4, 0 };
+/* rotation angles in degrees by modulation (dvbt2.c:205) */
+static const double T2_ROT_DEG[4] = { 29.0, 16.8, 8.6, 3.576334375 };
+static const char* const T2_RATES[6] = { "1_2", "3_5", "2_3", "3_4", "4_5", "5_6" };
+
+int oracle_t2_geometry(int cod, int mod, int fec_type, int* nldpc, int* ncells, char* code, int code_len)
+{
+    if (cod < 0 || cod > 5 || mod < 0 || mod > 3 || (fec_type != 0 && fec_type != 1))
+        return -1;
+    const int sf = fec_type == 0, N = sf ? 16200 : 64800, m = 2 * (mod + 1);
+    /* dvbt2.c:1966: T2 has its own 2/3 normal and 3/5 short codes */
+    const char* name = !sf && cod == 2 ? "2_3_t2" : sf && cod == 1 ? "3_5_t2" : T2_RATES[cod];
+    if (nldpc)
+        *nldpc = N;
+    if (ncells)
+        *ncells = N / m;
+    if (code && code_len > 0) {
+        strncpy(code, name, (size_t)code_len - 1);
+        code[code_len - 1] = 0;
+    }
+    return 0;
+}
+
+/* dvbt2.c:2052-2081: L0 from the nd-bit PRBS R' (taps by nd), with the
+ * toggling top bit, values >= ncells skipped. */
+int oracle_t2_cellperm(int ncells, int32_t* perm)
+{
+    int nd = 0;
+    while ((1 << nd) < ncells)
+        nd++;
+    static const int taps[16][6] = { [11] = { 0, 3 }, [12] = { 0, 2 }, [13] = { 0, 1, 4, 6 },
+                                     [14] = { 0, 1, 4, 5, 9, 11 }, [15] = { 0, 1, 2, 12 } };
+    static const int ntaps[16] = { [11] = 2, [12] = 2, [13] = 4, [14] = 6, [15] = 4 };
+    if (nd < 11 || nd > 15)
+        return -1;
+    int r = 0, q = 0;
+    for (int i = 0; i < (1 << nd) && q < ncells; i++) {
+        if (i < 2)
+            r = 0;
+        else if (i == 2)
+            r = 1;
+        else {
+            int fb = 0;
+            for (int t = 0; t < ntaps[nd]; t++)
+                fb ^= (r >> taps[nd][t]) & 1;
+            r = (r >> 1) | (fb << (nd - 2));
+        }
+        const int v = r | ((i & 1) << (nd - 1));
+        if (v < ncells)
+            perm[q++] = v;
+    }
+    return q == ncells ? 0 : -1;
+}
+
+/* dvbt2.c:2230-2254: with a type-0 time interleaver, blocks are dealt to
+ * TI-blocks in order (the last nblocks % nti TI-blocks take one more) and
+ * each block of a TI-block takes the next nd-bit bit-reversed counter
+ * value below ncells; otherwise the shift is 0. */
+int oracle_t2_shift(int nblocks, int til_length, int til_type, int ncells, int32_t* P)
+{
+    const int nti = til_length && til_type == 0 ? til_length : 0;
+    int nd = 0;
+    while ((1 << nd) < ncells)
+        nd++;
+    int s = 0, left = 0, k = 0;
+    for (int b = 0; b < nblocks; b++) {
+        P[b] = 0;
+        if (!nti)
+            continue;
+        while (left == 0 && s < nti) {
+            left = nblocks / nti + (s >= nti - nblocks % nti);
+            s++;
+            k = 0;
+        }
+        for (;;) {
+            int v = 0;
+            for (int j = 0; j < nd; j++)
+                v |= ((k >> j) & 1) << (nd - 1 - j);
+            k++;
+            if (v < ncells) {
+                P[b] = v;
+                break;
+            }
+        }
+        left--;
+    }
+    return 0;
+}
+
+static const int* t2_mux(int cod, int mod, int sf)
+{
+    const int m = 2 * (mod + 1);
+    return m == 4   ? (cod == 1 && !sf ? T2_MUX16_35 : T2_MUX16)
+           : m == 6 ? (cod == 1 && !sf ? T2_MUX64_35 : T2_MUX64)
+           : m == 8 ? (sf ? T2_MUX256S : cod == 1 ? T2_MUX256_35 : cod == 2 ? T2_MUX256_23 : T2_MUX256)
+                    : NULL;
+}
+
+/* dvbt2.c:1969-2050 and :2364-2385, as one map.  QPSK: no bit
+ * interleaving, codeword bit n is bit n % 2 of cell n / 2.  Otherwise
+ * codeword bit n sits at position x of the interleaver columns (x = n for
+ * the K information bits; parity bit K + t + Q s at K + 360 t + s); column
+ * c = x / nr, row (t_c + x % nr) mod nr, which is word row of that index
+ * and word bit mux[c]: bit mux[c] % m of the word's cell mux[c] / m (two
+ * cells per word but for 256-QAM short frames). */
+int oracle_t2_bitsrc(int cod, int mod, int fec_type, uint16_t* src)
+{
+    int N, ncells;
+    if (oracle_t2_geometry(cod, mod, fec_type, &N, &ncells, NULL, 0))
+        return -1;
+    const int sf = fec_type == 0, m = 2 * (mod + 1);
+    if (m == 2) {
+        for (int n = 0; n < N; n++)
+            src[n] = (uint16_t)n;
+        return 0;
+    }
+    char code[16];
+    oracle_t2_geometry(cod, mod, fec_type, NULL, NULL, code, sizeof code);
+    const struct lg_code_def* cd = find_code(sf, code);
+    if (!cd)
+        return -1;
+    const int nldpc = cd->n, kldpc = cd->k;
+    const int K = kldpc, Q = (nldpc - K) / 360;
+    const int nc = (m == 8 && sf) ? 8 : 2 * m, nr = N / nc;
+    const int* tw = sf ? T2_TW_S[mod - 1] : T2_TW_N[mod - 1];
+    const int* mux = t2_mux(cod, mod, sf);
+    const int pair = !(m == 8 && sf);
+    for (int n = 0; n < N; n++) {
+        const int x = n < K ? n : K + 360 * ((n - K) % Q) + (n - K) / Q;
+        const int c = x / nr, w = (tw[c] + x % nr) % nr;
+        const int cell = pair ? 2 * w + mux[c] / m : w;
+        src[n] = (uint16_t)(cell * m + mux[c] % m);
+    }
+    return 0;
+}
+
+static int8_t t2_to_llr(float v)
+{
+    /* dvbt2.c:1464-1476: clamp to +-127, round to nearest even */
+    v = v < 127.0f ? v : 127.0f;
+    v = v > -127.0f ? v : -127.0f;
+    return (int8_t)lrintf(v);
+}
+
+/* dvbt2.c:2213-2344, one block at a time; the reference's 8-wide vectors
+ * are restated lane by lane (cell i in lane i % 8), with its FMA
+ * contractions made explicit. */
+int oracle_t2_demap(int cod, int mod, int fec_type, int rotation, int til_length, int til_type, int nblocks,
+                    const float* cells, const float* csi, int8_t* llr, float* err_out, float* sig_out)
+{
+    int N, ncells;
+    if (oracle_t2_geometry(cod, mod, fec_type, &N, &ncells, NULL, 0))
+        return -1;
+    const int m = 2 * (mod + 1), nb = m / 2, rot = rotation != 0;
+    int32_t* perm = malloc(sizeof(int32_t) * ncells);
+    int32_t* P = malloc(sizeof(int32_t) * (nblocks > 0 ? nblocks : 1));
+    uint16_t* src = malloc(sizeof(uint16_t) * N);
+    float* w = malloc(sizeof(float) * 4 * (ncells + 1));
+    int8_t* soft = malloc((size_t)N);
+    int rc = -1;
+    if (!perm || !P || !src || !w || !soft || oracle_t2_cellperm(ncells, perm) ||
+        oracle_t2_bitsrc(cod, mod, fec_type, src))
+        goto out;
+    oracle_t2_shift(nblocks, til_length, til_type, ncells, P);
+    const double th = rot ? T2_ROT_DEG[mod] * M_PI / 180 : 0;
+    const float ct = (float)cos(th), sn = (float)sin(th);
+    const float norm = m == 2 ? (float)M_SQRT2 : m == 4 ? sqrtf(10) : m == 6 ? sqrtf(42) : sqrtf(170);
+    const float ct2 = ct * ct, sn2 = sn * sn, eps = 1e-12f;
+    const float amax = (float)((1 << (nb - 1)) - 1), s2 = norm * norm * 0.5f;
+    float* I = w;
+    float* Qv = w + (ncells + 1);
+    float* C = w + 2 * (ncells + 1);
+    for (int b = 0; b < nblocks; b++) {
+        const float* in = cells + 2 * (size_t)b * ncells;
+        const float* ic = csi + (size_t)b * ncells;
+        /* cell deinterleaving (dvbt2.c:2257-2264); entry ncells repeats
+         * cell 0 (the Q of the last rotated point) */
+        for (int i = 0; i <= ncells; i++) {
+            int s = perm[i % ncells] + P[b];
+            s -= s >= ncells ? ncells : 0;
+            I[i] = in[2 * s];
+            Qv[i] = in[2 * s + 1];
+            C[i] = ic[s];
+        }
+        float verr[8] = { 0 }, vsig[8] = { 0 };
+        float* wr = malloc(sizeof(float) * 4 * (size_t)ncells);
+        if (!wr)
+            goto out;
+        float *wi = wr + ncells, *kr = wr + 2 * ncells, *ki = wr + 3 * ncells;
+        for (int i = 0; i < ncells; i++) {
+            const float u = I[i], v = Qv[i + rot];
+            const float c1 = C[i] + eps, c2 = C[i + rot] + eps;
+            /* (ct u + sn v) norm, (ct v - sn u) norm: the first product
+             * fused (dvbt2.c:2285) */
+            wr[i] = fmaf(ct, u, sn * v) * norm;
+            wi[i] = fmaf(ct, v, -(sn * u)) * norm;
+            kr[i] = ki[i] = c1;
+            if (rot) {
+                /* c1 c2 / (ct^2 c2 + sn^2 c1) and c1 c2 / (sn^2 c2 +
+                 * ct^2 c1) (dvbt2.c:2289-2291), the c1 product fused */
+                const float c12 = c1 * c2;
+                kr[i] = c12 / fmaf(sn2, c1, ct2 * c2);
+                ki[i] = c12 / fmaf(ct2, c1, sn2 * c2);
+            }
+            /* MER against the nearest odd level (dvbt2.c:2296-2303) */
+            const float awr = fabsf(wr[i]), awi = fabsf(wi[i]);
+            const float ar = 2 * truncf(fminf(awr * 0.5f, amax)) + 1;
+            const float ai = 2 * truncf(fminf(awi * 0.5f, amax)) + 1;
+            const float er = awr - ar, ei = awi - ai;
+            verr[i % 8] += fmaf(er * er, kr[i], ei * ei * ki[i]);
+            vsig[i % 8] += s2 * (kr[i] + ki[i]);
+        }
+        float err = 0, sig = 0;
+        for (int l = 0; l < 8; l++) {
+            err += verr[l];
+            sig += vsig[l];
+        }
+        err_out[b] = err;
+        sig_out[b] = sig;
+        const float n0u = err / ncells + 1e-6f;
+        const float k = 8.0f / n0u;
+        /* piecewise-linear max-log values per axis (dvbt2.c:2333-2349) */
+        for (int i = 0; i < ncells; i++) {
+            float ur = wr[i], ui = wi[i];
+            for (int j = 0; j < nb; j++) {
+                if (j) {
+                    const float t = (float)(1 << (nb - j));
+                    ur = fabsf(ur) - t;
+                    ui = fabsf(ui) - t;
+                }
+                soft[i * m + 2 * j] = t2_to_llr(ur * kr[i] * k);
+                soft[i * m + 2 * j + 1] = t2_to_llr(ui * ki[i] * k);
+            }
+        }
+        free(wr);
+        int8_t* o = llr + (size_t)b * N;
+        for (int n = 0; n < N; n++)
+            o[n] = soft[src[n]];
+    }
+    rc = 0;
+out:
+    free(perm);
+    free(P);
+    free(src);
+    free(w);
+    free(soft);
+    return rc;
+}

@@ -119,4 +119,32 @@ int oracle_demap_frame(int pls, const float* sym, float n0, int8_t* llr);
 int oracle_bch_nbch(int short_frame, const char* rate);
 int oracle_bch_decode(int short_frame, const char* rate, uint8_t* cw);
 
+/* ---- DVB-T2 FEC block front end (libdvbrx/src/dvbt2.c:1953-2082,
+ * :2188-2390) ---------------------------------------------------------
+ *
+ * A PLP's FEC blocks as the reference's FEC thread takes them: cod 0..5
+ * (1/2 3/5 2/3 3/4 4/5 5/6), mod 0..3 (QPSK, 16-, 64-, 256-QAM), fec_type
+ * 1 normal / 0 short, rotation 0/1, and the time interleaver's length and
+ * type (the cell deinterleaver's shift restarts with each TI-block when
+ * the type is 0).
+ *
+ * oracle_t2_geometry: nldpc, ncells (= nldpc / bits per cell) and the LDPC
+ * code name ("2_3_t2" and "3_5_t2" for the T2-only codes); -1 unsupported.
+ * oracle_t2_cellperm: the cell interleaver's basic permutation L0
+ * (ncells entries).  oracle_t2_shift: each block's cyclic shift P.
+ * oracle_t2_bitsrc: for each codeword bit n, the soft value it takes:
+ * cell * m + plane (plane 2j: bit j of I, 2j + 1: bit j of Q) -- the
+ * demultiplexer, column twist and parity interleaving undone.
+ * oracle_t2_demap: nblocks blocks of ncells complex cells (re, im) and
+ * their channel state -> nldpc int8 soft values per block in codeword
+ * order (positive for a 0 bit) and the block's MER error and signal sums,
+ * as the reference's x86-64 build with FMA contraction computes them.
+ * Returns 0, -1 unsupported. */
+int oracle_t2_geometry(int cod, int mod, int fec_type, int* nldpc, int* ncells, char* code, int code_len);
+int oracle_t2_cellperm(int ncells, int32_t* perm);
+int oracle_t2_shift(int nblocks, int til_length, int til_type, int ncells, int32_t* P);
+int oracle_t2_bitsrc(int cod, int mod, int fec_type, uint16_t* src);
+int oracle_t2_demap(int cod, int mod, int fec_type, int rotation, int til_length, int til_type, int nblocks,
+                    const float* cells, const float* csi, int8_t* llr, float* err, float* sig);
+
 #endif

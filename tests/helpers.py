@@ -71,6 +71,12 @@ def oracle():
         L.oracle_demap_frame.argtypes = [C.c_int, vp, C.c_float, vp]
         L.oracle_bch_nbch.argtypes = [C.c_int, C.c_char_p]
         L.oracle_bch_decode.argtypes = [C.c_int, C.c_char_p, vp]
+        ip = C.POINTER(C.c_int)
+        L.oracle_t2_geometry.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, C.c_char_p, C.c_int]
+        L.oracle_t2_cellperm.argtypes = [C.c_int, vp]
+        L.oracle_t2_shift.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp]
+        L.oracle_t2_bitsrc.argtypes = [C.c_int, C.c_int, C.c_int, vp]
+        L.oracle_t2_demap.argtypes = [C.c_int] * 7 + [vp] * 5
         _oracle = L
     return _oracle
 
@@ -377,3 +383,178 @@ def flip(cw, bits):
     for bit in bits:
         out[bit // 8] ^= 0x80 >> (bit % 8)
     return out
+
+
+# ---- DVB-T2 FEC blocks -------------------------------------------------
+
+T2_RATES = ["1_2", "3_5", "2_3", "3_4", "4_5", "5_6"]
+
+
+def t2_geometry(cod, mod, fec_type):
+    """(N, ncells, LDPC code name) of a PLP configuration (oracle)."""
+    n, nc = C.c_int(), C.c_int()
+    name = C.create_string_buffer(16)
+    assert oracle().oracle_t2_geometry(cod, mod, fec_type, C.byref(n), C.byref(nc), name, 16) == 0
+    return n.value, nc.value, name.value.decode()
+
+
+def oracle_t2_demap(cod, mod, fec_type, rot, til, cells, csi):
+    """Oracle front end: (llr (nblocks, N), err, sig)."""
+    N, nc, _ = t2_geometry(cod, mod, fec_type)
+    nb = cells.shape[0]
+    llr = np.zeros((nb, N), np.int8)
+    err = np.zeros(nb, np.float32)
+    sig = np.zeros(nb, np.float32)
+    cells = np.ascontiguousarray(cells, np.float32)
+    csi = np.ascontiguousarray(csi, np.float32)
+    assert oracle().oracle_t2_demap(cod, mod, fec_type, rot, til[0], til[1], nb, cells.ctypes.data,
+                                    csi.ctypes.data, llr.ctypes.data, err.ctypes.data, sig.ctypes.data) == 0
+    return llr, err, sig
+
+
+def oracle_t2_job(cod, mod, fec_type, rot, til, cells, csi, max_iter, lanes):
+    """The reference's FEC job restated with the oracle: front end, LDPC in
+    calls of `lanes` blocks, BCH.  (codewords (nblocks, kldpc/8) after BCH,
+    BCH results, err, sig)."""
+    llr, err, sig = oracle_t2_demap(cod, mod, fec_type, rot, til, cells, csi)
+    sf = fec_type == 0
+    _, _, code = t2_geometry(cod, mod, fec_type)
+    fec = ldpcgpu.fec_lookup(sf, T2_RATES[cod])
+    d = oracle().oracle_ldpc_new(sf, code.encode(), fec.kldpc)
+    assert d, code
+    nb = llr.shape[0]
+    kb = fec.kldpc // 8
+    cw = np.zeros((nb, kb), np.uint8)
+    try:
+        for b0 in range(0, nb, lanes):
+            part = np.ascontiguousarray(llr[b0:b0 + lanes])
+            out = np.zeros((part.shape[0], kb), np.uint8)
+            oracle().oracle_ldpc_decode(d, part.ctypes.data, out.ctypes.data, part.shape[0], max_iter)
+            cw[b0:b0 + part.shape[0]] = out
+    finally:
+        oracle().oracle_ldpc_free(d)
+    res = np.zeros(nb, np.int32)
+    for b in range(nb):
+        res[b], cw[b] = oracle_bch(sf, T2_RATES[cod], cw[b])
+    return cw, res, err, sig
+
+
+def t2_ref_builds():
+    """The reference's T2 FEC job (oracle/_ref/libdrxt2ref_*.so), name ->
+    (CDLL, LDPC lanes); empty where not built or the host lacks the SIMD
+    level."""
+    flags = ""
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        pass
+    out = {}
+    for name, need, lanes in (("v3", "avx2", 32), ("v4", "avx512bw", 64)):
+        path = os.path.join(REF_DIR, f"libdrxt2ref_{name}.so")
+        if not os.path.exists(path) or need not in flags:
+            continue
+        L = C.CDLL(path)
+        L.t2ref_decode.argtypes = [C.c_int] * 7 + [C.c_void_p] * 2 + [C.c_int] + [C.c_void_p] * 5
+        out[name] = (L, lanes)
+    return out
+
+
+def ref_t2_job(L, cod, mod, fec_type, rot, til, cells, csi, max_iter, with_llr=False):
+    """decode_job of a reference build: (bb (nblocks, kbch/8), ok, err, sig,
+    llr or None)."""
+    N, nc, _ = t2_geometry(cod, mod, fec_type)
+    fec = ldpcgpu.fec_lookup(fec_type == 0, T2_RATES[cod])
+    nb = cells.shape[0]
+    kb = fec.kbch // 8
+    bb = np.zeros((nb, kb), np.uint8)
+    ok = np.zeros(nb, np.uint8)
+    err = np.zeros(nb, np.float32)
+    sig = np.zeros(nb, np.float32)
+    llr = np.zeros((nb, N), np.int8) if with_llr else None
+    cells = np.ascontiguousarray(cells, np.float32)
+    csi = np.ascontiguousarray(csi, np.float32)
+    rc = L.t2ref_decode(cod, mod, fec_type, rot, til[0], til[1], nb, cells.ctypes.data, csi.ctypes.data, max_iter,
+                        bb.ctypes.data, ok.ctypes.data, err.ctypes.data, sig.ctypes.data,
+                        llr.ctypes.data if with_llr else None)
+    assert rc == 0
+    return bb, ok, err, sig, llr
+
+
+def t2_random_cells(mod, ncells, nblocks, rng, snr_db=15.0):
+    """Random constellation points (any labels) plus noise, and random
+    channel state: exercises every soft-value branch."""
+    m = 2 * (mod + 1)
+    L = 2 ** (m // 2)
+    lev = np.arange(-(L - 1), L, 2)
+    norm = np.sqrt({2: 2.0, 4: 10.0, 6: 42.0, 8: 170.0}[m])
+    x = (rng.choice(lev, (nblocks, ncells)) + 1j * rng.choice(lev, (nblocks, ncells))) / norm
+    s = 10 ** (-snr_db / 20) / np.sqrt(2)
+    x = x + s * (rng.standard_normal(x.shape) + 1j * rng.standard_normal(x.shape))
+    cells = np.stack([x.real, x.imag], -1).astype(np.float32)
+    csi = rng.uniform(0.2, 2.0, (nblocks, ncells)).astype(np.float32)
+    return cells, csi
+
+
+T2_ROT_DEG = [29.0, 16.8, 8.6, 3.576334375]
+
+
+def t2_modulate(cod, mod, fec_type, rot, til, cw_bits, snr_db, rng):
+    """Transmit side of the front end, for round trips: LDPC codewords
+    (nblocks, N) of bits -> cells and channel state.  Each codeword bit goes
+    to the (cell, plane) oracle_t2_bitsrc names; the planes of an axis are
+    the Gray-coded PAM level whose max-log values have those signs (bit 0 the
+    sign, bit j > 0 the sign of |u| - 2^(nb-j)); the point is rotated and its
+    Q sent one cell later (cyclically); the cell interleaver puts cell i at
+    (L0(i) + P) mod ncells; AWGN at snr_db."""
+    N, nc, _ = t2_geometry(cod, mod, fec_type)
+    m = 2 * (mod + 1)
+    nb = m // 2
+    src = np.zeros(N, np.uint16)
+    assert oracle().oracle_t2_bitsrc(cod, mod, fec_type, src.ctypes.data) == 0
+    perm = np.zeros(nc, np.int32)
+    assert oracle().oracle_t2_cellperm(nc, perm.ctypes.data) == 0
+    nblocks = cw_bits.shape[0]
+    P = np.zeros(nblocks, np.int32)
+    oracle().oracle_t2_shift(nblocks, til[0], til[1], nc, P.ctypes.data)
+    norm = np.sqrt({2: 2.0, 4: 10.0, 6: 42.0, 8: 170.0}[m])
+    th = np.deg2rad(T2_ROT_DEG[mod]) if rot else 0.0
+    cells = np.zeros((nblocks, nc, 2), np.float32)
+    for b in range(nblocks):
+        planes = np.zeros(nc * m, np.uint8)
+        planes[src.astype(np.int64)] = cw_bits[b]
+        planes = planes.reshape(nc, m)
+        axis = []
+        for a in (0, 1):
+            bits = planes[:, a::2]
+            u = np.where(bits[:, nb - 1] == 0, 1.0, -1.0)
+            for j in range(nb - 1, 0, -1):
+                u = (u + 2.0 ** (nb - j)) * np.where(bits[:, j - 1] == 0, 1.0, -1.0)
+            axis.append(u)
+        x = (axis[0] + 1j * axis[1]) / norm * np.exp(1j * th)
+        tx_i = x.real
+        tx_q = np.roll(x.imag, 1) if rot else x.imag
+        y = tx_i + 1j * tx_q
+        s = 10 ** (-snr_db / 20) / np.sqrt(2)
+        y = y + s * (rng.standard_normal(nc) + 1j * rng.standard_normal(nc))
+        pos = (perm.astype(np.int64) + P[b]) % nc
+        cells[b, pos, 0] = y.real
+        cells[b, pos, 1] = y.imag
+    csi = np.ones((nblocks, nc), np.float32)
+    return cells, csi
+
+
+def t2_codewords(cod, fec_type, nblocks, rng):
+    """Random BBFRAMEs through the BCH and LDPC encoders: (info bytes
+    (nblocks, kbch/8), LDPC codeword bits (nblocks, N))."""
+    sf = fec_type == 0
+    rate = T2_RATES[cod]
+    fec = ldpcgpu.fec_lookup(sf, rate)
+    _, _, code = t2_geometry(cod, 0, fec_type)
+    L = ldpcgpu.sim_lib()
+    info = rng.integers(0, 2, (nblocks, fec.kbch), dtype=np.uint8)
+    bits = np.zeros((nblocks, fec.nldpc), np.uint8)
+    for b in range(nblocks):
+        bch = np.zeros(fec.kldpc, np.uint8)
+        assert L.sim_bch_encode(sf, rate.encode(), np.ascontiguousarray(info[b]).ctypes.data, bch.ctypes.data) == 0
+        assert L.sim_ldpc_encode(sf, code.encode(), bch.ctypes.data, bits[b].ctypes.data) == 0
+    return np.packbits(info, axis=1), bits
