@@ -320,6 +320,139 @@ def bch_record(dec, nframes=8192, reps=5):
     return out
 
 
+T2_PLPS = [
+    # (name, cod, mod, fec_type, Es/N0 dB: lowest 0.2 dB step at which 16
+    # distinct blocks all decode, plus 0.3 dB, blocks), rotated, with a
+    # type-0 time interleaver of 8 TI-blocks (one call is one interleaving
+    # frame; 8 keeps each TI-block within the cell interleaver's shifts)
+    ("normal 256-QAM 3/5 rotated", 1, 3, 1, 18.7, 4096),
+    ("short 64-QAM 2/3 rotated", 2, 2, 0, 14.1, 8192),
+]
+T2_TI_BLOCKS = 8
+
+
+def t2_cpu_job(plp, cells, csi, max_iter, seconds=8.0, max_threads=16):
+    """The reference's DVB-T2 FEC job (dvbt2.c decode_job, built from its
+    sources into oracle/_ref) on this host: one thread per core, each
+    decoding calls of 64 blocks of the same cells until the deadline."""
+    flags = cpu_flags()
+    lib, what = None, None
+    for name, need in (("v4", "avx512bw"), ("v3", "avx2")):
+        p = os.path.join(ROOT, "oracle", "_ref", f"libdrxt2ref_{name}.so")
+        if need in flags and os.path.exists(p):
+            lib, what = C.CDLL(p), f"libdvbrx dvbt2.c decode_job built -O3 -march=x86-64-{name} (clang)"
+            break
+    if lib is None:
+        return None
+    lib.t2ref_decode.argtypes = [C.c_int] * 7 + [C.c_void_p] * 2 + [C.c_int] + [C.c_void_p] * 5
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    nthreads = max(1, min(max_threads, len(cpus)))
+    nb = min(64, cells.shape[0])
+    c = np.ascontiguousarray(cells[:nb])
+    h = np.ascontiguousarray(csi[:nb])
+    N, K, kbch, nc = ldpcgpu.t2_geometry(plp)
+    count = [0] * nthreads
+    deadline = time.perf_counter() + seconds
+
+    def worker(tid):
+        try:
+            os.sched_setaffinity(0, {cpus[tid % len(cpus)]})
+        except (AttributeError, OSError):
+            pass
+        bb = np.zeros(nb * kbch // 8, np.uint8)
+        ok = np.zeros(nb, np.uint8)
+        e = np.zeros(nb, np.float32)
+        g = np.zeros(nb, np.float32)
+        while time.perf_counter() < deadline:
+            rc = lib.t2ref_decode(plp.cod, plp.mod, plp.fec_type, plp.rotation, plp.til_length, plp.til_type, nb,
+                                  c.ctypes.data, h.ctypes.data, max_iter, bb.ctypes.data, ok.ctypes.data,
+                                  e.ctypes.data, g.ctypes.data, None)
+            if rc:
+                break
+            count[tid] += nb
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(nthreads)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    sec = time.perf_counter() - t0
+    blocks = sum(count)
+    return {"value": round(blocks * kbch / sec / 1e9, 4), "unit": "Gbit/s", "cores": nthreads, "kind": "reference",
+            "blocks_s": round(blocks / sec, 1),
+            "sample": f"{what}, {nthreads} threads, calls of {nb} blocks, {blocks} blocks in {sec:.1f} s"}
+
+
+def t2_record(dec, reps=5, cpu=True):
+    """DVB-T2 FEC blocks of two PLPs (the T2 PLP codes of the streaming
+    mix: normal 3/5, short 2/3), resident in HBM: the front end alone
+    (ldpc_gpu_t2_demap: cell and bit deinterleaving, derotation, MER, soft
+    values) and the whole job (ldpc_gpu_t2_decode: front end, LDPC up to 30
+    iterations, BCH), every block checked against its BBFRAME.  Blocks
+    are 16 distinct encoded frames repeated; the reference's job on this
+    host's cores beside it."""
+    out = []
+    for name, cod, mod, ft, snr, nblocks in T2_PLPS:
+        plp = ldpcgpu.T2Plp(cod, mod, ft, 1, T2_TI_BLOCKS, 0)
+        N, K, kbch, nc = ldpcgpu.t2_geometry(plp)
+        cells, csi, info = workloads.t2_blocks(plp, nblocks, snr)
+        kb = K // 8
+        sizes = dict(cells=cells.nbytes, csi=csi.nbytes, llr=nblocks * N, cw=nblocks * kb, it=4 * nblocks,
+                     syn=nblocks, bch=4 * nblocks, err=4 * nblocks, sig=4 * nblocks)
+        d = {k: dec.malloc(v) for k, v in sizes.items()}
+        try:
+            dec.upload(d["cells"], cells)
+            dec.upload(d["csi"], csi)
+
+            def front():
+                ldpcgpu.t2_demap(dec, plp, nblocks, d["cells"], d["csi"], d["llr"], d["err"], d["sig"])
+                dec.synchronize()
+
+            def job(lanes):
+                ldpcgpu.t2_decode(dec, plp, nblocks, d["cells"], d["csi"], 30, lanes, d["llr"], d["cw"], d["it"],
+                                  d["syn"], d["bch"], d["err"], d["sig"])
+                dec.synchronize()
+
+            def timed(fn, *a):
+                fn(*a)
+                dt = []
+                for _ in range(reps):
+                    t = time.perf_counter()
+                    fn(*a)
+                    dt.append(time.perf_counter() - t)
+                return float(np.median(dt))
+
+            t_front = timed(front)
+            t_job = timed(job, 0)
+            res = np.zeros(nblocks, np.int32)
+            cw = np.zeros((nblocks, kb), np.uint8)
+            it = np.zeros(nblocks, np.int32)
+            dec.download(res, d["bch"])
+            dec.download(cw, d["cw"])
+            dec.download(it, d["it"])
+            good = int(((res >= 0) & (cw[:, :kbch // 8] == info).all(1)).sum())
+            t_job32 = timed(job, 32)
+            alg = nblocks * (12 * nc + N)
+            rec = {"plp": name, "es_n0_db": snr, "blocks": nblocks, "cells_per_block": nc,
+                   "front_end_ms": round(t_front * 1e3, 4), "front_end_gcells_s": round(nblocks * nc / t_front / 1e9, 2),
+                   "front_end_hbm_frac": round(alg / t_front / 1e9 / HBM_PEAK_GBS, 4),
+                   "job_ms": round(t_job * 1e3, 3), "job_info_gbit_s": round(nblocks * kbch / t_job / 1e9, 3),
+                   "job_ms_lanes32": round(t_job32 * 1e3, 3),
+                   "job_info_gbit_s_lanes32": round(nblocks * kbch / t_job32 / 1e9, 3),
+                   "avg_iterations": round(float(it.mean()), 2), "blocks_decoded": good}
+            if cpu:
+                rec["cpu_baseline"] = t2_cpu_job(plp, cells, csi, 30)
+            out.append(rec)
+        finally:
+            for v in d.values():
+                dec.free(v)
+    return out
+
+
 def config4(dec, nf=8192, db=0.2, reps=3):
     """Learned vs plain decoding on the two trained MODCODs (DVB-S2 normal
     3/5 8PSK and 2/3 16APSK) at Es/N0 = QEF + db: frames from pool
@@ -397,9 +530,9 @@ def main():
                     help="skip the side measurements (64-frame rate 1/2 batch, learned vs plain)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
-    ap.add_argument("--only", choices=["demapper", "bch", "learned"], default=None,
+    ap.add_argument("--only", choices=["demapper", "bch", "learned", "t2"], default=None,
                     help="only run that side record (the demapper's or the BCH decoder's throughput, learned vs "
-                         "plain decoding) and print it")
+                         "plain decoding, DVB-T2 FEC blocks) and print it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: frames are independent, so N GPUs would each decode their "
@@ -410,7 +543,7 @@ def main():
         return
     if args.only:
         dec = ldpcgpu.Decoder(0)
-        rec = {"demapper": demap_record, "bch": bch_record, "learned": config4}[args.only](dec)
+        rec = {"demapper": demap_record, "bch": bch_record, "learned": config4, "t2": t2_record}[args.only](dec)
         dec.close()
         print(json.dumps({args.only: rec}), flush=True)
         return
@@ -547,6 +680,7 @@ def main():
     c3 = None if args.no_config1 else [config3(dec, 0.2), config3(dec, 1.0)]
     dm = None if args.no_config1 else demap_record(dec)
     bch = None if args.no_config1 else bch_record(dec)
+    t2 = None if args.no_config1 else t2_record(dec, cpu=not args.no_cpu)
     c5 = None
     if not args.no_config1:
         import streaming
@@ -625,6 +759,7 @@ def main():
         "streaming": c5,
         "demapper": dm,
         "bch": bch,
+        "t2": t2,
     }
     if cpu:
         res["gpu_vs_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None

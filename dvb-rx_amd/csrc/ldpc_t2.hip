@@ -195,8 +195,10 @@ struct T2Args {
 /* Shift P of block b (dvbt2.c:2230-2254): blocks are dealt to the nti
  * TI-blocks in order, the last nblocks % nti taking one more; the r-th
  * block of a TI-block takes the r-th nd-bit bit-reversed counter value
- * below ncells. */
-__device__ int block_shift(int b, int nblocks, int nti, int ncells, int nd)
+ * below ncells.  The whole workgroup scans 256 counter values per step
+ * (wave ballots give each valid value its rank); every thread returns P.
+ * The host keeps r below the number of valid values. */
+__device__ int block_shift(int b, int nblocks, int nti, int ncells, int nd, int* s_P, int* s_wcount)
 {
     if (!nti)
         return 0;
@@ -208,45 +210,82 @@ __device__ int block_shift(int b, int nblocks, int nti, int ncells, int nd)
         cum += ns;
     }
     const int r = b - cum;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0)
+        *s_P = 0;
     int seen = 0;
-    for (unsigned k = 0; k < (1u << nd); k++) {
+    for (unsigned k0 = 0; k0 < (1u << nd); k0 += T2_BLOCK) {
+        const unsigned k = k0 + tid;
         const int v = (int)(__brev(k) >> (32 - nd));
-        if (v < ncells) {
-            if (seen == r)
-                return v;
-            seen++;
+        const bool valid = k < (1u << nd) && v < ncells;
+        const unsigned long long mask = __ballot(valid);
+        if (lane == 0)
+            s_wcount[wave] = __popcll(mask);
+        __syncthreads();
+        int off = seen, total = 0;
+        for (int w = 0; w < T2_BLOCK / 64; w++) {
+            off += w < wave ? s_wcount[w] : 0;
+            total += s_wcount[w];
         }
+        if (valid && off + __popcll(mask & ((1ull << lane) - 1)) == r)
+            *s_P = v;
+        __syncthreads();
+        if (seen + total > r)
+            break;
+        seen += total;
     }
-    return 0; /* not reached: the host bounds blocks per TI-block below ncells */
+    return *s_P;
 }
 
-/* Derotated point of cell i and its per-axis weights (dvbt2.c:2257-2292):
- * I from deinterleaved cell i, Q from cell i + 1 (cyclically) when the
- * constellation is rotated. */
-__device__ __forceinline__ void t2_point(const T2Args& a, const float2* in, const float* ic, int P, int i, float& wr,
-                                         float& wi, float& kr, float& ki)
+constexpr int T2_U = T2_CHUNK / T2_BLOCK; /* cells a thread takes per pass */
+
+/* Derotated points of cells i0 + u * T2_BLOCK (u < T2_U) and their
+ * per-axis weights (dvbt2.c:2257-2292): I from deinterleaved cell i, Q
+ * from cell i + 1 (cyclically) when the constellation is rotated.  All
+ * loads of the group are issued before any arithmetic (the cell gathers
+ * depend on the permutation loads; a cell at a time would leave one chain
+ * of two HBM latencies per thread in flight).  Cells past the end are
+ * computed on cell ncells - 1 and left for the caller to drop. */
+__device__ __forceinline__ void t2_points(const T2Args& a, const float2* in, const float* ic, int P, int i0,
+                                          float* wr, float* wi, float* kr, float* ki)
 {
 #pragma clang fp contract(off)
-    int s = a.perm[i] + P;
-    s -= s >= a.ncells ? a.ncells : 0;
-    const float2 x = in[s];
-    const float c1 = ic[s] + 1e-12f;
-    float v = x.y, c2 = c1;
-    if (a.rot) {
-        const int i1 = i + 1 == a.ncells ? 0 : i + 1;
-        int s1 = a.perm[i1] + P;
-        s1 -= s1 >= a.ncells ? a.ncells : 0;
-        v = in[s1].y;
-        c2 = ic[s1] + 1e-12f;
+    int s[T2_U], s1[T2_U];
+#pragma unroll
+    for (int u = 0; u < T2_U; u++) {
+        const int i = min(i0 + u * T2_BLOCK, a.ncells - 1);
+        s[u] = a.perm[i];
+        s1[u] = a.perm[i + 1 == a.ncells ? 0 : i + 1];
     }
-    const float u = x.x;
-    wr = fmaf(a.ct, u, a.sn * v) * a.norm;
-    wi = fmaf(a.ct, v, -(a.sn * u)) * a.norm;
-    kr = ki = c1;
-    if (a.rot) {
-        const float c12 = c1 * c2;
-        kr = c12 / fmaf(a.sn2, c1, a.ct2 * c2);
-        ki = c12 / fmaf(a.ct2, c1, a.sn2 * c2);
+    float2 x[T2_U];
+    float c1[T2_U], v[T2_U], c2[T2_U];
+#pragma unroll
+    for (int u = 0; u < T2_U; u++) {
+        int t = s[u] + P;
+        t -= t >= a.ncells ? a.ncells : 0;
+        x[u] = in[t];
+        c1[u] = ic[t];
+        if (a.rot) {
+            int t1 = s1[u] + P;
+            t1 -= t1 >= a.ncells ? a.ncells : 0;
+            v[u] = in[t1].y;
+            c2[u] = ic[t1];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < T2_U; u++) {
+        const float e1 = c1[u] + 1e-12f;
+        const float q = a.rot ? v[u] : x[u].y;
+        const float e2 = a.rot ? c2[u] + 1e-12f : e1;
+        const float p = x[u].x;
+        wr[u] = fmaf(a.ct, p, a.sn * q) * a.norm;
+        wi[u] = fmaf(a.ct, q, -(a.sn * p)) * a.norm;
+        kr[u] = ki[u] = e1;
+        if (a.rot) {
+            const float c12 = e1 * e2;
+            kr[u] = c12 / fmaf(a.sn2, e1, a.ct2 * e2);
+            ki[u] = c12 / fmaf(a.ct2, e1, a.sn2 * e2);
+        }
     }
 }
 
@@ -263,15 +302,12 @@ __global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
     constexpr int NB = M / 2;
     __shared__ __attribute__((aligned(16))) int8_t soft[T2_LDS];
     __shared__ float lane_sum[16];
-    __shared__ int s_P;
+    __shared__ int s_P, s_wcount[T2_BLOCK / 64];
     __shared__ float s_k;
     float* tE = (float*)soft;
     float* tS = tE + T2_CHUNK;
     const int tid = threadIdx.x, b = blockIdx.x;
-    if (tid == 0)
-        s_P = block_shift(b, a.nblocks, a.nti, a.ncells, a.nd);
-    __syncthreads();
-    const int P = s_P;
+    const int P = block_shift(b, a.nblocks, a.nti, a.ncells, a.nd, &s_P, s_wcount);
     const float2* in = a.cells + (size_t)b * a.ncells;
     const float* ic = a.csi + (size_t)b * a.ncells;
 
@@ -279,18 +315,19 @@ __global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
      * thread 8 + l sig lane l */
     float acc = 0.0f;
     for (int base = 0; base < a.ncells; base += T2_CHUNK) {
-        for (int j = tid; j < T2_CHUNK; j += T2_BLOCK) {
-            const int i = base + j;
+        float wr[T2_U], wi[T2_U], kr[T2_U], ki[T2_U];
+        t2_points(a, in, ic, P, base + tid, wr, wi, kr, ki);
+#pragma unroll
+        for (int u = 0; u < T2_U; u++) {
+            const int j = tid + u * T2_BLOCK;
             float te = 0.0f, ts = 0.0f;
-            if (i < a.ncells) {
-                float wr, wi, kr, ki;
-                t2_point(a, in, ic, P, i, wr, wi, kr, ki);
-                const float awr = fabsf(wr), awi = fabsf(wi);
+            if (base + j < a.ncells) {
+                const float awr = fabsf(wr[u]), awi = fabsf(wi[u]);
                 const float ar = 2.0f * truncf(fminf(awr * 0.5f, a.amax)) + 1.0f;
                 const float ai = 2.0f * truncf(fminf(awi * 0.5f, a.amax)) + 1.0f;
                 const float er = awr - ar, ei = awi - ai;
-                te = fmaf(er * er, kr, ei * ei * ki);
-                ts = a.s2 * (kr + ki);
+                te = fmaf(er * er, kr[u], ei * ei * ki[u]);
+                ts = a.s2 * (kr[u] + ki[u]);
             }
             tE[j] = te;
             tS[j] = ts;
@@ -323,34 +360,52 @@ __global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
     const float k = s_k;
 
     /* soft values by (cell, plane) (dvbt2.c:2333-2359) */
-    for (int i = tid; i < a.ncells; i += T2_BLOCK) {
-        float ur, ui, kr, ki;
-        t2_point(a, in, ic, P, i, ur, ui, kr, ki);
-        int8_t v[M];
+    for (int base = 0; base < a.ncells; base += T2_CHUNK) {
+        float wr[T2_U], wi[T2_U], kr[T2_U], ki[T2_U];
+        t2_points(a, in, ic, P, base + tid, wr, wi, kr, ki);
 #pragma unroll
-        for (int j = 0; j < NB; j++) {
-            if (j) {
-                const float t = (float)(1 << (NB - j));
-                ur = fabsf(ur) - t;
-                ui = fabsf(ui) - t;
+        for (int u = 0; u < T2_U; u++) {
+            const int i = base + tid + u * T2_BLOCK;
+            if (i >= a.ncells)
+                continue;
+            float ur = wr[u], ui = wi[u];
+            int8_t v[M];
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                if (j) {
+                    const float t = (float)(1 << (NB - j));
+                    ur = fabsf(ur) - t;
+                    ui = fabsf(ui) - t;
+                }
+                v[2 * j] = t2_llr((ur * kr[u]) * k);
+                v[2 * j + 1] = t2_llr((ui * ki[u]) * k);
             }
-            v[2 * j] = t2_llr((ur * kr) * k);
-            v[2 * j + 1] = t2_llr((ui * ki) * k);
-        }
 #pragma unroll
-        for (int p = 0; p < M; p++)
-            soft[i * M + p] = v[p];
+            for (int p = 0; p < M; p++)
+                soft[i * M + p] = v[p];
+        }
     }
     __syncthreads();
 
-    /* bit deinterleaving as a gather, 4 codeword bits a thread and step */
+    /* bit deinterleaving as a gather, 4 codeword bits a thread and word;
+     * the table words of 8 steps are loaded before their gathers */
     uint32_t* out = (uint32_t*)(a.llr + (size_t)b * a.N);
     const uint2* src = (const uint2*)a.src;
-    for (int w = tid; w < a.N / 4; w += T2_BLOCK) {
-        const uint2 s = src[w];
-        const uint32_t b0 = (uint8_t)soft[s.x & 0xFFFF], b1 = (uint8_t)soft[s.x >> 16];
-        const uint32_t b2 = (uint8_t)soft[s.y & 0xFFFF], b3 = (uint8_t)soft[s.y >> 16];
-        out[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+    const int nw = a.N / 4;
+    for (int w0 = tid; w0 < nw; w0 += 8 * T2_BLOCK) {
+        uint2 sv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            sv[u] = src[min(w0 + u * T2_BLOCK, nw - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int w = w0 + u * T2_BLOCK;
+            if (w < nw) {
+                const uint32_t b0 = (uint8_t)soft[sv[u].x & 0xFFFF], b1 = (uint8_t)soft[sv[u].x >> 16];
+                const uint32_t b2 = (uint8_t)soft[sv[u].y & 0xFFFF], b3 = (uint8_t)soft[sv[u].y >> 16];
+                out[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+            }
+        }
     }
 }
 
@@ -395,6 +450,55 @@ extern "C" int ldpc_gpu_t2_geometry(const ldpc_gpu_t2_plp* plp, int* nldpc, int*
         *kbch = g.kbch;
     if (ncells)
         *ncells = g.ncells;
+    return LDPC_GPU_OK;
+}
+
+extern "C" int ldpc_gpu_t2_maps(const ldpc_gpu_t2_plp* plp, int nblocks, int32_t* perm, uint16_t* bitsrc,
+                                int32_t* shift)
+{
+    PlpGeom g;
+    if (plp_geom(plp, g) || nblocks < 0 || plp->til_length < 0 || plp->til_length > 255)
+        return LDPC_GPU_EINVAL;
+    if (perm) {
+        std::vector<int32_t> h;
+        if (!build_cellperm(g.ncells, h))
+            return LDPC_GPU_EINVAL;
+        std::memcpy(perm, h.data(), sizeof(int32_t) * h.size());
+    }
+    if (bitsrc) {
+        std::vector<uint16_t> h;
+        if (!build_bitsrc(plp->cod, plp->mod, plp->fec_type == 0, g.N, g.K, h))
+            return LDPC_GPU_EINVAL;
+        std::memcpy(bitsrc, h.data(), sizeof(uint16_t) * h.size());
+    }
+    if (shift) {
+        /* the kernel's block_shift, on the host */
+        const int nti = plp->til_length && plp->til_type == 0 ? plp->til_length : 0;
+        const int nd = nbits_for(g.ncells);
+        int s = 0, left = 0;
+        unsigned k = 0;
+        for (int b = 0; b < nblocks; b++) {
+            shift[b] = 0;
+            if (!nti)
+                continue;
+            while (left == 0 && s < nti) {
+                left = nblocks / nti + (s >= nti - nblocks % nti);
+                s++;
+                k = 0;
+            }
+            for (; k < (1u << nd); k++) {
+                unsigned v = 0;
+                for (int j = 0; j < nd; j++)
+                    v |= ((k >> j) & 1u) << (nd - 1 - j);
+                if ((int)v < g.ncells) {
+                    shift[b] = (int32_t)v;
+                    k++;
+                    break;
+                }
+            }
+            left--;
+        }
+    }
     return LDPC_GPU_OK;
 }
 

@@ -80,6 +80,7 @@ def gpu_lib():
         L.ldpc_gpu_bch_decode.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, vp, C.c_size_t, vp]
         L.ldpc_gpu_t2_geometry.argtypes = [vp, ip, ip, ip, ip]
         L.ldpc_gpu_t2_demap.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
+        L.ldpc_gpu_t2_maps.argtypes = [vp, i32, vp, vp, vp]
         L.ldpc_gpu_t2_decode.argtypes = [vp, vp, i32, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp]
         L.ldpc_gpu_decode_batch.argtypes = [vp, vp, i32, vp, vp, vp, vp]
         L.ldpc_gpu_decode_host.argtypes = [vp, vp, i32, vp, sz, vp, sz, vp, vp]
@@ -481,6 +482,18 @@ def t2_geometry(plp):
     if gpu_lib().ldpc_gpu_t2_geometry(C.byref(plp), *[C.byref(x) for x in v]):
         raise ValueError(f"unsupported DVB-T2 PLP {[getattr(plp, f) for f, _ in plp._fields_]}")
     return tuple(x.value for x in v)
+
+
+def t2_maps(plp, nblocks=0):
+    """(perm, bitsrc, shift) of ldpc_gpu_t2_maps as numpy arrays (host
+    only: no device call)."""
+    N, K, kbch, nc = t2_geometry(plp)
+    perm = np.zeros(nc, np.int32)
+    src = np.zeros(N, np.uint16)
+    shift = np.zeros(max(nblocks, 1), np.int32)
+    if gpu_lib().ldpc_gpu_t2_maps(C.byref(plp), nblocks, perm.ctypes.data, src.ctypes.data, shift.ctypes.data):
+        raise ValueError("ldpc_gpu_t2_maps failed")
+    return perm, src, shift[:nblocks]
 
 
 def t2_demap(dec, plp, nblocks, d_cells, d_csi, d_llr, d_err, d_sig):

@@ -196,3 +196,60 @@ def s2x_high_order(dec, n_normal=16384, n_short=4096, margin_db=0.2, max_iter=50
         for k, pls in enumerate(lst):
             groups.append((pls, base + (1 if k < extra else 0)))
     return GpuBatch(dec, groups, margin_db, max_iter, seed=seed)
+
+
+T2_RATES = ["1_2", "3_5", "2_3", "3_4", "4_5", "5_6"]
+T2_ROT_DEG = [29.0, 16.8, 8.6, 3.576334375]
+
+
+def t2_blocks(plp, nblocks, snr_db, seed=3, distinct=16):
+    """Synthetic DVB-T2 FEC blocks of one PLP (ldpcgpu.T2Plp) as the
+    receiver's time deinterleaver hands them over: `distinct` random
+    BBFRAMEs through the BCH and LDPC encoders, mapped to Gray QAM
+    (codeword bit n to the (cell, plane) of ldpc_gpu_t2_maps' bitsrc;
+    bit 0 of an axis its sign, bit j > 0 the sign of |level| - 2^(nb-j)),
+    rotated with the Q part sent one cell later, cell-interleaved with the
+    block's shift, through AWGN at snr_db (Es/N0), channel state 1; then
+    repeated to nblocks.  Returns (cells (nblocks, ncells, 2) float32, csi
+    (nblocks, ncells) float32, info bytes (nblocks, kbch / 8))."""
+    N, K, kbch, nc = ldpcgpu.t2_geometry(plp)
+    sf = plp.fec_type == 0
+    rate = T2_RATES[plp.cod]
+    code = "2_3_t2" if (not sf and plp.cod == 2) else "3_5_t2" if (sf and plp.cod == 1) else rate
+    m = 2 * (plp.mod + 1)
+    nb = m // 2
+    perm, src, shift = ldpcgpu.t2_maps(plp, nblocks)
+    L = ldpcgpu.sim_lib()
+    rng = np.random.default_rng(seed)
+    norm = np.sqrt({2: 2.0, 4: 10.0, 6: 42.0, 8: 170.0}[m])
+    th = np.deg2rad(T2_ROT_DEG[plp.mod]) if plp.rotation else 0.0
+    d = min(distinct, nblocks)
+    cells = np.zeros((nblocks, nc, 2), np.float32)
+    info = np.zeros((nblocks, kbch // 8), np.uint8)
+    for b in range(d):
+        bits_in = rng.integers(0, 2, kbch, dtype=np.uint8)
+        bch = np.zeros(K, np.uint8)
+        cw = np.zeros(N, np.uint8)
+        if L.sim_bch_encode(sf, rate.encode(), bits_in.ctypes.data, bch.ctypes.data) or \
+                L.sim_ldpc_encode(sf, code.encode(), bch.ctypes.data, cw.ctypes.data):
+            raise RuntimeError(f"encoder failed for {rate}")
+        planes = np.zeros(nc * m, np.uint8)
+        planes[src.astype(np.int64)] = cw
+        planes = planes.reshape(nc, m)
+        axis = []
+        for a in (0, 1):
+            bits = planes[:, a::2]
+            u = np.where(bits[:, nb - 1] == 0, 1.0, -1.0)
+            for j in range(nb - 1, 0, -1):
+                u = (u + 2.0 ** (nb - j)) * np.where(bits[:, j - 1] == 0, 1.0, -1.0)
+            axis.append(u)
+        x = (axis[0] + 1j * axis[1]) / norm * np.exp(1j * th)
+        y = x.real + 1j * (np.roll(x.imag, 1) if plp.rotation else x.imag)
+        s = 10 ** (-snr_db / 20) / np.sqrt(2)
+        y = y + s * (rng.standard_normal(nc) + 1j * rng.standard_normal(nc))
+        for bb in range(b, nblocks, d):
+            pos = (perm.astype(np.int64) + shift[bb]) % nc
+            cells[bb, pos, 0] = y.real
+            cells[bb, pos, 1] = y.imag
+            info[bb] = np.packbits(bits_in)
+    return cells, np.ones((nblocks, nc), np.float32), info

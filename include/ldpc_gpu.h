@@ -190,6 +190,15 @@ typedef struct {
  * (the checks of setup_plp_fec, dvbt2.c:1963). */
 int ldpc_gpu_t2_geometry(const ldpc_gpu_t2_plp* plp, int* nldpc, int* kldpc, int* kbch, int* ncells);
 
+/* The maps the front end applies, computed on the host (no device call):
+ * perm (ncells entries) the cell interleaver's L0 (setup_plp_fec,
+ * dvbt2.c:2052-2081), bitsrc (nldpc entries) for codeword bit n the soft
+ * value cell * m + plane it takes (plane 2j: bit j of I, 2j + 1: bit j of
+ * Q; dvbt2.c:1969-2050, :2364-2385), shift (nblocks entries) each block's
+ * cyclic shift (dvbt2.c:2230-2254).  Null pointers are skipped.  For
+ * transmit-side models and checks. */
+int ldpc_gpu_t2_maps(const ldpc_gpu_t2_plp* plp, int nblocks, int32_t* perm, uint16_t* bitsrc, int32_t* shift);
+
 /*
  * Front end (dvbt2.c:2213-2386): nblocks blocks of ncells time-
  * deinterleaved cells, d_cells as (re, im) float pairs (8-byte aligned)

@@ -140,3 +140,23 @@ def test_noisy_job_matches_reference():
 def test_unsupported_plp():
     for cod, mod, ft in [(6, 0, 1), (0, 4, 1), (0, 0, 2), (-1, 0, 0)]:
         assert H.oracle().oracle_t2_geometry(cod, mod, ft, None, None, None, 0) == -1
+
+
+@pytest.mark.parametrize("til", [(0, 0), (3, 0), (5, 1), (7, 0)])
+def test_library_maps_match_oracle(til):
+    """The maps the HIP library's kernel uses (ldpc_gpu_t2_maps, computed
+    on the host: no device needed) equal the oracle's, for every PLP
+    configuration."""
+    for fec_type, mod, cod in CONFIGS:
+        plp = H.ldpcgpu.T2Plp(cod, mod, fec_type, 1, til[0], til[1])
+        perm, src, shift = H.ldpcgpu.t2_maps(plp, 23)
+        N, nc, _ = H.t2_geometry(cod, mod, fec_type)
+        assert H.ldpcgpu.t2_geometry(plp)[0] == N and H.ldpcgpu.t2_geometry(plp)[3] == nc
+        op = np.zeros(nc, np.int32)
+        H.oracle().oracle_t2_cellperm(nc, op.ctypes.data)
+        osrc = np.zeros(N, np.uint16)
+        H.oracle().oracle_t2_bitsrc(cod, mod, fec_type, osrc.ctypes.data)
+        osh = np.zeros(23, np.int32)
+        H.oracle().oracle_t2_shift(23, til[0], til[1], nc, osh.ctypes.data)
+        assert np.array_equal(perm, op) and np.array_equal(src, osrc) and np.array_equal(shift, osh), \
+            (fec_type, mod, cod)
