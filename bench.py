@@ -399,7 +399,7 @@ def t2_record(dec, reps=5, cpu=True):
     for name, cod, mod, ft, snr, nblocks in T2_PLPS:
         plp = ldpcgpu.T2Plp(cod, mod, ft, 1, T2_TI_BLOCKS, 0)
         N, K, kbch, nc = ldpcgpu.t2_geometry(plp)
-        cells, csi, info = workloads.t2_blocks(plp, nblocks, snr)
+        cells, csi, info, _ = workloads.t2_blocks(plp, nblocks, snr)
         kb = K // 8
         sizes = dict(cells=cells.nbytes, csi=csi.nbytes, llr=nblocks * N, cw=nblocks * kb, it=4 * nblocks,
                      syn=nblocks, bch=4 * nblocks, err=4 * nblocks, sig=4 * nblocks)

@@ -27,9 +27,12 @@ ldpc_gpu_decode_batch (or the learned decoder):
 
 A frame's latency is its batch's completion time minus its arrival time.
 
-DVB-T2 QAM constellations are not among the DVB-S2/S2X MODCODs of the
-GPU channel; the T2 streams' LLRs come from its +-1 per bit channel at an
-Es/N0 placing them at the same distance above their code's threshold.
+The DVB-S2/S2X streams' LLRs come from the GPU channel (MODCOD
+constellation, AWGN, max-log demapper).  The DVB-T2 streams' blocks are
+rotated-QAM cells of encoded frames (workloads.t2_blocks, AWGN) put
+through the GPU T2 front end (ldpc_gpu_t2_demap: cell and bit
+deinterleaving, soft values), 0.3 dB above the lowest Es/N0 at which 16
+such blocks all decode.
 """
 import argparse
 import json
@@ -48,10 +51,13 @@ import ldpcgpu  # noqa: E402
 STREAMS = [
     ("s2_8psk_3_4", 8, 0x38, 0, "3_4", 27.5e6 / 22194, None),
     ("s2x_qpsk_11_20", 4, 0x88, 0, "11_20", 52e6 / 33282, None),
-    ("t2_256qam_3_5", 2, -1, 0, "3_5", 900.0, 3.0),
-    ("t2_64qam_2_3_short", 2, -1, 1, "2_3", 2700.0, 3.6),
+    ("t2_256qam_3_5", 2, -1, 0, "3_5", 900.0, 18.7),
+    ("t2_64qam_2_3_short", 2, -1, 1, "2_3", 2700.0, 14.1),
 ]
 MARGIN_DB = {0x38: 0.3, 0x88: 1.0}  # above the MODCOD's threshold (S2: QEF, S2X: Shannon + 1 dB)
+# the T2 streams' PLPs: (cod, mod, fec_type), rotated, type-0 time
+# interleaver
+T2_PLP = {"t2_256qam_3_5": (1, 3, 1), "t2_64qam_2_3_short": (2, 2, 0)}
 
 
 def arrivals(duration, load_scale, seed):
@@ -103,6 +109,30 @@ def schedule(arr, decode, max_batch=4096, slots=None):
     return done, batches
 
 
+def t2_frames(dec, dv, name, esn0, sel, seed, fec, buf, llr_off, refs, rows_all, k):
+    """The soft values of a T2 stream's blocks (positions sel of the trace)
+    through the GPU T2 front end, placed in their slots; the reference bits
+    of each into refs / rows_all."""
+    import workloads
+
+    cod, mod, ft = T2_PLP[name]
+    nb = len(sel)
+    # at most 1000 blocks per TI-block, as in a T2 frame (the cell
+    # interleaver's shifts bound it)
+    plp = ldpcgpu.T2Plp(cod, mod, ft, 1, min(255, max(8, -(-nb // 1000))), 0)
+    cells, csi, _, bchcw = workloads.t2_blocks(plp, nb, esn0, seed=seed, distinct=64)
+    refs[k] = bchcw
+    rows_all[sel] = np.arange(nb) % len(bchcw)
+    d_cells, d_csi = dv.put(cells), dv.put(csi)
+    tmp = dv.empty(nb * fec.nldpc)
+    d_err, d_sig = dv.empty(4 * nb), dv.empty(4 * nb)
+    ldpcgpu.t2_demap(dec, plp, nb, d_cells, d_csi, tmp, d_err, d_sig)
+    dec.synchronize()
+    host = dv.get(tmp, (nb, fec.nldpc), np.int8)
+    for q, f in enumerate(sel):
+        buf[llr_off[f]:llr_off[f] + fec.nldpc] = host[q]
+
+
 def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, max_batch=4096, slots=8192,
         lanes=None):
     """Generate the trace on the GPU and decode it.  lanes=None: the
@@ -130,6 +160,9 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
         d_out = dv.empty(int(((KL + 7) // 8).sum()))
         d_it = dv.empty(4 * n)
         d_ok = dv.empty(n)
+        d_it_mix = dv.empty(4 * n)
+        d_ok_mix = dv.empty(n)
+        mixed, mixed_at = [], [0]
         rng = np.random.default_rng(seed + 1)
         buf = np.zeros(int(N.sum()), np.int8)
         refs = [None] * len(STREAMS)
@@ -137,6 +170,9 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
         for k, (pls, short, rate, fec, code, e) in enumerate(info):
             sel = np.nonzero(kind == k)[0]
             if len(sel) == 0:
+                continue
+            if STREAMS[k][0] in T2_PLP:
+                t2_frames(dec, dv, STREAMS[k][0], e, sel, seed + 10 + k, fec, buf, llr_off, refs, rows_all, k)
                 continue
             pool = codeword_pool(short, rate, 64, seed + 10 + k)
             refs[k] = np.packbits(pool[:, :fec.kldpc], axis=1)
@@ -160,7 +196,8 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
         frames["group"] = -1
         frames["llr_off"] = llr_off
         frames["out_off"] = out_off
-        frames["max_iter"] = 50
+        # the T2 streams take the reference's T2 FEC thread's cap (dvbt2.c:127)
+        frames["max_iter"] = [30 if STREAMS[k][0] in T2_PLP else 50 for k in kind]
         decs = lanes if lanes else [dec]
         if learned:
             for dk in decs:
@@ -173,16 +210,41 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
                         dk.set_nms(code, t[0], t[1])
         learned_frames = frames.copy()
         learned_frames["max_iter"] = 30
+        # the T2 streams keep the plain decoder in learned runs too: the
+        # reference's T2 FEC thread has none, and the tables were trained on
+        # DVB-S2 frames
+        t2_pos = np.array([STREAMS[k][0] in T2_PLP for k in kind])
 
         def enqueue(dk, pos):
             # trace positions pos (ascending); iteration counts and flags
-            # land at those positions
+            # land at those positions.  In learned runs the T2 frames of the
+            # range go to the plain decoder in a call of their own (the
+            # live scheduler's queues never mix them)
             i0, i1 = int(pos[0]), int(pos[-1]) + 1
             assert i1 - i0 == len(pos)
-            if learned:
-                dk.decode_batch_nms(learned_frames[i0:i1], d_llr, d_out, d_it + 4 * i0, d_ok + i0, 50)
-            else:
+            if not learned:
                 dk.decode_batch(frames[i0:i1], d_llr, d_out, d_it + 4 * i0, d_ok + i0)
+                return
+            for plain in (True, False):
+                idx = np.nonzero(t2_pos[i0:i1] == plain)[0] + i0
+                if len(idx) == 0:
+                    continue
+                if len(idx) == i1 - i0:
+                    tab, it_p, ok_p = (frames if plain else learned_frames)[i0:i1], d_it + 4 * i0, d_ok + i0
+                else:
+                    # a mixed range: the subset's results go to scratch and
+                    # are moved to their positions after the run
+                    tab = (frames if plain else learned_frames)[idx]
+                    j = mixed_at[0]
+                    if j + len(idx) > n:
+                        raise RuntimeError("streaming: mixed-range scratch exhausted")
+                    mixed_at[0] += len(idx)
+                    mixed.append((idx, j))
+                    it_p, ok_p = d_it_mix + 4 * j, d_ok_mix + j
+                if plain:
+                    dk.decode_batch(tab, d_llr, d_out, it_p, ok_p)
+                else:
+                    dk.decode_batch_nms(tab, d_llr, d_out, it_p, ok_p, 50)
 
         def decode(idx):
             t0 = time.perf_counter()
@@ -191,18 +253,22 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
             return time.perf_counter() - t0
 
         if lanes:
-            # one queue per launch kind (degree class x frame size), each
-            # context launching one kind at a time on its own stream
+            # one queue per launch kind (degree class x frame size, T2 or
+            # not), each context launching one kind at a time on its own
+            # stream
             kkey = []
             for k, inf in enumerate(info):
                 p = ldpcgpu.code_params(inf[4])
-                kkey.append((p["degree_class"], p["n"]))
+                # T2 frames queue apart: their non-converging blocks run
+                # to the T2 cap and would hold back S2 frames of the kind
+                kkey.append((p["degree_class"], p["n"], STREAMS[k][0] in T2_PLP))
             key = np.array([kkey.index(kkey[k]) for k in kind])
             # frames are renumbered so that each queue's frames (in arrival
             # order) sit at consecutive positions
             perm = np.concatenate([np.nonzero(key == u)[0] for u in np.unique(key)])
             frames[:] = frames[perm]
             learned_frames[:] = learned_frames[perm]
+            t2_pos[:] = t2_pos[perm]
             qstart = np.concatenate([[0], np.cumsum([int((key == u).sum()) for u in np.unique(key)])])
             groups = [perm[qstart[g]:qstart[g + 1]] for g in range(len(qstart) - 1)]
             for dk in decs:
@@ -221,14 +287,22 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
             back = np.empty(n, np.int64)
             back[perm] = np.arange(n)
         else:
-            # warm the kernels
+            # warm the kernels (the warm-up's mixed-range results are
+            # dropped: the run decodes those frames again)
             decode(np.arange(min(64, n)))
+            mixed.clear()
+            mixed_at[0] = 0
             done, batches = schedule(arr, decode, max_batch, slots)
             back = np.arange(n)
         out = np.zeros(int(((KL + 7) // 8).sum()), np.uint8)
         it = np.zeros(n, np.int32)
         dec.download(out, d_out)
         dec.download(it, d_it)
+        if mixed:
+            it_mix = np.zeros(mixed_at[0], np.int32)
+            dec.download(it_mix, d_it_mix)
+            for idx, j in mixed:
+                it[idx] = it_mix[j:j + len(idx)]
         it = it[back]
     finally:
         dv.free()
@@ -246,6 +320,7 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
         ref = refs[k][rows_all[f]]
         o = out[out_off[f]:out_off[f] + len(ref)]
         errors += int(not np.array_equal(o, ref))
+    errors_bch = bch_errors(dec, info, kind, out, out_off, refs, rows_all)
     busy = sum(b[-1] for b in batches)
     span = max(float(done.max()), duration)
     return {
@@ -259,8 +334,42 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
         "avg_batch": round(n / len(batches), 1), "avg_iterations": round(float(it.mean()), 3),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(lat.max()), 3), "mean": round(float(lat.mean()), 3)},
+        "latency_ms_by_stream": {
+            STREAMS[k][0]: {"p50": round(float(np.percentile(lat[kind == k], 50)), 3),
+                            "p99": round(float(np.percentile(lat[kind == k], 99)), 3)}
+            for k in range(len(STREAMS)) if (kind == k).any()},
         "frame_errors_ldpc": errors,
+        "frame_errors_after_bch": errors_bch,
     }
+
+
+def bch_errors(dec, info, kind, out, out_off, refs, rows_all):
+    """Frames not equal to their BCH codeword after BCH decoding (on the
+    GPU, one call per stream kind, after the run: not in the latencies).
+    The T2 streams need it: at 256-QAM the reference's soft-value scale
+    leaves some frames with residual LDPC errors that BCH corrects, as the
+    reference's FEC thread does (dvbt2.c:2390-2395)."""
+    errors = 0
+    for k, (pls, short, rate, fec, code, e) in enumerate(info):
+        sel = np.nonzero(kind == k)[0]
+        if len(sel) == 0:
+            continue
+        kb = fec.kldpc // 8
+        cw = np.stack([out[out_off[f]:out_off[f] + kb] for f in sel])
+        d_cw, d_res = dec.malloc(cw.nbytes), dec.malloc(4 * len(sel))
+        try:
+            dec.upload(d_cw, cw)
+            ldpcgpu.bch_decode(dec, short, rate, len(sel), d_cw, kb, d_res)
+            dec.synchronize()
+            res = np.zeros(len(sel), np.int32)
+            dec.download(cw, d_cw)
+            dec.download(res, d_res)
+        finally:
+            dec.free(d_cw)
+            dec.free(d_res)
+        for q, f in enumerate(sel):
+            errors += int(res[q] < 0 or not np.array_equal(cw[q], refs[k][rows_all[f]]))
+    return errors
 
 
 def run_live(arr, decs, enqueue, groups=None, max_batch=4096, slots=None, poll_s=20e-6):

@@ -210,8 +210,10 @@ def t2_blocks(plp, nblocks, snr_db, seed=3, distinct=16):
     bit 0 of an axis its sign, bit j > 0 the sign of |level| - 2^(nb-j)),
     rotated with the Q part sent one cell later, cell-interleaved with the
     block's shift, through AWGN at snr_db (Es/N0), channel state 1; then
-    repeated to nblocks.  Returns (cells (nblocks, ncells, 2) float32, csi
-    (nblocks, ncells) float32, info bytes (nblocks, kbch / 8))."""
+    repeated to nblocks (block b is frame b % distinct).  Returns (cells
+    (nblocks, ncells, 2) float32, csi (nblocks, ncells) float32, info bytes
+    (nblocks, kbch / 8), BCH codewords (distinct, kldpc / 8) packed: what
+    the LDPC decoder returns for a decoded block)."""
     N, K, kbch, nc = ldpcgpu.t2_geometry(plp)
     sf = plp.fec_type == 0
     rate = T2_RATES[plp.cod]
@@ -226,6 +228,7 @@ def t2_blocks(plp, nblocks, snr_db, seed=3, distinct=16):
     d = min(distinct, nblocks)
     cells = np.zeros((nblocks, nc, 2), np.float32)
     info = np.zeros((nblocks, kbch // 8), np.uint8)
+    bchcw = np.zeros((d, K // 8), np.uint8)
     for b in range(d):
         bits_in = rng.integers(0, 2, kbch, dtype=np.uint8)
         bch = np.zeros(K, np.uint8)
@@ -233,6 +236,7 @@ def t2_blocks(plp, nblocks, snr_db, seed=3, distinct=16):
         if L.sim_bch_encode(sf, rate.encode(), bits_in.ctypes.data, bch.ctypes.data) or \
                 L.sim_ldpc_encode(sf, code.encode(), bch.ctypes.data, cw.ctypes.data):
             raise RuntimeError(f"encoder failed for {rate}")
+        bchcw[b] = np.packbits(bch)
         planes = np.zeros(nc * m, np.uint8)
         planes[src.astype(np.int64)] = cw
         planes = planes.reshape(nc, m)
@@ -252,4 +256,4 @@ def t2_blocks(plp, nblocks, snr_db, seed=3, distinct=16):
             cells[bb, pos, 0] = y.real
             cells[bb, pos, 1] = y.imag
             info[bb] = np.packbits(bits_in)
-    return cells, np.ones((nblocks, nc), np.float32), info
+    return cells, np.ones((nblocks, nc), np.float32), info, bchcw

@@ -2,7 +2,8 @@
 decode every frame of a short trace, and decoding is deterministic, so the
 live scheduler with two contexts (frames renumbered into per-launch-kind
 queues) must report the same iteration counts and frame errors as the
-one-batch-at-a-time scheduler."""
+one-batch-at-a-time scheduler.  The trace's DVB-T2 blocks come through the
+GPU T2 front end; every frame is error-free after BCH."""
 import os
 import sys
 
@@ -30,5 +31,7 @@ def test_live_matches_simulated(learned):
         dec.close()
     assert sim["frames"] == live["frames"] > 100
     assert live["avg_iterations"] == sim["avg_iterations"]
-    assert live["frame_errors_ldpc"] == sim["frame_errors_ldpc"] == 0
+    # the T2 256-QAM frames leave residual LDPC errors that BCH corrects
+    assert live["frame_errors_ldpc"] == sim["frame_errors_ldpc"]
+    assert live["frame_errors_after_bch"] == sim["frame_errors_after_bch"] == 0
     assert live["latency_ms"]["max"] < 1000 and live["latency_ms"]["p50"] > 0
