@@ -177,3 +177,45 @@ def test_t2_rejects_bad_calls(dec):
     # 8100-cell interleaver's counter gives for 16384 blocks
     with pytest.raises(RuntimeError):
         ldpcgpu.t2_demap(dec, _plp(0, 3, 1, 0, (1, 0)), 16384, 8, 8, 8, 8, 8)
+
+
+@pytest.mark.parametrize("nblocks,til,mod,ft", [(1, (0, 0), 3, 1), (1, (4, 0), 0, 0), (5, (8, 0), 2, 1),
+                                                 (7, (3, 1), 3, 0), (2, (1, 0), 1, 0)])
+def test_t2_demap_edge_shapes(dec, nblocks, til, mod, ft):
+    """One block; fewer blocks than TI-blocks (some TI-blocks empty); a
+    type-1 interleaver (no shift); 256-QAM short frames (one cell per
+    demultiplexer word): equal to the oracle."""
+    print(f"# t2_demap edge nblocks {nblocks} til {til} mod {mod} fec_type {ft}", flush=True)
+    cod = 4
+    N, nc, _ = H.t2_geometry(cod, mod, ft)
+    rng = np.random.default_rng(nblocks * 31 + mod)
+    cells, csi = H.t2_random_cells(mod, nc, nblocks, rng, snr_db=9.0)
+    for rot in (0, 1):
+        llr, err, sig = gpu_demap(dec, cod, mod, ft, rot, til, cells, csi)
+        want, werr, wsig = H.oracle_t2_demap(cod, mod, ft, rot, til, cells, csi)
+        assert np.array_equal(llr, want), rot
+        assert np.array_equal(err.view(np.int32), werr.view(np.int32)), rot
+        assert np.array_equal(sig.view(np.int32), wsig.view(np.int32)), rot
+
+
+def test_t2_saturated_and_zero_state(dec):
+    """A block scaled 40x (its own noise estimate grows with it: every soft
+    value rounds to 0), a 30 dB block with exact zeros (soft values
+    saturate at +-127), and channel state 0 (the 1e-12 guard keeps the
+    weights finite): equal to the oracle."""
+    cod, mod, ft = 3, 2, 0
+    N, nc, _ = H.t2_geometry(cod, mod, ft)
+    rng = np.random.default_rng(8)
+    cells, csi = H.t2_random_cells(mod, nc, 3, rng, snr_db=30.0)
+    cells[0] *= 40.0
+    cells[1, :100] = 0.0
+    csi[2, ::7] = 0.0
+    for rot in (0, 1):
+        llr, err, sig = gpu_demap(dec, cod, mod, ft, rot, (0, 0), cells, csi)
+        want, werr, wsig = H.oracle_t2_demap(cod, mod, ft, rot, (0, 0), cells, csi)
+        assert np.array_equal(llr, want), rot
+        assert np.array_equal(err.view(np.int32), werr.view(np.int32)), rot
+        assert np.array_equal(sig.view(np.int32), wsig.view(np.int32)), rot
+        assert (llr[0] == 0).all()
+        if rot == 0:
+            assert (np.abs(llr[1].astype(int)) == 127).mean() > 0.3

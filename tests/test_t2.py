@@ -160,3 +160,24 @@ def test_library_maps_match_oracle(til):
         H.oracle().oracle_t2_shift(23, til[0], til[1], nc, osh.ctypes.data)
         assert np.array_equal(perm, op) and np.array_equal(src, osrc) and np.array_equal(shift, osh), \
             (fec_type, mod, cod)
+
+
+def test_saturated_and_zero_state_match_reference():
+    """The same extreme cells as the GPU test: saturating soft values, exact
+    zeros, channel state 0 -- the oracle equals the reference's build."""
+    if not REFS:
+        pytest.skip("reference T2 build (oracle/_ref) not present")
+    cod, mod, ft = 3, 2, 0
+    N, nc, _ = H.t2_geometry(cod, mod, ft)
+    rng = np.random.default_rng(8)
+    cells, csi = H.t2_random_cells(mod, nc, 3, rng, snr_db=30.0)
+    cells[0] *= 40.0
+    cells[1, :100] = 0.0
+    csi[2, ::7] = 0.0
+    for rot in (0, 1):
+        llr, err, sig = H.oracle_t2_demap(cod, mod, ft, rot, (0, 0), cells, csi)
+        for name, (L, lanes) in REFS.items():
+            _, _, rerr, rsig, rllr = H.ref_t2_job(L, cod, mod, ft, rot, (0, 0), cells, csi, 1, with_llr=True)
+            assert np.array_equal(rllr, llr), (name, rot)
+            assert np.array_equal(rerr.view(np.int32), err.view(np.int32)), (name, rot)
+            assert np.array_equal(rsig.view(np.int32), sig.view(np.int32)), (name, rot)
