@@ -295,11 +295,77 @@ __device__ __forceinline__ int8_t t2_llr(float v)
     return (int8_t)(int)rintf(v);
 }
 
+/* MER terms of one chunk of cells into LDS, then the ordered sums: thread
+ * l < 8 adds err lane l, thread 8 + l sig lane l, in cell order. */
+__device__ __forceinline__ void chunk_sums(const T2Args& a, int base, const float* wr, const float* wi,
+                                           const float* kr, const float* ki, float* tE, float* tS, float& acc)
+{
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < T2_U; u++) {
+        const int j = tid + u * T2_BLOCK;
+        float te = 0.0f, ts = 0.0f;
+        if (base + j < a.ncells) {
+            const float awr = fabsf(wr[u]), awi = fabsf(wi[u]);
+            const float ar = 2.0f * truncf(fminf(awr * 0.5f, a.amax)) + 1.0f;
+            const float ai = 2.0f * truncf(fminf(awi * 0.5f, a.amax)) + 1.0f;
+            const float er = awr - ar, ei = awi - ai;
+            te = fmaf(er * er, kr[u], ei * ei * ki[u]);
+            ts = a.s2 * (kr[u] + ki[u]);
+        }
+        tE[j] = te;
+        tS[j] = ts;
+    }
+    __syncthreads();
+    if (tid < 16) {
+        const float* t = (tid < 8 ? tE : tS) + (tid & 7);
+        const int cnt = min(T2_CHUNK, a.ncells - base);
+#pragma unroll 8
+        for (int q = 0; 8 * q < cnt; q++)
+            acc = acc + t[8 * q];
+    }
+    __syncthreads();
+}
+
+/* Soft values of one chunk of cells by (cell, plane) (dvbt2.c:2333-2359). */
 template <int M>
-__global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
+__device__ __forceinline__ void chunk_soft(const T2Args& a, int base, const float* wr, const float* wi,
+                                           const float* kr, const float* ki, float k, int8_t* soft)
 {
 #pragma clang fp contract(off)
     constexpr int NB = M / 2;
+#pragma unroll
+    for (int u = 0; u < T2_U; u++) {
+        const int i = base + threadIdx.x + u * T2_BLOCK;
+        if (i >= a.ncells)
+            continue;
+        float ur = wr[u], ui = wi[u];
+        int8_t v[M];
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            if (j) {
+                const float t = (float)(1 << (NB - j));
+                ur = fabsf(ur) - t;
+                ui = fabsf(ui) - t;
+            }
+            v[2 * j] = t2_llr((ur * kr[u]) * k);
+            v[2 * j + 1] = t2_llr((ui * ki[u]) * k);
+        }
+#pragma unroll
+        for (int p = 0; p < M; p++)
+            soft[i * M + p] = v[p];
+    }
+}
+
+/* M bits per cell.  NCH > 0: the block has at most NCH chunks of cells
+ * and each thread keeps its derotated points in registers between the MER
+ * pass and the soft-value pass; NCH = 0: the second pass gathers the cells
+ * again. */
+template <int M, int NCH>
+__global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
+{
+#pragma clang fp contract(off)
     __shared__ __attribute__((aligned(16))) int8_t soft[T2_LDS];
     __shared__ float lane_sum[16];
     __shared__ int s_P, s_wcount[T2_BLOCK / 64];
@@ -311,36 +377,24 @@ __global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
     const float2* in = a.cells + (size_t)b * a.ncells;
     const float* ic = a.csi + (size_t)b * a.ncells;
 
-    /* MER sums in the reference's order: thread l < 8 holds err lane l,
-     * thread 8 + l sig lane l */
+    /* MER sums in the reference's order */
     float acc = 0.0f;
-    for (int base = 0; base < a.ncells; base += T2_CHUNK) {
-        float wr[T2_U], wi[T2_U], kr[T2_U], ki[T2_U];
-        t2_points(a, in, ic, P, base + tid, wr, wi, kr, ki);
+    constexpr int NR = NCH > 0 ? NCH : 1;
+    float cwr[NR][T2_U], cwi[NR][T2_U], ckr[NR][T2_U], cki[NR][T2_U];
+    if constexpr (NCH > 0) {
 #pragma unroll
-        for (int u = 0; u < T2_U; u++) {
-            const int j = tid + u * T2_BLOCK;
-            float te = 0.0f, ts = 0.0f;
-            if (base + j < a.ncells) {
-                const float awr = fabsf(wr[u]), awi = fabsf(wi[u]);
-                const float ar = 2.0f * truncf(fminf(awr * 0.5f, a.amax)) + 1.0f;
-                const float ai = 2.0f * truncf(fminf(awi * 0.5f, a.amax)) + 1.0f;
-                const float er = awr - ar, ei = awi - ai;
-                te = fmaf(er * er, kr[u], ei * ei * ki[u]);
-                ts = a.s2 * (kr[u] + ki[u]);
+        for (int c = 0; c < NCH; c++) {
+            const int base = c * T2_CHUNK;
+            if (base < a.ncells) {
+                t2_points(a, in, ic, P, base + tid, cwr[c], cwi[c], ckr[c], cki[c]);
+                chunk_sums(a, base, cwr[c], cwi[c], ckr[c], cki[c], tE, tS, acc);
             }
-            tE[j] = te;
-            tS[j] = ts;
         }
-        __syncthreads();
-        if (tid < 16) {
-            const float* t = (tid < 8 ? tE : tS) + (tid & 7);
-            const int cnt = min(T2_CHUNK, a.ncells - base);
-#pragma unroll 8
-            for (int q = 0; 8 * q < cnt; q++)
-                acc = acc + t[8 * q];
+    } else {
+        for (int base = 0; base < a.ncells; base += T2_CHUNK) {
+            t2_points(a, in, ic, P, base + tid, cwr[0], cwi[0], ckr[0], cki[0]);
+            chunk_sums(a, base, cwr[0], cwi[0], ckr[0], cki[0], tE, tS, acc);
         }
-        __syncthreads();
     }
     if (tid < 16)
         lane_sum[tid] = acc;
@@ -359,30 +413,16 @@ __global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
     __syncthreads();
     const float k = s_k;
 
-    /* soft values by (cell, plane) (dvbt2.c:2333-2359) */
-    for (int base = 0; base < a.ncells; base += T2_CHUNK) {
-        float wr[T2_U], wi[T2_U], kr[T2_U], ki[T2_U];
-        t2_points(a, in, ic, P, base + tid, wr, wi, kr, ki);
+    /* soft values */
+    if constexpr (NCH > 0) {
 #pragma unroll
-        for (int u = 0; u < T2_U; u++) {
-            const int i = base + tid + u * T2_BLOCK;
-            if (i >= a.ncells)
-                continue;
-            float ur = wr[u], ui = wi[u];
-            int8_t v[M];
-#pragma unroll
-            for (int j = 0; j < NB; j++) {
-                if (j) {
-                    const float t = (float)(1 << (NB - j));
-                    ur = fabsf(ur) - t;
-                    ui = fabsf(ui) - t;
-                }
-                v[2 * j] = t2_llr((ur * kr[u]) * k);
-                v[2 * j + 1] = t2_llr((ui * ki[u]) * k);
-            }
-#pragma unroll
-            for (int p = 0; p < M; p++)
-                soft[i * M + p] = v[p];
+        for (int c = 0; c < NCH; c++)
+            if (c * T2_CHUNK < a.ncells)
+                chunk_soft<M>(a, c * T2_CHUNK, cwr[c], cwi[c], ckr[c], cki[c], k, soft);
+    } else {
+        for (int base = 0; base < a.ncells; base += T2_CHUNK) {
+            t2_points(a, in, ic, P, base + tid, cwr[0], cwi[0], ckr[0], cki[0]);
+            chunk_soft<M>(a, base, cwr[0], cwi[0], ckr[0], cki[0], k, soft);
         }
     }
     __syncthreads();
@@ -557,18 +597,35 @@ extern "C" int ldpc_gpu_t2_demap(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, 
     a.s2 = a.norm * a.norm * 0.5f;
     a.amax = (float)((1 << (g.m / 2 - 1)) - 1);
     const hipStream_t st = (hipStream_t)ldpc_gpu_stream(ctx);
+    /* points kept in registers when the block has at most 4 chunks */
+    const int nch = (g.ncells + T2_CHUNK - 1) / T2_CHUNK;
+    const dim3 grid(nblocks), blk(T2_BLOCK);
     switch (g.m) {
     case 2:
-        hipLaunchKernelGGL(t2_demap_kernel<2>, dim3(nblocks), dim3(T2_BLOCK), 0, st, a);
+        if (nch <= 4)
+            hipLaunchKernelGGL((t2_demap_kernel<2, 4>), grid, blk, 0, st, a);
+        else
+            hipLaunchKernelGGL((t2_demap_kernel<2, 0>), grid, blk, 0, st, a);
         break;
     case 4:
-        hipLaunchKernelGGL(t2_demap_kernel<4>, dim3(nblocks), dim3(T2_BLOCK), 0, st, a);
+        if (nch <= 2)
+            hipLaunchKernelGGL((t2_demap_kernel<4, 2>), grid, blk, 0, st, a);
+        else
+            hipLaunchKernelGGL((t2_demap_kernel<4, 0>), grid, blk, 0, st, a);
         break;
     case 6:
-        hipLaunchKernelGGL(t2_demap_kernel<6>, dim3(nblocks), dim3(T2_BLOCK), 0, st, a);
+        if (nch <= 2)
+            hipLaunchKernelGGL((t2_demap_kernel<6, 2>), grid, blk, 0, st, a);
+        else
+            hipLaunchKernelGGL((t2_demap_kernel<6, 0>), grid, blk, 0, st, a);
         break;
     default:
-        hipLaunchKernelGGL(t2_demap_kernel<8>, dim3(nblocks), dim3(T2_BLOCK), 0, st, a);
+        if (nch <= 1)
+            hipLaunchKernelGGL((t2_demap_kernel<8, 1>), grid, blk, 0, st, a);
+        else if (nch <= 4)
+            hipLaunchKernelGGL((t2_demap_kernel<8, 4>), grid, blk, 0, st, a);
+        else
+            hipLaunchKernelGGL((t2_demap_kernel<8, 0>), grid, blk, 0, st, a);
         break;
     }
     if (hipGetLastError() != hipSuccess)
