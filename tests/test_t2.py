@@ -12,11 +12,16 @@ libdvbrx/src/dvbt2.c:1953-2082 and :2188-2400) pinned three ways:
   job and the reference's.
 """
 import itertools
+import json
+import os
+import sys
 
 import numpy as np
 import pytest
 
 import helpers as H
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
 
 CONFIGS = list(itertools.product([1, 0], range(4), range(6)))  # fec_type, mod, cod
 REFS = H.t2_ref_builds()
@@ -181,3 +186,35 @@ def test_saturated_and_zero_state_match_reference():
             assert np.array_equal(rllr, llr), (name, rot)
             assert np.array_equal(rerr.view(np.int32), err.view(np.int32)), (name, rot)
             assert np.array_equal(rsig.view(np.int32), sig.view(np.int32)), (name, rot)
+
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "t2_golden.json")))
+
+
+@pytest.mark.parametrize("k", range(len(GOLDEN["front_end"])))
+def test_oracle_front_end_matches_golden(k):
+    """The reference's soft values and MER sums, as digests written by
+    tests/golden/make_t2_golden.py from its own decode_job: the oracle
+    reproduces them from the same seeded cells (no reference build
+    needed)."""
+    import make_t2_golden as G
+
+    g = GOLDEN["front_end"][k]
+    cod, mod, ft, rot, tl, tt, nb, snr, seed = g["case"]
+    cells, csi = G.front_inputs(g["case"])
+    llr, err, sig = H.oracle_t2_demap(cod, mod, ft, rot, (tl, tt), cells, csi)
+    assert G.sha(llr) == g["llr_sha256"] and G.sha(err) == g["err_sha256"] and G.sha(sig) == g["sig_sha256"]
+
+
+@pytest.mark.parametrize("k", range(len(GOLDEN["jobs"])))
+def test_oracle_job_matches_golden(k):
+    import make_t2_golden as G
+
+    g = GOLDEN["jobs"][k]
+    cod, mod, ft, rot, tl, tt, nb, snr, seed = g["case"]
+    info, cells, csi = G.job_inputs(g["case"])
+    cw, res, err, sig = H.oracle_t2_job(cod, mod, ft, rot, (tl, tt), cells, csi, g["max_iter"], g["lanes"])
+    kb = info.shape[1]
+    assert G.sha(np.ascontiguousarray(cw[:, :kb])) == g["bb_sha256"]
+    assert (res >= 0).astype(int).tolist() == g["ok"]
+    assert G.sha(err) == g["err_sha256"] and G.sha(sig) == g["sig_sha256"]
