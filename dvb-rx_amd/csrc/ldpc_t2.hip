@@ -666,3 +666,74 @@ extern "C" int ldpc_gpu_t2_decode(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp,
         return rc;
     return ldpc_gpu_bch_decode(ctx, plp->fec_type == 0, RATES[plp->cod], nblocks, d_cw, stride, d_bch);
 }
+
+extern "C" int ldpc_gpu_t2_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, int nblocks, const float* cells,
+                                       const float* csi, int max_iter, int lanes, uint8_t* bb, uint8_t* ok,
+                                       float* err, float* sig)
+{
+    if (!ctx)
+        return LDPC_GPU_EINVAL;
+    PlpGeom g;
+    if (plp_geom(plp, g))
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_decode_host: unsupported PLP (cod 0..5, mod 0..3, fec_type 0/1)");
+    if (nblocks < 0 || (nblocks > 0 && (!cells || !csi || !bb || !ok || !err || !sig)))
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_decode_host: bad arguments");
+    if (nblocks == 0)
+        return LDPC_GPU_OK;
+    const int dev = lg_ctx_device(ctx);
+    if (hipSetDevice(dev) != hipSuccess)
+        return lg_ctx_fail(ctx, LDPC_GPU_EDEVICE, "t2_decode_host: hipSetDevice");
+    const size_t nc = (size_t)nblocks * g.ncells, kb = (size_t)g.K / 8, bbb = (size_t)g.kbch / 8;
+    float *d_cells = nullptr, *d_csi = nullptr, *d_err = nullptr, *d_sig = nullptr;
+    int8_t* d_llr = nullptr;
+    uint8_t *d_cw = nullptr, *d_syn = nullptr;
+    int32_t *d_it = nullptr, *d_bch = nullptr;
+    int rc = LDPC_GPU_OK;
+    if (hipMalloc((void**)&d_cells, 8 * nc) != hipSuccess || hipMalloc((void**)&d_csi, 4 * nc) != hipSuccess ||
+        hipMalloc((void**)&d_llr, (size_t)nblocks * g.N) != hipSuccess ||
+        hipMalloc((void**)&d_cw, (size_t)nblocks * kb) != hipSuccess ||
+        hipMalloc((void**)&d_it, sizeof(int32_t) * nblocks) != hipSuccess ||
+        hipMalloc((void**)&d_syn, nblocks) != hipSuccess ||
+        hipMalloc((void**)&d_bch, sizeof(int32_t) * nblocks) != hipSuccess ||
+        hipMalloc((void**)&d_err, sizeof(float) * nblocks) != hipSuccess ||
+        hipMalloc((void**)&d_sig, sizeof(float) * nblocks) != hipSuccess)
+        rc = lg_ctx_fail(ctx, LDPC_GPU_ENOMEM, "t2_decode_host: device allocation failed");
+    const hipStream_t st = (hipStream_t)ldpc_gpu_stream(ctx);
+    if (!rc && (hipMemcpyAsync(d_cells, cells, 8 * nc, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(d_csi, csi, 4 * nc, hipMemcpyHostToDevice, st) != hipSuccess))
+        rc = lg_ctx_fail(ctx, LDPC_GPU_EDEVICE, "t2_decode_host: upload failed");
+    if (!rc)
+        rc = ldpc_gpu_t2_decode(ctx, plp, nblocks, d_cells, d_csi, max_iter, lanes, d_llr, d_cw, d_it, d_syn, d_bch,
+                                d_err, d_sig);
+    std::vector<uint8_t> cw;
+    std::vector<int32_t> res;
+    if (!rc) {
+        cw.resize((size_t)nblocks * kb);
+        res.resize(nblocks);
+        if (hipMemcpyAsync(cw.data(), d_cw, cw.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(res.data(), d_bch, sizeof(int32_t) * nblocks, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(err, d_err, sizeof(float) * nblocks, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(sig, d_sig, sizeof(float) * nblocks, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = lg_ctx_fail(ctx, LDPC_GPU_EDEVICE, "t2_decode_host: download failed");
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d_cells);
+    (void)hipFree(d_csi);
+    (void)hipFree(d_llr);
+    (void)hipFree(d_cw);
+    (void)hipFree(d_it);
+    (void)hipFree(d_syn);
+    (void)hipFree(d_bch);
+    (void)hipFree(d_err);
+    (void)hipFree(d_sig);
+    if (rc)
+        return rc;
+    /* the BBFRAME is the first kbch bits of the corrected BCH codeword
+     * (dvbt2.c:2393-2394) */
+    for (int b = 0; b < nblocks; b++) {
+        std::memcpy(bb + (size_t)b * bbb, cw.data() + (size_t)b * kb, bbb);
+        ok[b] = res[b] >= 0;
+    }
+    return LDPC_GPU_OK;
+}

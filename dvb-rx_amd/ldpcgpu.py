@@ -81,6 +81,7 @@ def gpu_lib():
         L.ldpc_gpu_t2_geometry.argtypes = [vp, ip, ip, ip, ip]
         L.ldpc_gpu_t2_demap.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
         L.ldpc_gpu_t2_maps.argtypes = [vp, i32, vp, vp, vp]
+        L.ldpc_gpu_t2_decode_host.argtypes = [vp, vp, i32, vp, vp, i32, i32, vp, vp, vp, vp]
         L.ldpc_gpu_t2_decode.argtypes = [vp, vp, i32, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp]
         L.ldpc_gpu_decode_batch.argtypes = [vp, vp, i32, vp, vp, vp, vp]
         L.ldpc_gpu_decode_host.argtypes = [vp, vp, i32, vp, sz, vp, sz, vp, vp]
@@ -506,6 +507,26 @@ def t2_decode(dec, plp, nblocks, d_cells, d_csi, max_iter, lanes, d_llr, d_cw, d
     """ldpc_gpu_t2_decode: front end, LDPC and BCH of nblocks FEC blocks."""
     dec._check(gpu_lib().ldpc_gpu_t2_decode(dec.ctx, C.byref(plp), nblocks, d_cells, d_csi, max_iter, lanes, d_llr,
                                             d_cw, d_iters, d_synd, d_bch, d_err, d_sig), "t2_decode")
+
+
+def t2_decode_host(dec, plp, cells, csi, max_iter=30, lanes=0):
+    """ldpc_gpu_t2_decode_host on host arrays: cells (nblocks, ncells, 2)
+    float32, csi (nblocks, ncells) float32 -> (BBFRAMEs (nblocks, kbch/8),
+    ok, err, sig)."""
+    N, K, kbch, nc = t2_geometry(plp)
+    cells = np.ascontiguousarray(cells, np.float32)
+    csi = np.ascontiguousarray(csi, np.float32)
+    nb = cells.shape[0]
+    if cells.shape != (nb, nc, 2) or csi.shape != (nb, nc):
+        raise ValueError("cells must be (nblocks, ncells, 2) and csi (nblocks, ncells)")
+    bb = np.zeros((nb, kbch // 8), np.uint8)
+    ok = np.zeros(nb, np.uint8)
+    err = np.zeros(nb, np.float32)
+    sig = np.zeros(nb, np.float32)
+    dec._check(gpu_lib().ldpc_gpu_t2_decode_host(dec.ctx, C.byref(plp), nb, cells.ctypes.data, csi.ctypes.data,
+                                                 max_iter, lanes, bb.ctypes.data, ok.ctypes.data, err.ctypes.data,
+                                                 sig.ctypes.data), "t2_decode_host")
+    return bb, ok, err, sig
 
 
 class DrxLdpc:

@@ -228,6 +228,18 @@ int ldpc_gpu_t2_decode(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, int nblock
                        const float* d_csi, int max_iter, int lanes, int8_t* d_llr, uint8_t* d_cw, int32_t* d_iters,
                        uint8_t* d_synd, int32_t* d_bch, float* d_err, float* d_sig);
 
+/*
+ * decode_job's own interface on host memory (dvbt2.c:2188-2400): the job's
+ * cells and channel state in (job->buf, job->csi: nblocks x ncells (re, im)
+ * floats and floats), its results out: bb nblocks x kbch / 8 BBFRAME bytes
+ * (job->bb), ok[b] = 1 when BCH decoded block b (job->ok), err / sig the
+ * MER sums (job->err, job->sig).  Copies in, runs ldpc_gpu_t2_decode,
+ * copies out; synchronous.  Device buffers are allocated per call.
+ */
+int ldpc_gpu_t2_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, int nblocks, const float* cells,
+                            const float* csi, int max_iter, int lanes, uint8_t* bb, uint8_t* ok, float* err,
+                            float* sig);
+
 /* Kernel choice for frames with their own early stop (group -1):
  * LDPC_GPU_KERNEL_AUTO runs two frames of one code per workgroup in packed
  * 16-bit arithmetic; LDPC_GPU_KERNEL_SINGLE one frame per workgroup (the

@@ -256,3 +256,25 @@ def test_t2_job_matches_golden(dec, k):
     assert G.sha(np.ascontiguousarray(cw[:, :kb])) == g["bb_sha256"]
     assert (res >= 0).astype(int).tolist() == g["ok"]
     assert G.sha(err) == g["err_sha256"] and G.sha(sig) == g["sig_sha256"]
+
+
+@pytest.mark.parametrize("k", range(len(GOLDEN["jobs"])))
+def test_t2_decode_host_matches_golden(dec, k):
+    """ldpc_gpu_t2_decode_host, decode_job's own interface on host memory:
+    BBFRAMEs, ok flags and MER sums equal to the reference's (golden
+    digests, and the reference build where present)."""
+    import make_t2_golden as G
+
+    g = GOLDEN["jobs"][k]
+    cod, mod, ft, rot, tl, tt, nb, snr, seed = g["case"]
+    print(f"# t2_decode_host golden {g['case']}", flush=True)
+    info, cells, csi = G.job_inputs(g["case"])
+    plp = _plp(cod, mod, ft, rot, (tl, tt))
+    bb, ok, err, sig = ldpcgpu.t2_decode_host(dec, plp, cells, csi, g["max_iter"], g["lanes"])
+    assert G.sha(bb) == g["bb_sha256"] and ok.tolist() == g["ok"]
+    assert G.sha(err) == g["err_sha256"] and G.sha(sig) == g["sig_sha256"]
+    for name, (L, lanes) in H.t2_ref_builds().items():
+        bb2, ok2, err2, sig2 = ldpcgpu.t2_decode_host(dec, plp, cells, csi, 30, lanes)
+        rbb, rok, rerr, rsig, _ = H.ref_t2_job(L, cod, mod, ft, rot, (tl, tt), cells, csi, 30)
+        assert np.array_equal(bb2, rbb) and np.array_equal(ok2, rok), name
+        assert np.array_equal(err2.view(np.int32), rerr.view(np.int32)), name
