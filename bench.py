@@ -436,6 +436,12 @@ def t2_record(dec, reps=5, cpu=True):
             dec.download(it, d["it"])
             good = int(((res >= 0) & (cw[:, :kbch // 8] == info).all(1)).sum())
             t_job32 = timed(job, 32)
+            # the same job on host buffers (decode_job's interface): PCIe and
+            # per-call allocation included
+            t0 = time.perf_counter()
+            hbb, hok, _, _ = ldpcgpu.t2_decode_host(dec, plp, cells, csi, 30, 0)
+            t_host = time.perf_counter() - t0
+            host_ok = bool(np.array_equal(hbb, info) and hok.all())
             alg = nblocks * (12 * nc + N)
             rec = {"plp": name, "es_n0_db": snr, "blocks": nblocks, "cells_per_block": nc,
                    "front_end_ms": round(t_front * 1e3, 4), "front_end_gcells_s": round(nblocks * nc / t_front / 1e9, 2),
@@ -443,6 +449,9 @@ def t2_record(dec, reps=5, cpu=True):
                    "job_ms": round(t_job * 1e3, 3), "job_info_gbit_s": round(nblocks * kbch / t_job / 1e9, 3),
                    "job_ms_lanes32": round(t_job32 * 1e3, 3),
                    "job_info_gbit_s_lanes32": round(nblocks * kbch / t_job32 / 1e9, 3),
+                   "job_host_buffers_ms": round(t_host * 1e3, 2),
+                   "job_host_buffers_info_gbit_s": round(nblocks * kbch / t_host / 1e9, 3),
+                   "job_host_buffers_all_decoded": host_ok,
                    "avg_iterations": round(float(it.mean()), 2), "blocks_decoded": good}
             if cpu:
                 rec["cpu_baseline"] = t2_cpu_job(plp, cells, csi, 30)
