@@ -18,9 +18,10 @@
  *     their count is the result.
  *
  * Work layout: one workgroup of 256 threads per codeword.  Odd syndromes:
- * each thread takes a contiguous run of bits and, for each set bit of
- * degree p, adds alpha^(j p) for the t odd j (exponents reduced mod
- * 2^m - 1 by folding), then a workgroup XOR reduction.  Codewords with a
+ * each thread takes bytes strided by 256; a byte of lowest degree p0 adds
+ * alpha^(j p0) times its own polynomial at alpha^j, whose log comes from a
+ * 256-entry table per j in LDS (exponents reduced mod 2^m - 1 by folding),
+ * then a workgroup XOR reduction.  Codewords with a
  * non-zero syndrome (few, after LDPC decoding) run Berlekamp-Massey on one
  * thread and the Chien search on all, positions strided by 256 with
  * incremental exponents.  The field tables (exp, log) are read from
@@ -43,15 +44,20 @@ constexpr int BCH_NT = 256;
 constexpr int BCH_MAX_T = 12;
 constexpr int BCH_MAX_DEV = 64;
 
-/* exp (2q entries) and log (q + 1) of GF(2^m), m = 14 or 16, per device */
+/* exp (2q entries) and log (q + 1) of GF(2^m), m = 14 or 16, per device;
+ * and the byte table: for odd j = 2k + 1 (k < 12) and byte value c, the
+ * log of sum over set bits b of c of alpha^(j b), i.e. the byte's
+ * polynomial at alpha^j (never 0 for c != 0: alpha^j's minimal polynomial
+ * has degree m > 7); 0xFFFF for c = 0 */
 struct GfTables {
     uint16_t* d_exp[2] = { nullptr, nullptr };
     uint16_t* d_log[2] = { nullptr, nullptr };
+    uint16_t* d_blog[2] = { nullptr, nullptr };
 };
 static GfTables g_gf[BCH_MAX_DEV];
 static std::mutex g_gf_lock;
 
-static int gf_tables(int dev, int shortf, const uint16_t** exp, const uint16_t** log)
+static int gf_tables(int dev, int shortf, const uint16_t** exp, const uint16_t** log, const uint16_t** blog)
 {
     std::lock_guard<std::mutex> lk(g_gf_lock);
     GfTables& g = g_gf[dev];
@@ -67,24 +73,42 @@ static int gf_tables(int dev, int shortf, const uint16_t** exp, const uint16_t**
             if (x >> m)
                 x ^= prim;
         }
-        uint16_t *de = nullptr, *dl = nullptr;
+        std::vector<uint16_t> bl((size_t)BCH_MAX_T * 256, 0xFFFF);
+        for (int k = 0; k < BCH_MAX_T; k++)
+            for (int c = 1; c < 256; c++) {
+                unsigned v = 0;
+                for (int b = 0; b < 8; b++)
+                    if (c >> b & 1)
+                        v ^= e[((2 * k + 1) * b) % q];
+                bl[(size_t)k * 256 + c] = v ? l[v] : 0xFFFF;
+            }
+        uint16_t *de = nullptr, *dl = nullptr, *db = nullptr;
         if (hipMalloc((void**)&de, sizeof(uint16_t) * e.size()) != hipSuccess)
             return LDPC_GPU_ENOMEM;
         if (hipMalloc((void**)&dl, sizeof(uint16_t) * l.size()) != hipSuccess) {
             (void)hipFree(de);
             return LDPC_GPU_ENOMEM;
         }
-        if (hipMemcpy(de, e.data(), sizeof(uint16_t) * e.size(), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(dl, l.data(), sizeof(uint16_t) * l.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMalloc((void**)&db, sizeof(uint16_t) * bl.size()) != hipSuccess) {
             (void)hipFree(de);
             (void)hipFree(dl);
+            return LDPC_GPU_ENOMEM;
+        }
+        if (hipMemcpy(de, e.data(), sizeof(uint16_t) * e.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dl, l.data(), sizeof(uint16_t) * l.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(db, bl.data(), sizeof(uint16_t) * bl.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(de);
+            (void)hipFree(dl);
+            (void)hipFree(db);
             return LDPC_GPU_EDEVICE;
         }
         g.d_exp[shortf] = de;
         g.d_log[shortf] = dl;
+        g.d_blog[shortf] = db;
     }
     *exp = g.d_exp[shortf];
     *log = g.d_log[shortf];
+    *blog = g.d_blog[shortf];
     return LDPC_GPU_OK;
 }
 
@@ -94,6 +118,7 @@ struct BchArgs {
     int32_t* result;
     const uint16_t* exp;
     const uint16_t* log;
+    const uint16_t* blog; /* byte table, BCH_MAX_T x 256 */
     int nbch, t, m, q;
 };
 
@@ -125,27 +150,30 @@ __global__ void __launch_bounds__(BCH_NT) bch_kernel(BchArgs a)
         s_nroots = 0;
     __syncthreads();
 
-    /* ---- odd syndromes: runs of 32 bits (4 bytes) per thread, in turn ---- */
+    /* ---- odd syndromes: bytes strided over the threads.  The byte at
+     * index B holds degrees p0 .. p0 + 7 (p0 = nbch - 8 - 8 B, its LSB the
+     * lowest), so it adds alpha^(j p0) times its polynomial at alpha^j:
+     * one exponent sum with the byte table's log ---- */
+    __shared__ uint16_t s_blog[BCH_MAX_T * 256];
+    for (int i = tid; i < t * 256; i += BCH_NT)
+        s_blog[i] = a.blog[i];
+    __syncthreads();
     uint32_t syn[BCH_MAX_T];
 #pragma unroll
     for (int k = 0; k < BCH_MAX_T; k++)
         syn[k] = 0;
-    const int nwords = nbch / 32, tail = nbch - 32 * nwords;
-    for (int w = tid; w <= nwords; w += BCH_NT) {
-        const int nb = w < nwords ? 32 : tail;
-        if (nb == 0)
+    const int nbytes = nbch / 8;
+    for (int B = tid; B < nbytes; B += BCH_NT) {
+        const uint32_t c = cw[B];
+        if (!c)
             continue;
-        uint32_t v = 0;
-        for (int b = 0; b < nb / 8; b++)
-            v |= (uint32_t)cw[4 * w + b] << (24 - 8 * b);
-        while (v) {
-            const int lead = __builtin_clz(v);
-            v &= ~(0x80000000u >> lead);
-            const uint32_t p = (uint32_t)(nbch - 1 - (32 * w + lead));
+        const uint32_t p0 = (uint32_t)(nbch - 8 - 8 * B);
 #pragma unroll
-            for (int k = 0; k < BCH_MAX_T; k++)
-                if (k < t)
-                    syn[k] ^= a.exp[mod_q((uint64_t)(2 * k + 1) * p, m, q)];
+        for (int k = 0; k < BCH_MAX_T; k++) {
+            if (k < t) {
+                const uint32_t lg = s_blog[k * 256 + c];
+                syn[k] ^= a.exp[mod_q((uint64_t)(2 * k + 1) * p0 + lg, m, q)];
+            }
         }
     }
     /* XOR over the workgroup: within each wave, then one LDS atomic per
@@ -292,6 +320,8 @@ extern "C" int ldpc_gpu_bch_decode(ldpc_gpu_ctx* ctx, int short_frame, const cha
         return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "bch: bad arguments (stride below the codeword's bytes?)");
     if (nframes == 0)
         return LDPC_GPU_OK;
+    if (nbch % 8)
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "bch: codeword not a whole number of bytes");
     const int dev = lg_ctx_device(ctx);
     if (dev < 0 || dev >= BCH_MAX_DEV)
         return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "bch: device index");
@@ -299,7 +329,7 @@ extern "C" int ldpc_gpu_bch_decode(ldpc_gpu_ctx* ctx, int short_frame, const cha
         return lg_ctx_fail(ctx, LDPC_GPU_EDEVICE, "bch: hipSetDevice");
     const int shortf = short_frame ? 1 : 0;
     BchArgs a;
-    int rc = gf_tables(dev, shortf, &a.exp, &a.log);
+    int rc = gf_tables(dev, shortf, &a.exp, &a.log, &a.blog);
     if (rc)
         return lg_ctx_fail(ctx, rc, "bch: field tables");
     a.cw = d_cw;
