@@ -34,6 +34,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -525,7 +527,9 @@ __device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, 
         }
     };
     fetch(0);
-    for (int i = 0; i < q; i++) {
+    /* one layer with its degree D fixed */
+    auto layer = [&](auto dtag, int i) {
+        constexpr int D = decltype(dtag)::value;
         uint32_t mw[WM];
 #pragma unroll
         for (int w = 0; w < WM; w++)
@@ -540,28 +544,53 @@ __device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, 
          * node (0, 0) has no previous parity, one edge less */
         lg_pk ps = pk_dup(16), pb = pk_dup(16);
         if constexpr (NMS) {
-            const int dc = lay_deg(ly) + 2 - (i == 0 && j == 0);
+            const int dc = D + 2 - (i == 0 && j == 0);
             ps = lg_pk{ (short)rows.S0[dc], (short)rows.S1[dc] };
             pb = lg_pk{ (short)rows.B0[dc], (short)rows.B1[dc] };
         }
-#define X(D)                                                                                      \
-    case D:                                                                                       \
-        if (nlev > 1)                                                                             \
-            pk_layer_t<D, true, NMS>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so, ps, pb);      \
-        else                                                                                      \
-            pk_layer_t<D, false, NMS>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so, ps, pb);     \
-        break;
-        if constexpr (CLS == 0) {
-            switch (lay_deg(ly)) {
-                LOW_DEGS(X)
-            }
-        } else {
-            switch (lay_deg(ly)) {
-                HIGH_DEGS(X)
-            }
-        }
-#undef X
+        if (nlev > 1)
+            pk_layer_t<D, true, NMS>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so, ps, pb);
+        else
+            pk_layer_t<D, false, NMS>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so, ps, pb);
         __syncthreads();
+    };
+    /* a layer of no degree of the class has no node to run */
+    auto skip = [&](int i) {
+        fetch(i + 1);
+        __syncthreads();
+    };
+    if constexpr (CLS == 0) {
+        /* low degrees: layers in runs of one degree, dispatched once per
+         * run (the per-layer dispatch is a large share of these layers'
+         * scalar work) */
+        for (int i = 0; i < q;) {
+#define X(D)                                                  \
+    case D:                                                   \
+        do                                                    \
+            layer(std::integral_constant<int, D>{}, i++);     \
+        while (i < q && lay_deg(lay[i]) == D);                \
+        break;
+            switch (lay_deg(lay[i])) {
+                LOW_DEGS(X)
+            default:
+                skip(i++);
+            }
+#undef X
+        }
+    } else {
+        /* high degrees: one dispatch per layer (runs would spill) */
+        for (int i = 0; i < q; i++) {
+#define X(D)                                          \
+    case D:                                           \
+        layer(std::integral_constant<int, D>{}, i);   \
+        break;
+            switch (lay_deg(lay[i])) {
+                HIGH_DEGS(X)
+            default:
+                skip(i);
+            }
+#undef X
+        }
     }
 }
 
