@@ -346,9 +346,9 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
 def bch_errors(dec, info, kind, out, out_off, refs, rows_all):
     """Frames not equal to their BCH codeword after BCH decoding (on the
     GPU, one call per stream kind, after the run: not in the latencies).
-    The T2 streams need it: at 256-QAM the reference's soft-value scale
-    leaves some frames with residual LDPC errors that BCH corrects, as the
-    reference's FEC thread does (dvbt2.c:2390-2395)."""
+    The T2 streams need it: the 256-QAM blocks leave some frames with
+    residual LDPC errors (in the reference's chain as here), which BCH
+    corrects, as the reference's FEC thread does (dvbt2.c:2390-2395)."""
     errors = 0
     for k, (pls, short, rate, fec, code, e) in enumerate(info):
         sel = np.nonzero(kind == k)[0]
