@@ -108,3 +108,20 @@ def test_live_scheduler_queues_by_kind():
     assert [b[0] for b in batches] == [g for g, _, _ in seen]
     assert sorted(i for g, s, e in seen for i in groups[g][s:e]) == list(range(6))
     assert seen[0][0] == 0
+
+
+def test_in_order_delivery():
+    """A frame leaves only after every earlier frame of its own stream;
+    other streams do not hold it back."""
+    done = np.array([5.0, 1.0, 2.0, 7.0, 3.0])
+    sid = np.array([0, 0, 1, 1, 0])
+    assert np.allclose(streaming.in_order(done, sid), [5.0, 5.0, 2.0, 7.0, 5.0])
+
+
+def test_arrivals_streams():
+    arr, kind = streaming.arrivals(0.5, 1.0, 9)
+    arr2, kind2, sid = streaming.arrivals(0.5, 1.0, 9, streams=True)
+    assert np.array_equal(arr, arr2) and np.array_equal(kind, kind2)
+    assert set(np.unique(sid).tolist()) == set(range(sum(s[1] for s in streaming.STREAMS)))
+    for s in np.unique(sid):
+        assert len(set(kind[sid == s].tolist())) == 1
