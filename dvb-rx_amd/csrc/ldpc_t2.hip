@@ -46,6 +46,7 @@ constexpr int T2_BLOCK = 256;
 constexpr int T2_CHUNK = 2048; /* cells per pass of the ordered sums */
 constexpr int T2_LDS = 64800;  /* one block's soft values (the sums' terms alias it) */
 static_assert(2 * T2_CHUNK * 4 <= T2_LDS, "sum terms fit the soft-value area");
+static_assert(T2_LDS + 2 * T2_CHUNK * 4 + 256 <= 160 * 1024 / 2, "two workgroups per CU with staged cells");
 
 /* Column twist (EN 302 755 table 8; libdvbrx/src/dvbt2.c:188-193),
  * 16/64/256-QAM, normal then short frames. */
@@ -136,7 +137,7 @@ bool build_bitsrc(int cod, int mod, bool sf, int N, int K, std::vector<uint16_t>
 }
 
 struct T2Maps {
-    int32_t* d_perm[2][4] = {};     /* by (fec_type, mod): ncells = N / m */
+    int32_t* d_perm[2][4] = {};     /* by (fec_type, mod): L0, then its inverse (2 ncells) */
     uint16_t* d_src[2][6][4] = {};  /* by (fec_type, cod, mod) */
 };
 T2Maps g_maps[T2_MAX_DEV];
@@ -151,8 +152,12 @@ int get_maps(int dev, int cod, int mod, int fec_type, int N, int K, const int32_
     uint16_t*& ds = t.d_src[fec_type][cod][mod];
     if (!dp) {
         std::vector<int32_t> h;
-        if (!build_cellperm(N / (2 * (mod + 1)), h))
+        const int nc = N / (2 * (mod + 1));
+        if (!build_cellperm(nc, h))
             return LDPC_GPU_EINVAL;
+        h.resize(2 * (size_t)nc);
+        for (int i = 0; i < nc; i++)
+            h[nc + h[i]] = i;
         int32_t* p = nullptr;
         if (hipMalloc((void**)&p, sizeof(int32_t) * h.size()) != hipSuccess)
             return LDPC_GPU_ENOMEM;
@@ -184,6 +189,7 @@ struct T2Args {
     const float2* cells;
     const float* csi;
     const int32_t* perm;
+    const int32_t* inv; /* inverse of perm */
     const uint16_t* src;
     int8_t* llr;
     float* err;
@@ -289,6 +295,51 @@ __device__ __forceinline__ void t2_points(const T2Args& a, const float2* in, con
     }
 }
 
+/* The same points with the block's cells already in LDS in deinterleaved
+ * order (stage[i] = cell i); only the channel state is gathered. */
+__device__ __forceinline__ void t2_points_staged(const T2Args& a, const float2* stage, const float* ic, int P,
+                                                 int i0, float* wr, float* wi, float* kr, float* ki)
+{
+#pragma clang fp contract(off)
+    int s[T2_U], s1[T2_U], ii[T2_U], ii1[T2_U];
+#pragma unroll
+    for (int u = 0; u < T2_U; u++) {
+        const int i = min(i0 + u * T2_BLOCK, a.ncells - 1);
+        ii[u] = i;
+        ii1[u] = i + 1 == a.ncells ? 0 : i + 1;
+        s[u] = a.perm[i];
+        s1[u] = a.perm[ii1[u]];
+    }
+    float c1[T2_U], c2[T2_U];
+#pragma unroll
+    for (int u = 0; u < T2_U; u++) {
+        int t = s[u] + P;
+        t -= t >= a.ncells ? a.ncells : 0;
+        c1[u] = ic[t];
+        if (a.rot) {
+            int t1 = s1[u] + P;
+            t1 -= t1 >= a.ncells ? a.ncells : 0;
+            c2[u] = ic[t1];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < T2_U; u++) {
+        const float2 x = stage[ii[u]];
+        const float e1 = c1[u] + 1e-12f;
+        const float q = a.rot ? stage[ii1[u]].y : x.y;
+        const float e2 = a.rot ? c2[u] + 1e-12f : e1;
+        const float p = x.x;
+        wr[u] = fmaf(a.ct, p, a.sn * q) * a.norm;
+        wi[u] = fmaf(a.ct, q, -(a.sn * p)) * a.norm;
+        kr[u] = ki[u] = e1;
+        if (a.rot) {
+            const float c12 = e1 * e2;
+            kr[u] = c12 / fmaf(a.sn2, e1, a.ct2 * e2);
+            ki[u] = c12 / fmaf(a.ct2, e1, a.sn2 * e2);
+        }
+    }
+}
+
 __device__ __forceinline__ int8_t t2_llr(float v)
 {
     v = fminf(fmaxf(v, -127.0f), 127.0f);
@@ -361,21 +412,40 @@ __device__ __forceinline__ void chunk_soft(const T2Args& a, int base, const floa
 /* M bits per cell.  NCH > 0: the block has at most NCH chunks of cells
  * and each thread keeps its derotated points in registers between the MER
  * pass and the soft-value pass; NCH = 0: the second pass gathers the cells
- * again. */
-template <int M, int NCH>
+ * again.  STAGE (with NCH > 0): the block's cells are first read in input
+ * order into LDS, so the deinterleaving gathers hit LDS (for blocks whose
+ * input would not stay in the L2 between chunks). */
+template <int M, int NCH, bool STAGE>
 __global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
 {
 #pragma clang fp contract(off)
     __shared__ __attribute__((aligned(16))) int8_t soft[T2_LDS];
+    /* STAGE: the sum terms get their own LDS, and soft[] first holds the
+     * block's cells in deinterleaved order (8 bytes a cell, at most
+     * T2_LDS); it turns into the soft values once every thread holds its
+     * points in registers */
+    static_assert(!STAGE || NCH > 0, "staging needs the points in registers");
+    __shared__ __attribute__((aligned(16))) float tsum[STAGE ? 2 * T2_CHUNK : 1];
     __shared__ float lane_sum[16];
     __shared__ int s_P, s_wcount[T2_BLOCK / 64];
     __shared__ float s_k;
-    float* tE = (float*)soft;
+    float* tE = STAGE ? tsum : (float*)soft;
     float* tS = tE + T2_CHUNK;
     const int tid = threadIdx.x, b = blockIdx.x;
     const int P = block_shift(b, a.nblocks, a.nti, a.ncells, a.nd, &s_P, s_wcount);
     const float2* in = a.cells + (size_t)b * a.ncells;
     const float* ic = a.csi + (size_t)b * a.ncells;
+    float2* stage = (float2*)soft;
+    if constexpr (STAGE) {
+        /* input order, coalesced: input cell x is deinterleaved cell
+         * inv[(x - P) mod ncells] */
+        for (int x = tid; x < a.ncells; x += T2_BLOCK) {
+            int y = x - P;
+            y += y < 0 ? a.ncells : 0;
+            stage[a.inv[y]] = in[x];
+        }
+        __syncthreads();
+    }
 
     /* MER sums in the reference's order */
     float acc = 0.0f;
@@ -386,7 +456,10 @@ __global__ __launch_bounds__(T2_BLOCK) void t2_demap_kernel(T2Args a)
         for (int c = 0; c < NCH; c++) {
             const int base = c * T2_CHUNK;
             if (base < a.ncells) {
-                t2_points(a, in, ic, P, base + tid, cwr[c], cwi[c], ckr[c], cki[c]);
+                if constexpr (STAGE)
+                    t2_points_staged(a, stage, ic, P, base + tid, cwr[c], cwi[c], ckr[c], cki[c]);
+                else
+                    t2_points(a, in, ic, P, base + tid, cwr[c], cwi[c], ckr[c], cki[c]);
                 chunk_sums(a, base, cwr[c], cwi[c], ckr[c], cki[c], tE, tS, acc);
             }
         }
@@ -577,6 +650,7 @@ extern "C" int ldpc_gpu_t2_demap(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, 
     a.cells = (const float2*)d_cells;
     a.csi = d_csi;
     a.perm = perm;
+    a.inv = perm + g.ncells;
     a.src = src;
     a.llr = d_llr;
     a.err = d_err;
@@ -597,35 +671,43 @@ extern "C" int ldpc_gpu_t2_demap(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, 
     a.s2 = a.norm * a.norm * 0.5f;
     a.amax = (float)((1 << (g.m / 2 - 1)) - 1);
     const hipStream_t st = (hipStream_t)ldpc_gpu_stream(ctx);
-    /* points kept in registers when the block has at most 4 chunks */
+    /* points kept in registers when the block has at most 4 chunks; the
+     * cells staged in LDS as well when the block's input is over 64 KB
+     * (measured: 0.71 -> 0.58 ms for 4096 normal 256-QAM blocks, while
+     * 2700-cell blocks, whose input stays in the L2, got slower) */
     const int nch = (g.ncells + T2_CHUNK - 1) / T2_CHUNK;
+    const bool stage = nch <= 4 && (size_t)12 * g.ncells > 65536;
+    if (stage && (size_t)8 * g.ncells > (size_t)T2_LDS)
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "t2_demap: block too large to stage");
     const dim3 grid(nblocks), blk(T2_BLOCK);
     switch (g.m) {
     case 2:
-        if (nch <= 4)
-            hipLaunchKernelGGL((t2_demap_kernel<2, 4>), grid, blk, 0, st, a);
+        /* QPSK: 8100 cells (short, staged) or 32400 */
+        if (stage)
+            hipLaunchKernelGGL((t2_demap_kernel<2, 4, true>), grid, blk, 0, st, a);
         else
-            hipLaunchKernelGGL((t2_demap_kernel<2, 0>), grid, blk, 0, st, a);
+            hipLaunchKernelGGL((t2_demap_kernel<2, 0, false>), grid, blk, 0, st, a);
         break;
     case 4:
         if (nch <= 2)
-            hipLaunchKernelGGL((t2_demap_kernel<4, 2>), grid, blk, 0, st, a);
+            hipLaunchKernelGGL((t2_demap_kernel<4, 2, false>), grid, blk, 0, st, a);
         else
-            hipLaunchKernelGGL((t2_demap_kernel<4, 0>), grid, blk, 0, st, a);
+            hipLaunchKernelGGL((t2_demap_kernel<4, 0, false>), grid, blk, 0, st, a);
         break;
     case 6:
         if (nch <= 2)
-            hipLaunchKernelGGL((t2_demap_kernel<6, 2>), grid, blk, 0, st, a);
+            hipLaunchKernelGGL((t2_demap_kernel<6, 2, false>), grid, blk, 0, st, a);
         else
-            hipLaunchKernelGGL((t2_demap_kernel<6, 0>), grid, blk, 0, st, a);
+            hipLaunchKernelGGL((t2_demap_kernel<6, 0, false>), grid, blk, 0, st, a);
         break;
     default:
+        /* 256-QAM: 2025 cells (short) or 8100 (normal, staged) */
         if (nch <= 1)
-            hipLaunchKernelGGL((t2_demap_kernel<8, 1>), grid, blk, 0, st, a);
-        else if (nch <= 4)
-            hipLaunchKernelGGL((t2_demap_kernel<8, 4>), grid, blk, 0, st, a);
+            hipLaunchKernelGGL((t2_demap_kernel<8, 1, false>), grid, blk, 0, st, a);
+        else if (stage)
+            hipLaunchKernelGGL((t2_demap_kernel<8, 4, true>), grid, blk, 0, st, a);
         else
-            hipLaunchKernelGGL((t2_demap_kernel<8, 0>), grid, blk, 0, st, a);
+            hipLaunchKernelGGL((t2_demap_kernel<8, 0, false>), grid, blk, 0, st, a);
         break;
     }
     if (hipGetLastError() != hipSuccess)
