@@ -229,15 +229,17 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
                         dk.set_nms(code, t[0], t[1])
         learned_frames = frames.copy()
         learned_frames["max_iter"] = 30
-        # the T2 streams keep the plain decoder in learned runs too: the
-        # reference's T2 FEC thread has none, and the tables were trained on
-        # DVB-S2 frames
-        t2_pos = np.array([STREAMS[k][0] in T2_PLP for k in kind])
+        # in learned runs, streams whose code has no trained table keep the
+        # plain decoder, and so do the T2 streams (the reference's T2 FEC
+        # thread has none, and the tables were trained on DVB-S2 frames)
+        plain_kind = [STREAMS[k][0] in T2_PLP or (tables or {}).get(info[k][4]) is None
+                      for k in range(len(STREAMS))]
+        plain_pos = np.array([plain_kind[k] for k in kind])
 
         def enqueue(dk, pos):
             # trace positions pos (ascending); iteration counts and flags
-            # land at those positions.  In learned runs the T2 frames of the
-            # range go to the plain decoder in a call of their own (the
+            # land at those positions.  In learned runs the plain-decoder
+            # frames of the range go in a call of their own (the
             # live scheduler's queues never mix them)
             i0, i1 = int(pos[0]), int(pos[-1]) + 1
             assert i1 - i0 == len(pos)
@@ -245,7 +247,7 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
                 dk.decode_batch(frames[i0:i1], d_llr, d_out, d_it + 4 * i0, d_ok + i0)
                 return
             for plain in (True, False):
-                idx = np.nonzero(t2_pos[i0:i1] == plain)[0] + i0
+                idx = np.nonzero(plain_pos[i0:i1] == plain)[0] + i0
                 if len(idx) == 0:
                     continue
                 if len(idx) == i1 - i0:
@@ -280,14 +282,14 @@ def run(dec, duration=1.0, load_scale=1.0, learned=False, tables=None, seed=5, m
                 p = ldpcgpu.code_params(inf[4])
                 # T2 frames queue apart: their non-converging blocks run
                 # to the T2 cap and would hold back S2 frames of the kind
-                kkey.append((p["degree_class"], p["n"], STREAMS[k][0] in T2_PLP))
+                kkey.append((p["degree_class"], p["n"], STREAMS[k][0] in T2_PLP, learned and plain_kind[k]))
             key = np.array([kkey.index(kkey[k]) for k in kind])
             # frames are renumbered so that each queue's frames (in arrival
             # order) sit at consecutive positions
             perm = np.concatenate([np.nonzero(key == u)[0] for u in np.unique(key)])
             frames[:] = frames[perm]
             learned_frames[:] = learned_frames[perm]
-            t2_pos[:] = t2_pos[perm]
+            plain_pos[:] = plain_pos[perm]
             qstart = np.concatenate([[0], np.cumsum([int((key == u).sum()) for u in np.unique(key)])])
             groups = [perm[qstart[g]:qstart[g + 1]] for g in range(len(qstart) - 1)]
             for dk in decs:
