@@ -462,6 +462,57 @@ def t2_record(dec, reps=5, cpu=True):
     return out
 
 
+# codes of the per-code microbenchmark: (frame size, rate); the unit
+# channel's Es/N0 sits below each code's waterfall, so frames iterate
+MICRO_CODES = [(0, "1_4"), (0, "1_3"), (0, "1_2"), (0, "3_5"), (0, "2_3"), (0, "3_4"), (0, "5_6"), (0, "9_10"),
+               (1, "1_2"), (1, "3_4"), (1, "8_9"), (0, "154_180"), (0, "2_3_t2")]
+MICRO_ESN0 = {"1_4": -3.5, "1_3": -2.6, "1_2": -0.5, "3_5": 0.7, "2_3": 1.5, "3_4": 2.5, "5_6": 3.7, "9_10": 4.9,
+              "8_9": 4.7, "154_180": 5.0, "2_3_t2": 1.5}
+
+
+def microbench(dec, nframes=2048, iters=10, reps=3, codes=None):
+    """Decoder cost per code: nframes frames of one code on the unit
+    channel (+-1 per bit, AWGN) below the code's waterfall, max `iters`
+    iterations, per-frame early stop.  Reports microseconds per
+    frame-iteration over the whole GPU and picoseconds per edge (edge =
+    one of the code's links_total per iteration), plus the level steps and
+    the hand-over rounds of the kernel's schedule per iteration."""
+    out = []
+    for short, rate in (codes or MICRO_CODES):
+        fec = ldpcgpu.fec_lookup(short, rate)
+        code = ldpcgpu.code_lookup(short, rate)
+        prm = ldpcgpu.code_params(code)
+        specs = [dict(short_frame=short, rate=rate, esn0_db=MICRO_ESN0[rate], stream=f) for f in range(nframes)]
+        llr, _, layout = ldpcgpu.generate(specs, 99, with_ref=False)
+        fr = ldpcgpu.frames_for(layout, iters)
+        d_llr = dec.malloc(llr.nbytes)
+        d_out = dec.malloc(nframes * (fec.kldpc // 8 + 1))
+        d_it = dec.malloc(4 * nframes)
+        d_ok = dec.malloc(nframes)
+        dec.upload(d_llr, llr)
+        dec.decode_batch(fr, d_llr, d_out, d_it, d_ok)
+        dec.synchronize()
+        ms = []
+        for _ in range(reps):
+            dec.decode_batch(fr, d_llr, d_out, d_it, d_ok)
+            dec.synchronize()
+            ms.append(dec.last_kernel_ms())
+        it = np.zeros(nframes, np.int32)
+        dec.download(it, d_it)
+        for ptr in (d_llr, d_out, d_it, d_ok):
+            dec.free(ptr)
+        kms = float(np.median(ms))
+        fi = int(it.sum())
+        out.append({"code": f"{'short' if short else 'normal'} {rate}", "frames": nframes,
+                    "avg_iterations": round(fi / nframes, 3), "kernel_ms": round(kms, 3),
+                    "us_per_frame_iteration": round(kms * 1e3 / max(fi, 1), 4),
+                    "ps_per_edge": round(kms * 1e9 / max(fi * prm["links_total"], 1), 3),
+                    "layers": prm["q"], "level_steps": prm["level_steps"],
+                    "gbit_s": round(nframes * fec.kldpc / (kms * 1e-3) / 1e9, 3)})
+        print(f"# microbench {out[-1]}", file=sys.stderr, flush=True)
+    return out
+
+
 def config4(dec, nf=8192, db=0.2, reps=3):
     """Learned vs plain decoding on the two trained MODCODs (DVB-S2 normal
     3/5 8PSK and 2/3 16APSK) at Es/N0 = QEF + db: frames from pool
@@ -539,7 +590,11 @@ def main():
                     help="skip the side measurements (64-frame rate 1/2 batch, learned vs plain)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
-    ap.add_argument("--only", choices=["demapper", "bch", "learned", "t2"], default=None,
+    ap.add_argument("--serial", action="store_true",
+                    help="run the launches of one decode one after the other (per-kernel durations for profiling)")
+    ap.add_argument("--codes", default=None, metavar="S:RATE,...",
+                    help="microbench only these codes (frame size 0/1 : rate name), e.g. 0:3_4,1:1_2")
+    ap.add_argument("--only", choices=["demapper", "bch", "learned", "t2", "microbench"], default=None,
                     help="only run that side record (the demapper's or the BCH decoder's throughput, learned vs "
                          "plain decoding, DVB-T2 FEC blocks) and print it")
     args = ap.parse_args()
@@ -552,7 +607,14 @@ def main():
         return
     if args.only:
         dec = ldpcgpu.Decoder(0)
-        rec = {"demapper": demap_record, "bch": bch_record, "learned": config4, "t2": t2_record}[args.only](dec)
+        if args.serial:
+            dec.set_concurrent(False)
+        if args.only == "microbench" and args.codes:
+            codes = [(int(c.split(":")[0]), c.split(":")[1]) for c in args.codes.split(",")]
+            rec = microbench(dec, codes=codes)
+        else:
+            rec = {"demapper": demap_record, "bch": bch_record, "learned": config4, "t2": t2_record,
+                   "microbench": microbench}[args.only](dec)
         dec.close()
         print(json.dumps({args.only: rec}), flush=True)
         return
@@ -562,6 +624,8 @@ def main():
     print(f"# generated {batch.nframes} frames ({batch.llr.nbytes / 1e6:.0f} MB LLRs) in {time.time() - t0:.1f} s",
           file=sys.stderr, flush=True)
     dec = ldpcgpu.Decoder(0)
+    if args.serial:
+        dec.set_concurrent(False)
     d_llr = dec.malloc(batch.llr.nbytes)
     d_out = dec.malloc(batch.ref.nbytes)
     d_it = dec.malloc(4 * batch.nframes)
@@ -646,7 +710,7 @@ def main():
         e.append(dict(size=size, ms=ms, frames=nf, hbm=int(hbm_f[sel].sum()), lds=int(lds_f[sel].sum())))
     dom_di = max(inst, key=lambda d: sum(x["ms"] for x in inst[d]))
     dl = inst[dom_di]
-    kname = f"ldpc_pk_kernel<{0 if dom_di == 12 else 1}>"
+    kname = f"ldpc_pk_kernel<{dom_di}>"
     dom_ms = float(np.mean([x["ms"] for x in dl]))
     hbm_bytes = float(np.mean([x["hbm"] for x in dl]))
     lds_bytes = float(np.mean([x["lds"] for x in dl]))

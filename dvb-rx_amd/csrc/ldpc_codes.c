@@ -154,6 +154,69 @@ static int build_code(lg_code* c, int id)
     }
     c->msg_words = rows * LG_M;
     c->pk_msg_words = pk_rows * LG_M;
+
+    /* hand-over slots: per layer, the groups that occur k > 1 times get
+     * k - 1 consecutive hand-over indices; each node's rank on such an
+     * edge is the number of earlier nodes of the layer touching its bit */
+    c->xent = malloc(sizeof(uint32_t) * (size_t)naddr);
+    c->pos = calloc((size_t)c->r, sizeof(uint64_t));
+    c->dupm = calloc((size_t)q, sizeof(uint32_t));
+    if (!c->xent || !c->pos || !c->dupm) {
+        free(owner);
+        return -1;
+    }
+    for (int i = 0; i < q; i++) {
+        const int s = c->layer_start[i], d = c->layer_start[i + 1] - s;
+        int nx = 0;
+        for (int e = 0; e < d; e++) {
+            const uint32_t base = c->ent[s + e] >> 16;
+            int k = 0, first = e;
+            for (int f = 0; f < d; f++) {
+                if ((c->ent[s + f] >> 16) == base) {
+                    k++;
+                    if (f < first)
+                        first = f;
+                }
+            }
+            if (k > 4) {
+                free(owner);
+                return -1;
+            }
+            uint32_t t0 = 0;
+            if (k > 1) {
+                c->dupm[i] |= 1u << e;
+                if (first == e) {
+                    t0 = (uint32_t)nx;
+                    nx += k - 1;
+                } else {
+                    t0 = LG_XENT_T0(c->xent[s + first]);
+                }
+            }
+            c->xent[s + e] = c->ent[s + e] | (uint32_t)(k - 1) << 9 | t0 << 11;
+        }
+        if (nx > LG_MAX_XFER || (nx > 0) != (c->nlev[i] > 1)) {
+            free(owner);
+            return -1;
+        }
+        if (nx > c->max_xfer)
+            c->max_xfer = nx;
+        if (!nx)
+            continue;
+        /* rank: count of earlier nodes touching the bit (owner[] counts) */
+        for (int e = 0; e < d; e++) {
+            const int base = (int)(c->ent[s + e] >> 16);
+            for (int m = 0; m < LG_M; m++)
+                owner[base + m] = 0;
+        }
+        for (int j = 0; j < LG_M; j++) {
+            for (int e = 0; e < d; e++) {
+                const int base = (int)(c->ent[s + e] >> 16), sh = (int)(c->ent[s + e] & 0xffff);
+                const int b = base + (j - sh + LG_M) % LG_M;
+                c->pos[(size_t)i * LG_M + j] |= (uint64_t)(owner[b] & 3) << (2 * e);
+                owner[b]++;
+            }
+        }
+    }
     free(owner);
     /* every edge counted: info edges plus the 2R-1 of the staircase */
     if ((long)naddr * LG_M + 2L * c->r - 1 != c->links_total)

@@ -58,7 +58,24 @@ typedef struct lg_code {
     int* msg_off;          /* q: first message word row of each layer (rows of 360 words) */
     int pk_msg_words;      /* the same for two frames, two edges a word: 360 * sum of ceil((deg_i+2)/2) */
     int* pk_msg_off;       /* q: first row of each layer in that layout */
+    /* Hand-over of shared bits between the nodes of a layer (the two-frame
+     * kernel's rounds).  A bit touched by k > 1 nodes of a layer (a group
+     * that appears k times in the layer) passes through k - 1 hand-overs,
+     * node order r -> r + 1; hand-over t of a layer uses the slots
+     * t * 360 + m (m: the bit's index in its group). */
+    uint32_t* xent;        /* nent: ent | (k - 1) << 9 | t0 << 11, t0 the
+                              group's first hand-over index (k = 1: 0) */
+    uint64_t* pos;         /* r: 2 bits per information edge, the node's
+                              rank r among the k nodes touching the bit */
+    uint32_t* dupm;        /* q: information edges of the layer in a group
+                              with k > 1 */
+    int max_xfer;          /* largest number of hand-overs of one layer */
 } lg_code;
+
+#define LG_MAX_XFER 8      /* hand-overs per layer the kernels provide for */
+#define LG_XENT_SHIFT(e) ((int)((e) & 0x1ffu))
+#define LG_XENT_KM1(e) ((int)(((e) >> 9) & 3u))
+#define LG_XENT_T0(e) ((int)(((e) >> 11) & 15u))
 
 #define LG_META_LEVEL(m) ((int)((m) & 0xff))
 #define LG_META_DEP(m) ((uint32_t)(((m) >> 8) & 0xfffffffu))

@@ -47,745 +47,24 @@
 
 #include "ldpc_codes.h"
 #include "ldpc_gpu.h"
+#include "ldpc_kern.h"
 #include "ldpc_nms.h"
-#include "ldpc_node.h"
-#include "ldpc_pk.h"
 
-#define NT 384 /* threads per workgroup of the one-frame kernel: 360 check nodes of a layer */
-/* the two-slot kernel: exactly one thread per check node, so no thread of
- * a layer is idle and the node code needs no lane test (the sixth wave has
- * 40 lanes) */
-#define PK_NT 360
-#define NCLASS 2 /* degree classes: one kernel instance each (registers grow with the degree) */
-#define NLAUNCH 4 /* launches of one decode: degree class x frame size */
-static const int CLASS_DI[NCLASS] = { 12, 28 }; /* largest layer degree of each class */
+using namespace lgk;
+
+static const int CLASS_DI[NCLASS] = { 12, 28 }; /* largest layer degree of each class (one-frame kernel) */
+static const int PK_DM[PK_NDM] = PK_DMS;        /* largest layer degree of each two-slot instance */
+
+/* two-slot kernel instance of a code: the smallest whose degree covers its layers */
+static int pk_instance(int dmax)
+{
+    for (int k = 0; k < PK_NDM; k++)
+        if (dmax <= PK_DM[k])
+            return k;
+    return -1;
+}
 
 namespace {
-
-struct DCode {
-    int n, k, r, q;
-    int ent_off;   /* into ent: base << 16 | shift */
-    int lay_off;   /* into lay (see lay_deg / lay_nlev / lay_start) */
-    int meta_off;  /* into meta: level and dependency masks per node */
-    int nent;      /* (base, shift) pairs */
-    int dmax;      /* largest information degree of a layer */
-    int dmin;      /* smallest */
-    int msg_words; /* 32-bit message words per frame */
-    int pk_msg_words; /* the same for the two-frame layout */
-    int code_id;   /* index in the registry */
-};
-
-struct DFrame {
-    int code, max_iter, nbits, pad;
-    unsigned long long llr_off, out_off;
-};
-
-struct KArgs {
-    const DCode* codes;
-    const uint32_t* ent;
-    const uint2* lay;
-    const uint64_t* meta;
-    const DFrame* frames;
-    const int* list;   /* frames to run (nullptr: 0..nrun-1) */
-    int nrun;
-    const int8_t* llr;
-    uint8_t* out;
-    int32_t* iters;
-    uint8_t* synd;
-    uint32_t* msg;     /* messages: per workgroup or per frame */
-    size_t msg_stride; /* words per message block */
-    int msg_per_frame;
-    int8_t* state;     /* group mode: posteriors saved per frame (N_max bytes) */
-    size_t state_stride;
-    int32_t* it_state; /* group mode: iterations done per frame */
-    const int32_t* target; /* resume mode: iteration to reach per frame */
-    int resume;        /* 0: decode from the LLRs with per-frame early stop;
-                          1: continue from the saved state up to target */
-    int save;          /* save posteriors + iteration count at the end */
-    unsigned* ticket;
-};
-
-/* largest table of one code staged in LDS: (base, shift) pairs and layers */
-#define MAX_ENT 1024
-#define MAX_LAYERS 192
-#define SYN_CHUNK 4  /* layers checked between two early-exit votes */
-#define WMAX 8       /* message words of a node: (28 + 2 + 3) / 4 */
-#define LOW_MAX_DEG 12 /* class 0: layers of information degree 1..12 */
-#define HIGH_MIN_DEG 9 /* class 1: 9..28 */
-
-/* layer table entry: x = first (base, shift) pair | degree << 16 | levels << 24,
- * y = first message row (rows of 360 words) */
-__device__ __forceinline__ int lay_deg(uint2 ly) { return (ly.x >> 16) & 0xff; }
-__device__ __forceinline__ int lay_nlev(uint2 ly) { return ly.x >> 24; }
-__device__ __forceinline__ int lay_start(uint2 ly) { return ly.x & 0xffff; }
-
-#define LOW_DEGS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
-#define HIGH_DEGS(X)                                                                                        \
-    X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) \
-        X(27) X(28)
-
-/* Syndrome (bad(), ldpc.c:286-305) over chunks of layers, stopping at the
- * first chunk where some check fails: an unconverged frame usually fails
- * in the first chunk. */
-template <int CLS>
-__device__ __forceinline__ int syndrome_ok(const int8_t* sm, const uint32_t* s_ent, const uint2* s_lay, int j, int K,
-                                           int q)
-{
-    for (int i0 = 0; i0 < q; i0 += SYN_CHUNK) {
-        int ok = 1;
-        if (j < 360) {
-            const int i1 = min(i0 + SYN_CHUNK, q);
-            for (int i = i0; i < i1; i++) {
-                const uint2 ly = s_lay[i];
-                const uint32_t* e = s_ent + lay_start(ly);
-#define X(D)                                                   \
-    case D:                                                    \
-        ok &= lg_node_check_t<D>(sm, e, D, i, j, K, q);        \
-        break;
-                if constexpr (CLS == 0) {
-                    switch (lay_deg(ly)) {
-                        LOW_DEGS(X)
-                    default:
-                        ok = 0;
-                    }
-                } else {
-                    switch (lay_deg(ly)) {
-                        HIGH_DEGS(X)
-                    default:
-                        ok = 0;
-                    }
-                }
-#undef X
-            }
-        }
-        if (!__syncthreads_and(ok))
-            return 0;
-    }
-    return 1;
-}
-
-/* One layer by the workgroup: phases A, S (per level), C of ldpc_node.h.
- * mw: the node's message words (zero in the first iteration). */
-template <int DEG>
-__device__ __forceinline__ void layer_t(int8_t* sm, const uint32_t* ent, int i, int j, int K, int q, int nlev,
-                                        const uint32_t* mw, uint64_t meta, uint32_t* mrow, int first)
-{
-    const bool act = j < 360;
-    const uint32_t dep = LG_META_DEP(meta), succ = LG_META_SUCC(meta);
-    const int lvl = LG_META_LEVEL(meta);
-    LgNode<DEG> n;
-    if (act) {
-        lg_phase_a<DEG>(n, sm, ent, i, j, K, q, dep, mw, first);
-        if (nlev > 1 && lvl == 0)
-            lg_write_succ<DEG>(n, sm, succ);
-    }
-    for (int l = 1; l < nlev; l++) {
-        __syncthreads();
-        if (act && lvl == l) {
-            lg_step<DEG>(n, sm, dep, mw, first);
-            lg_write_succ<DEG>(n, sm, succ);
-        }
-    }
-    if (act) {
-        uint32_t o[LG_MSG_WORDS(DEG)];
-        lg_phase_c<DEG>(n, sm, succ, o);
-#pragma unroll
-        for (int w = 0; w < LG_MSG_WORDS(DEG); w++)
-            mrow[360 * w + j] = o[w];
-    }
-}
-
-/*
- * One decode (from the LLRs or resumed) of one frame by the workgroup.
- * Returns the syndrome flag; *itp holds the iterations run.  The messages
- * and dependency masks of layer i+1 are fetched while layer i runs.
- */
-template <int CLS>
-__device__ __forceinline__ int decode_frame(int8_t* sm, const uint32_t* s_ent, const uint2* s_lay,
-                                            const uint64_t* meta, uint32_t* msg, int K, int q, int* itp, int target,
-                                            int early_stop)
-{
-    const int j = threadIdx.x;
-    const bool act = j < 360;
-    int it = *itp, ok;
-    for (;;) {
-        if (early_stop || it >= target) {
-            ok = syndrome_ok<CLS>(sm, s_ent, s_lay, j, K, q);
-            if (ok || it >= target)
-                break;
-        }
-        const int first = it == 0;
-        uint32_t nx[WMAX];
-        uint64_t nmeta = 0;
-#pragma unroll
-        for (int w = 0; w < WMAX; w++)
-            nx[w] = 0;
-        {
-            const uint2 ly = s_lay[0];
-            const int W = LG_MSG_WORDS(lay_deg(ly));
-            if (act && !first) {
-#pragma unroll
-                for (int w = 0; w < WMAX; w++)
-                    if (w < W)
-                        nx[w] = msg[360 * (ly.y + w) + j];
-            }
-            if (act && lay_nlev(ly) > 1)
-                nmeta = meta[j];
-        }
-        for (int i = 0; i < q; i++) {
-            uint32_t mw[WMAX];
-#pragma unroll
-            for (int w = 0; w < WMAX; w++)
-                mw[w] = nx[w];
-            const uint64_t mt = nmeta;
-            const uint2 ly = s_lay[i];
-            if (i + 1 < q) {
-                const uint2 ln = s_lay[i + 1];
-                const int W = LG_MSG_WORDS(lay_deg(ln));
-                if (act && !first) {
-#pragma unroll
-                    for (int w = 0; w < WMAX; w++)
-                        if (w < W)
-                            nx[w] = msg[360 * (ln.y + w) + j];
-                }
-                nmeta = act && lay_nlev(ln) > 1 ? meta[360 * (i + 1) + j] : 0;
-            }
-            const uint32_t* e = s_ent + lay_start(ly);
-            uint32_t* mrow = msg + 360 * ly.y;
-            const int nlev = lay_nlev(ly);
-#define X(D)                                                            \
-    case D:                                                             \
-        layer_t<D>(sm, e, i, j, K, q, nlev, mw, mt, mrow, first);       \
-        break;
-            if constexpr (CLS == 0) {
-                switch (lay_deg(ly)) {
-                    LOW_DEGS(X)
-                }
-            } else {
-                switch (lay_deg(ly)) {
-                    HIGH_DEGS(X)
-                }
-            }
-#undef X
-            __syncthreads();
-        }
-        it++;
-    }
-    *itp = it;
-    return ok;
-}
-
-#define NCOUNTERS 256 /* work counters: tickets and per-segment claims */
-
-__global__ void __launch_bounds__(256) reset_ticket(unsigned* t)
-{
-    t[threadIdx.x] = 0;
-}
-
-template <int CLS>
-__global__ void __launch_bounds__(NT) ldpc_layered_kernel(KArgs a)
-{
-    extern __shared__ __align__(16) int8_t sm[];
-    __shared__ int s_run;
-    __shared__ uint32_t s_ent[MAX_ENT];
-    __shared__ uint2 s_lay[MAX_LAYERS];
-    const int tid = threadIdx.x;
-
-    for (;;) {
-        if (tid == 0)
-            s_run = (int)atomicAdd(a.ticket, 1u);
-        __syncthreads();
-        const int run = s_run;
-        __syncthreads();
-        if (run >= a.nrun)
-            break;
-        const int f = a.list ? a.list[run] : run;
-        const DFrame fr = a.frames[f];
-        const DCode cd = a.codes[fr.code];
-        const uint32_t* ent = a.ent + cd.ent_off;
-        const uint2* lay = a.lay + cd.lay_off;
-        const uint64_t* meta = a.meta + cd.meta_off;
-        const int K = cd.k, N = cd.n, R = cd.r, q = cd.q;
-        uint32_t* msg = a.msg + a.msg_stride * (size_t)(a.msg_per_frame ? f : blockIdx.x);
-
-        int it;
-        int target;
-        if (!a.resume) {
-            /* info bits straight, parity transposed to layer order */
-            const uint2* src = (const uint2*)(a.llr + fr.llr_off);
-            for (int v = tid; v < K / 8; v += NT)
-                ((uint2*)sm)[v] = src[v];
-            const int8_t* par = a.llr + fr.llr_off + K;
-            for (int t = tid; t < R; t += NT) {
-                const int jj = t / q, ii = t - jj * q;
-                sm[K + 360 * ii + jj] = par[t];
-            }
-            it = 0;
-            target = fr.max_iter;
-        } else {
-            const uint2* src = (const uint2*)(a.state + a.state_stride * (size_t)f);
-            for (int v = tid; v < N / 8; v += NT)
-                ((uint2*)sm)[v] = src[v];
-            it = a.it_state[f];
-            target = a.target[f];
-        }
-        /* the code's layer table and (base, shift) pairs, read by every
-         * node of the frame */
-        for (int t = tid; t < cd.nent; t += NT)
-            s_ent[t] = ent[t];
-        for (int t = tid; t < q; t += NT)
-            s_lay[t] = lay[t];
-        __syncthreads();
-
-        const int ok = decode_frame<CLS>(sm, s_ent, s_lay, meta, msg, K, q, &it, target, !a.resume);
-
-        /* hard decisions of the first nbits bits, MSB first, 4 bytes a thread */
-        const int nbytes = (fr.nbits + 7) / 8;
-        uint8_t* o = a.out + fr.out_off;
-        for (int b4 = 4 * tid; b4 < nbytes; b4 += 4 * NT) {
-            uint32_t v = 0;
-            for (int k = 0; k < 32; k++) {
-                const int bit = 8 * b4 + k;
-                v = v << 1 | (uint32_t)(bit < fr.nbits && sm[bit] < 0);
-            }
-            for (int k = 0; k < 4 && b4 + k < nbytes; k++)
-                o[b4 + k] = (uint8_t)(v >> (24 - 8 * k));
-        }
-        if (a.save) {
-            uint2* dst = (uint2*)(a.state + a.state_stride * (size_t)f);
-            for (int v = tid; v < N / 8; v += NT)
-                dst[v] = ((const uint2*)sm)[v];
-        }
-        if (tid == 0) {
-            a.iters[f] = it;
-            a.synd[f] = (uint8_t)ok;
-            if (a.save)
-                a.it_state[f] = it;
-        }
-        __syncthreads();
-    }
-}
-
-/* ---- two frames per workgroup: packed 16-bit arithmetic (ldpc_pk.h) ----- */
-
-#define PK_WMAX 15 /* message words of a node: (28 + 2 + 1) / 2 */
-
-struct PkArgs {
-    const DCode* codes;
-    const uint32_t* ent;
-    const uint2* lay;
-    const uint32_t* pkoff; /* per layer: first message row in the two-frame layout */
-    const uint64_t* meta;
-    const DFrame* frames;
-    const int* list;       /* frame indices, grouped by code into segments */
-    const int2* seg;       /* per segment: first list entry, length */
-    int nseg;
-    unsigned* cnt;         /* per segment: frames claimed so far */
-    const int8_t* llr;
-    uint8_t* out;
-    int32_t* iters;
-    uint8_t* synd;
-    uint32_t* msg;         /* message block per workgroup */
-    size_t msg_stride;
-    /* learned mode: parameter rows (row 0: the plain decoder's S = 16,
-     * B = 1), per code its first row and row count, plain fallback cap,
-     * per-frame fallback flag (may be NULL) */
-    const uint8_t* nS;
-    const int16_t* nB;
-    const int* nrow0;
-    const int* nT;
-    int fallback;
-    uint8_t* fell;
-};
-
-/* parameter rows of the two slots for one iteration (learned mode) */
-struct PkRowPtrs {
-    const uint8_t* S0;
-    const uint8_t* S1;
-    const int16_t* B0;
-    const int16_t* B1;
-};
-
-/* AND of a 2-bit value over the workgroup, one barrier: three vote words
- * in turn, the next one reset before the barrier of the current vote */
-__device__ __forceinline__ uint32_t wg_and2(uint32_t v, unsigned* s_vote, int& n)
-{
-    const int slot = n % 3;
-    n++;
-    const uint64_t f0 = __ballot(!(v & 1)), f1 = __ballot(!(v & 2));
-    if ((threadIdx.x & 63) == 0)
-        atomicAnd(&s_vote[slot], (f0 ? 0u : 1u) | (f1 ? 0u : 2u));
-    if (threadIdx.x == 0)
-        s_vote[(slot + 1) % 3] = 3;
-    __syncthreads();
-    return s_vote[slot];
-}
-
-/* syndrome of both slots, chunks of layers with an early exit once every
- * slot that is still decoding has failed a check; bit k: slot k holds */
-template <int CLS>
-__device__ __forceinline__ uint32_t pk_syndrome(const uint16_t* sm, const uint32_t* ent, const uint2* lay, int j,
-                                                int K, int q, uint32_t live, unsigned* s_vote, int& nvote)
-{
-    uint32_t good = 3;
-    for (int i0 = 0; i0 < q; i0 += SYN_CHUNK) {
-        uint32_t ok = 3;
-        if (j < 360) {
-            const int i1 = min(i0 + SYN_CHUNK, q);
-            for (int i = i0; i < i1; i++) {
-                const uint2 ly = lay[i];
-                const uint32_t* e = ent + lay_start(ly);
-#define X(D)                                             \
-    case D:                                              \
-        ok &= pk_check<D>(sm, e, i, j, K, q);            \
-        break;
-                if constexpr (CLS == 0) {
-                    switch (lay_deg(ly)) {
-                        LOW_DEGS(X)
-                    default:
-                        ok = 0;
-                    }
-                } else {
-                    switch (lay_deg(ly)) {
-                        HIGH_DEGS(X)
-                    default:
-                        ok = 0;
-                    }
-                }
-#undef X
-            }
-        }
-        good &= wg_and2(ok, s_vote, nvote);
-        if (!(good & live))
-            break;
-    }
-    return good;
-}
-
-template <int DEG, bool TAIL, bool NMS>
-__device__ __forceinline__ void pk_layer_t(uint16_t* sm, const uint32_t* ent, int i, int j, int K, int q, int nlev,
-                                           const uint32_t* mw, uint64_t meta, uint32_t* mrow, uint32_t se, uint32_t so,
-                                           lg_pk ps, lg_pk pb)
-{
-    const bool act = j < 360;
-    const uint32_t dep = TAIL ? LG_META_DEP(meta) : 0, succ = TAIL ? LG_META_SUCC(meta) : 0;
-    const int lvl = LG_META_LEVEL(meta);
-    PkNode<DEG> n;
-    if (act) {
-        pk_phase_a<DEG, TAIL>(n, sm, ent, i, j, K, q, dep, mw, se, so);
-        if (TAIL && lvl == 0)
-            pk_write_succ<DEG, NMS>(n, sm, succ, ps, pb);
-    }
-    if (TAIL) {
-        for (int l = 1; l < nlev; l++) {
-            __syncthreads();
-            if (act && lvl == l) {
-                pk_step<DEG>(n, sm, dep, mw, se, so);
-                pk_write_succ<DEG, NMS>(n, sm, succ, ps, pb);
-            }
-        }
-    }
-    if (act) {
-        uint32_t o[LG_PK_WORDS(DEG)];
-        pk_phase_c<DEG, TAIL, NMS>(n, sm, succ, o, ps, pb);
-#pragma unroll
-        for (int w = 0; w < LG_PK_WORDS(DEG); w++)
-            mrow[360 * w + j] = o[w];
-    }
-}
-
-/* one iteration (all layers) on both slots; the code's tables are read
- * through uniform pointers (scalar loads) */
-template <int CLS, bool NMS>
-__device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, const uint2* lay,
-                                             const uint32_t* pkoff, const uint64_t* meta, uint32_t* msg, int K,
-                                             int q, int first0, int first1, PkRowPtrs rows)
-{
-    const int j = threadIdx.x;
-    const bool act = j < 360;
-    const uint32_t se = pk_msg_sel(0, first0, first1), so = pk_msg_sel(1, first0, first1);
-    const bool load = !(first0 && first1);
-    constexpr int WM = CLS == 0 ? LG_PK_WORDS(LOW_MAX_DEG) : PK_WMAX;
-    /* messages and masks of layer i+1 are fetched while layer i runs */
-    uint32_t nx[WM];
-    uint64_t nm = 0;
-#pragma unroll
-    for (int w = 0; w < WM; w++)
-        nx[w] = 0;
-    auto fetch = [&](int l) {
-        if (l < q && act) {
-            const uint2 ln = lay[l];
-            const int W = LG_PK_WORDS(lay_deg(ln));
-            if (load) {
-                const uint32_t* row = msg + 360 * pkoff[l] + j;
-#pragma unroll
-                for (int w = 0; w < WM; w++)
-                    if (w < W)
-                        nx[w] = row[360 * w];
-            }
-            nm = lay_nlev(ln) > 1 ? meta[360 * l + j] : 0;
-        }
-    };
-    fetch(0);
-    /* one layer with its degree D fixed */
-    auto layer = [&](auto dtag, int i) {
-        constexpr int D = decltype(dtag)::value;
-        uint32_t mw[WM];
-#pragma unroll
-        for (int w = 0; w < WM; w++)
-            mw[w] = nx[w];
-        const uint64_t mt = nm;
-        fetch(i + 1);
-        const uint2 ly = lay[i];
-        const uint32_t* e = ent + lay_start(ly);
-        uint32_t* mrow = msg + 360 * pkoff[i];
-        const int nlev = lay_nlev(ly);
-        /* learned scale / offset of the layer's check degree, per slot;
-         * node (0, 0) has no previous parity, one edge less */
-        lg_pk ps = pk_dup(16), pb = pk_dup(16);
-        if constexpr (NMS) {
-            const int dc = D + 2 - (i == 0 && j == 0);
-            ps = lg_pk{ (short)rows.S0[dc], (short)rows.S1[dc] };
-            pb = lg_pk{ (short)rows.B0[dc], (short)rows.B1[dc] };
-        }
-        if (nlev > 1)
-            pk_layer_t<D, true, NMS>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so, ps, pb);
-        else
-            pk_layer_t<D, false, NMS>(sm, e, i, j, K, q, nlev, mw, mt, mrow, se, so, ps, pb);
-        __syncthreads();
-    };
-    /* a layer of no degree of the class has no node to run */
-    auto skip = [&](int i) {
-        fetch(i + 1);
-        __syncthreads();
-    };
-    if constexpr (CLS == 0) {
-        /* low degrees: layers in runs of one degree, dispatched once per
-         * run (the per-layer dispatch is a large share of these layers'
-         * scalar work) */
-        for (int i = 0; i < q;) {
-#define X(D)                                                  \
-    case D:                                                   \
-        do                                                    \
-            layer(std::integral_constant<int, D>{}, i++);     \
-        while (i < q && lay_deg(lay[i]) == D);                \
-        break;
-            switch (lay_deg(lay[i])) {
-                LOW_DEGS(X)
-            default:
-                skip(i++);
-            }
-#undef X
-        }
-    } else {
-        /* high degrees: one dispatch per layer (runs would spill) */
-        for (int i = 0; i < q; i++) {
-#define X(D)                                          \
-    case D:                                           \
-        layer(std::integral_constant<int, D>{}, i);   \
-        break;
-            switch (lay_deg(lay[i])) {
-                HIGH_DEGS(X)
-            default:
-                skip(i);
-            }
-#undef X
-        }
-    }
-}
-
-/*
- * Persistent two-slot decoder.  Frames of the launch are grouped by code
- * into segments; a workgroup works on one segment at a time, and whenever
- * a slot's frame is done (syndrome holds or max_iter reached) the slot is
- * refilled with the next frame of the same segment while the other slot
- * continues.  When the segment is exhausted and both slots are empty, the
- * workgroup moves to another segment with frames left, or exits.
- */
-template <int CLS, bool NMS>
-__global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
-{
-    extern __shared__ __align__(16) uint16_t sm16[];
-    __shared__ unsigned s_vote[3];
-    __shared__ int s_claim[2], s_seg, s_newseg;
-    const int tid = threadIdx.x;
-    if (tid < 3)
-        s_vote[tid] = 3;
-    int nvote = 0;
-    int seg = -1;                 /* current segment (uniform) */
-    int frame[2] = { -1, -1 };    /* frame in each slot, -1: empty */
-    int it[2] = { 0, 0 }, maxit[2] = { 0, 0 };
-    int phase[2] = { 0, 0 }, lit[2] = { 0, 0 }; /* learned mode: 1 = plain fallback; learned iterations */
-    uint32_t* msg = a.msg + a.msg_stride * (size_t)blockIdx.x;
-    DCode cd;
-    const uint64_t* meta = nullptr;
-
-    for (;;) {
-        /* ---- claim frames for the empty slots ---- */
-        if (tid == 0) {
-            int cl[2] = { -1, -1 }, sg = seg, newseg = 0;
-            for (int s = 0; s < 2; s++) {
-                if (frame[s] >= 0 || sg < 0)
-                    continue;
-                const unsigned t = atomicAdd(&a.cnt[sg], 1u);
-                if ((int)t < a.seg[sg].y)
-                    cl[s] = a.list[a.seg[sg].x + t];
-            }
-            if (frame[0] < 0 && frame[1] < 0 && cl[0] < 0 && cl[1] < 0) {
-                /* both slots empty, segment done: take another one */
-                const int start = sg < 0 ? (int)(blockIdx.x % (unsigned)a.nseg) : sg + 1;
-                for (int k = 0; k < a.nseg && cl[0] < 0; k++) {
-                    const int g = (start + k) % a.nseg;
-                    if (__hip_atomic_load(&a.cnt[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (unsigned)a.seg[g].y)
-                        continue;
-                    for (int s = 0; s < 2; s++) {
-                        const unsigned t = atomicAdd(&a.cnt[g], 1u);
-                        if ((int)t < a.seg[g].y) {
-                            cl[s] = a.list[a.seg[g].x + t];
-                            sg = g;
-                            newseg = 1;
-                        }
-                    }
-                }
-            }
-            s_claim[0] = cl[0];
-            s_claim[1] = cl[1];
-            s_seg = sg;
-            s_newseg = newseg;
-        }
-        __syncthreads();
-        const int cl0 = __builtin_amdgcn_readfirstlane(s_claim[0]);
-        const int cl1 = __builtin_amdgcn_readfirstlane(s_claim[1]);
-        const int newseg = __builtin_amdgcn_readfirstlane(s_newseg);
-        seg = __builtin_amdgcn_readfirstlane(s_seg);
-        __syncthreads(); /* s_claim is rewritten only after this */
-        if (cl0 >= 0)
-            frame[0] = cl0;
-        if (cl1 >= 0)
-            frame[1] = cl1;
-        if (frame[0] < 0 && frame[1] < 0)
-            break;
-        const int fany = frame[0] >= 0 ? frame[0] : frame[1];
-        if (newseg) {
-            cd = a.codes[a.frames[fany].code];
-            meta = a.meta + cd.meta_off;
-        }
-        const uint32_t* ent = a.ent + cd.ent_off;
-        const uint2* lay = a.lay + cd.lay_off;
-        const uint32_t* pkoff = a.pkoff + cd.lay_off;
-        const int K = cd.k, R = cd.r, q = cd.q;
-        /* a frame's LLRs into its slot bytes: information bits in order,
-         * parity bit K + q*jj + ii at K + 360*ii + jj (layer order) */
-        auto load_slot = [&](int s, int f) {
-            const DFrame fr = a.frames[f];
-            const uint32_t* src = (const uint32_t*)(a.llr + fr.llr_off);
-            uint8_t* sb = (uint8_t*)sm16 + s;
-            for (int v4 = tid; v4 < K / 4; v4 += PK_NT) {
-                const uint32_t w = src[v4];
-                for (int b = 0; b < 4; b++)
-                    sb[2 * (4 * v4 + b)] = (uint8_t)(w >> (8 * b));
-            }
-            const uint8_t* par = (const uint8_t*)(a.llr + fr.llr_off + K);
-            for (int t = tid; t < R; t += PK_NT) {
-                const int jj = t / q, ii = t - jj * q;
-                sb[2 * (K + 360 * ii + jj)] = par[t];
-            }
-        };
-        /* ---- load the new frames into their slot bytes ---- */
-        for (int s = 0; s < 2; s++) {
-            const int f = s ? cl1 : cl0;
-            if (f < 0)
-                continue;
-            it[s] = 0;
-            maxit[s] = a.frames[f].max_iter;
-            phase[s] = 0;
-            lit[s] = 0;
-            load_slot(s, f);
-        }
-        __syncthreads();
-
-        /* ---- iterate until a slot is done ---- */
-        for (;;) {
-            const uint32_t live = (frame[0] >= 0 ? 1u : 0u) | (frame[1] >= 0 ? 2u : 0u);
-            const uint32_t good = pk_syndrome<CLS>(sm16, ent, lay, tid, K, q, live, s_vote, nvote);
-            uint32_t done = 0;
-            for (int s = 0; s < 2; s++) {
-                const int cap = NMS && phase[s] ? a.fallback : maxit[s];
-                if (frame[s] >= 0 && (((good >> s) & 1) || it[s] >= cap))
-                    done |= 1u << s;
-            }
-            if (done) {
-                uint32_t reload = 0;
-                for (int s = 0; s < 2; s++) {
-                    if (!((done >> s) & 1))
-                        continue;
-                    if constexpr (NMS) {
-                        /* learned iterations failed: decode the frame again
-                         * from its LLRs with the plain arithmetic */
-                        if (!((good >> s) & 1) && phase[s] == 0 && a.fallback > 0) {
-                            reload |= 1u << s;
-                            continue;
-                        }
-                    }
-                    const DFrame fr = a.frames[frame[s]];
-                    const int nbytes = (fr.nbits + 7) / 8;
-                    uint8_t* o = a.out + fr.out_off;
-                    const uint8_t* sb = (const uint8_t*)sm16 + s;
-                    for (int b4 = 4 * tid; b4 < nbytes; b4 += 4 * PK_NT) {
-                        uint32_t v = 0;
-                        for (int k = 0; k < 32; k++) {
-                            const int bit = 8 * b4 + k;
-                            v = v << 1 | (uint32_t)(bit < fr.nbits && (int8_t)sb[2 * bit] < 0);
-                        }
-                        for (int k = 0; k < 4 && b4 + k < nbytes; k++)
-                            o[b4 + k] = (uint8_t)(v >> (24 - 8 * k));
-                    }
-                    if (tid == 0) {
-                        a.iters[frame[s]] = lit[s] + it[s];
-                        a.synd[frame[s]] = (uint8_t)((good >> s) & 1);
-                        if (NMS && a.fell)
-                            a.fell[frame[s]] = (uint8_t)phase[s];
-                    }
-                    frame[s] = -1;
-                }
-                __syncthreads(); /* slot bytes read before a refill overwrites them */
-                if constexpr (NMS) {
-                    if (reload) {
-                        for (int s = 0; s < 2; s++) {
-                            if (!((reload >> s) & 1))
-                                continue;
-                            lit[s] = it[s];
-                            it[s] = 0;
-                            phase[s] = 1;
-                            load_slot(s, frame[s]);
-                        }
-                        __syncthreads();
-                    }
-                    if (reload == done)
-                        continue; /* no slot emptied: keep iterating */
-                }
-                break;
-            }
-            PkRowPtrs rows;
-            if constexpr (NMS) {
-                const int r0c = a.nrow0[cd.code_id], tc = max(a.nT[cd.code_id], 1);
-                int row[2];
-                for (int s = 0; s < 2; s++)
-                    row[s] = frame[s] >= 0 && phase[s] == 0 ? r0c + min(it[s], tc - 1) : 0;
-                rows.S0 = a.nS + LDPC_NMS_DEGS * row[0];
-                rows.S1 = a.nS + LDPC_NMS_DEGS * row[1];
-                rows.B0 = a.nB + LDPC_NMS_DEGS * row[0];
-                rows.B1 = a.nB + LDPC_NMS_DEGS * row[1];
-            } else {
-                rows = PkRowPtrs{ nullptr, nullptr, nullptr, nullptr };
-            }
-            pk_iteration<CLS, NMS>(sm16, ent, lay, pkoff, meta, msg, K, q, frame[0] < 0 || it[0] == 0,
-                                   frame[1] < 0 || it[1] == 0, rows);
-            for (int s = 0; s < 2; s++)
-                it[s] += frame[s] >= 0;
-        }
-    }
-}
 
 /* ---- device-resident code tables (one copy per device) ------------------ */
 
@@ -793,8 +72,12 @@ struct DeviceTables {
     int ready = 0;
     DCode* codes = nullptr;
     uint32_t* ent = nullptr;
+    uint32_t* xent = nullptr;
+    uint64_t* pos = nullptr;
+    uint32_t* dupm = nullptr;
     uint2* lay = nullptr;
-    uint32_t* pkoff = nullptr;
+    uint4* pkl = nullptr;    /* per layer of every code: the two-slot kernel's descriptor */
+    uint32_t* adt = nullptr; /* its address rows */
     uint64_t* meta = nullptr;
     std::vector<DCode> h_codes;
 };
@@ -830,7 +113,7 @@ struct ldpc_gpu_ctx {
     hipEvent_t ev0, ev1;
     hipEvent_t ev_cls[2 * NLAUNCH];
     int cls_frames[NLAUNCH]; /* frames of each launch of the last decode */
-    int cls_di[NLAUNCH];     /* its degree class bound */
+    int cls_di[NLAUNCH];     /* its kernel instance's layer degree bound */
     int timed;
     int force_scalar;        /* per-frame batches through the one-frame kernel */
     int serial;              /* launches of a decode one after the other on the context's stream */
@@ -908,10 +191,11 @@ static int upload_tables(ldpc_gpu_ctx* c)
         return LDPC_GPU_OK;
     const int nc = lg_code_count();
     std::vector<DCode> codes(nc);
-    std::vector<uint32_t> ent;
+    std::vector<uint32_t> ent, xent, dupm;
     std::vector<uint2> lay;
-    std::vector<uint32_t> pkoff;
-    std::vector<uint64_t> meta;
+    std::vector<uint4> pkl;
+    std::vector<uint32_t> adt;
+    std::vector<uint64_t> meta, pos;
     for (int id = 0; id < nc; id++) {
         const lg_code* cd = lg_code_get(id);
         if (!cd)
@@ -930,15 +214,34 @@ static int upload_tables(ldpc_gpu_ctx* c)
         d.msg_words = cd->msg_words;
         d.pk_msg_words = cd->pk_msg_words;
         d.code_id = id;
+        d.max_xfer = cd->max_xfer;
         ent.insert(ent.end(), cd->ent, cd->ent + cd->nent);
+        xent.insert(xent.end(), cd->xent, cd->xent + cd->nent);
+        dupm.insert(dupm.end(), cd->dupm, cd->dupm + cd->q);
         for (int i = 0; i < cd->q; i++) {
             const int st = cd->layer_start[i], deg = cd->layer_start[i + 1] - st;
             d.dmin = std::min(d.dmin, deg);
             lay.push_back(make_uint2((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24,
                                      (uint32_t)cd->msg_off[i]));
-            pkoff.push_back((uint32_t)cd->pk_msg_off[i]);
+            /* address rows of the layer: node j's edges 2w, 2w + 1 in word j
+             * of row w (posterior indices, < 2^16) */
+            const uint32_t arow = (uint32_t)(adt.size() / 360);
+            for (int w = 0; w < (deg + 1) / 2; w++) {
+                for (int j = 0; j < 360; j++) {
+                    uint32_t v = 0;
+                    for (int h = 0; h < 2 && 2 * w + h < deg; h++) {
+                        const uint32_t en = cd->ent[st + 2 * w + h];
+                        const int m = (j - (int)(en & 0xffff) + 360) % 360;
+                        v |= ((en >> 16) + (uint32_t)m) << (16 * h);
+                    }
+                    adt.push_back(v);
+                }
+            }
+            pkl.push_back(make_uint4((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24,
+                                     (uint32_t)cd->pk_msg_off[i], cd->dupm[i], arow));
         }
         meta.insert(meta.end(), cd->meta, cd->meta + cd->r);
+        pos.insert(pos.end(), cd->pos, cd->pos + cd->r);
         /* every layer degree must be one the code's class kernel handles */
         const int cls = d.dmax <= LOW_MAX_DEG ? 0 : 1;
         if (cd->nent > MAX_ENT || cd->q > MAX_LAYERS || d.dmax > 28 || (cls == 1 && d.dmin < HIGH_MIN_DEG) ||
@@ -949,8 +252,16 @@ static int upload_tables(ldpc_gpu_ctx* c)
     HIPCHK(c, hipMalloc((void**)&t.ent, sizeof(uint32_t) * ent.size()));
     HIPCHK(c, hipMalloc((void**)&t.lay, sizeof(uint2) * lay.size()));
     HIPCHK(c, hipMalloc((void**)&t.meta, sizeof(uint64_t) * meta.size()));
-    HIPCHK(c, hipMalloc((void**)&t.pkoff, sizeof(uint32_t) * pkoff.size()));
-    HIPCHK(c, hipMemcpy(t.pkoff, pkoff.data(), sizeof(uint32_t) * pkoff.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc((void**)&t.pkl, sizeof(uint4) * pkl.size()));
+    HIPCHK(c, hipMalloc((void**)&t.adt, sizeof(uint32_t) * adt.size()));
+    HIPCHK(c, hipMemcpy(t.adt, adt.data(), sizeof(uint32_t) * adt.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc((void**)&t.xent, sizeof(uint32_t) * xent.size()));
+    HIPCHK(c, hipMalloc((void**)&t.dupm, sizeof(uint32_t) * dupm.size()));
+    HIPCHK(c, hipMalloc((void**)&t.pos, sizeof(uint64_t) * pos.size()));
+    HIPCHK(c, hipMemcpy(t.xent, xent.data(), sizeof(uint32_t) * xent.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.dupm, dupm.data(), sizeof(uint32_t) * dupm.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.pos, pos.data(), sizeof(uint64_t) * pos.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.pkl, pkl.data(), sizeof(uint4) * pkl.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.codes, codes.data(), sizeof(DCode) * codes.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.ent, ent.data(), sizeof(uint32_t) * ent.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.lay, lay.data(), sizeof(uint2) * lay.size(), hipMemcpyHostToDevice));
@@ -1073,6 +384,13 @@ int ldpc_gpu_code_max_degree(int code)
 {
     const lg_code* c = lg_code_get(code);
     return c ? c->max_layer_deg : LDPC_GPU_EINVAL;
+}
+
+int ldpc_gpu_code_kernel_degree(int code)
+{
+    const lg_code* c = lg_code_get(code);
+    const int k = c ? pk_instance(c->max_layer_deg) : -1;
+    return k >= 0 ? PK_DM[k] : LDPC_GPU_EINVAL;
 }
 
 int ldpc_gpu_fec_lookup(int short_frame, const char* rate, int* nldpc, int* kldpc, int* kbch, int* t)
@@ -1200,6 +518,31 @@ float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* c)
 
 } // extern "C"
 
+namespace lgk {
+#define LGK_PK_CASE_L(D, N)                                                                         \
+    if (PK_DM[k] == D && nms == (N != 0))                                                             \
+        return pk_launch_##D##_##N(grid, lds, st, a);
+#define LGK_PK_CASE_O(D, N)                                                                         \
+    if (PK_DM[k] == D && nms == (N != 0))                                                             \
+        return pk_occupancy_##D##_##N(lds, per_cu);
+
+hipError_t pk_launch(int k, bool nms, int grid, size_t lds, hipStream_t st, const PkArgs& a)
+{
+    if (k >= 0 && k < PK_NDM) {
+        LGK_PK_ALL(LGK_PK_CASE_L)
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t pk_occupancy(int k, bool nms, size_t lds, int* per_cu)
+{
+    if (k >= 0 && k < PK_NDM) {
+        LGK_PK_ALL(LGK_PK_CASE_O)
+    }
+    return hipErrorInvalidValue;
+}
+} // namespace lgk
+
 template <typename T>
 static int grow(ldpc_gpu_ctx* c, T** p, size_t* cap_bytes, size_t need)
 {
@@ -1221,7 +564,7 @@ template <int DI>
 static int class_grid(ldpc_gpu_ctx* c, size_t lds, int nrun, int* grid)
 {
     int per_cu = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_layered_kernel<DI>, NT, lds));
+    HIPCHK(c, layered_occupancy(DI, lds, &per_cu));
     if (per_cu < 1)
         return set_err(c, LDPC_GPU_EDEVICE, "kernel does not fit (LDS %zu)", lds);
     *grid = std::min(nrun, per_cu * c->ncu);
@@ -1234,8 +577,7 @@ static int launch_class(ldpc_gpu_ctx* c, const KArgs& a, size_t lds, int grid, i
     if (!a.msg_per_frame && (size_t)grid * a.msg_stride * sizeof(uint32_t) > c->msg_bytes)
         return set_err(c, LDPC_GPU_EINVAL, "message scratch too small");
     HIPCHK(c, hipEventRecord(c->ev_cls[2 * cls], c->stream));
-    hipLaunchKernelGGL(ldpc_layered_kernel<DI>, dim3(grid), dim3(NT), lds, c->stream, a);
-    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, layered_launch(DI, grid, lds, c->stream, a));
     HIPCHK(c, hipEventRecord(c->ev_cls[2 * cls + 1], c->stream));
     return LDPC_GPU_OK;
 }
@@ -1286,7 +628,7 @@ static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFr
     if ((rc = grow(c, &c->msg, &c->msg_bytes, blocks * msg_stride * sizeof(uint32_t))))
         return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_list, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(NCOUNTERS), 0, c->stream, c->ticket);
+    HIPCHK(c, reset_counters(c->stream, c->ticket));
     KArgs a;
     memset(&a, 0, sizeof(a));
     a.codes = c->tab->codes;
@@ -1331,11 +673,10 @@ static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFr
     return LDPC_GPU_OK;
 }
 
-template <int CLS, bool NMS>
-static int pk_grid(ldpc_gpu_ctx* c, size_t lds, int nframes, int* grid)
+static int pk_grid(ldpc_gpu_ctx* c, int inst, bool nms, size_t lds, int nframes, int* grid)
 {
     int per_cu = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ldpc_pk_kernel<CLS, NMS>, PK_NT, lds));
+    HIPCHK(c, pk_occupancy(inst, nms, lds, &per_cu));
     if (per_cu < 1)
         return set_err(c, LDPC_GPU_EDEVICE, "two-slot kernel does not fit (LDS %zu)", lds);
     *grid = std::min((nframes + 1) / 2, per_cu * c->ncu);
@@ -1349,10 +690,15 @@ static int pk_grid(ldpc_gpu_ctx* c, size_t lds, int nframes, int* grid)
 struct PkLayout {
     std::vector<int> list;
     std::vector<int2> segs;
-    int seg0[NLAUNCH], nseg[NLAUNCH], nfr[NLAUNCH], grid[NLAUNCH], maxn[NLAUNCH];
+    int seg0[NLAUNCH], nseg[NLAUNCH], nfr[NLAUNCH], grid[NLAUNCH], maxn[NLAUNCH], maxx[NLAUNCH];
     int maxw;
     size_t blocks; /* message blocks: sum of the grids */
 };
+
+/* LDS of a launch: both slots' posteriors (2 bytes per bit of its longest
+ * code), then two hand-over buffers of maxx * 360 cells */
+static size_t pk_hoff(const PkLayout& L, int k) { return (2 * (size_t)L.maxn[k] + 15) & ~(size_t)15; }
+static size_t pk_lds(const PkLayout& L, int k) { return pk_hoff(L, k) + 2 * sizeof(uint16_t) * 360 * (size_t)L.maxx[k]; }
 
 static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, float grid_scale, PkLayout& L)
 {
@@ -1361,10 +707,10 @@ static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, f
     std::vector<std::vector<int>> gseg[NLAUNCH];
     L.maxw = 0;
     for (int k = 0; k < NLAUNCH; k++)
-        L.maxn[k] = 0;
+        L.maxn[k] = L.maxx[k] = 0;
     for (int f = 0; f < (int)hf.size(); f++) {
         const DCode& cd = hc[hf[f].code];
-        const int k = class_of(cd) * 2 + (cd.n < 64800);
+        const int k = pk_instance(cd.dmax) * 2 + (cd.n < 64800);
         auto it = std::find(gcodes[k].begin(), gcodes[k].end(), hf[f].code);
         size_t g = it - gcodes[k].begin();
         if (it == gcodes[k].end()) {
@@ -1373,6 +719,7 @@ static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, f
         }
         gseg[k][g].push_back(f);
         L.maxn[k] = std::max(L.maxn[k], cd.n);
+        L.maxx[k] = std::max(L.maxx[k], cd.max_xfer);
         L.maxw = std::max(L.maxw, cd.pk_msg_words);
     }
     L.list.clear();
@@ -1390,11 +737,8 @@ static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, f
         }
         if (!L.nfr[k])
             continue;
-        const size_t lds = 2 * (size_t)L.maxn[k];
-        const int rc = nms ? (k < 2 ? pk_grid<0, true>(c, lds, L.nfr[k], &L.grid[k])
-                                    : pk_grid<1, true>(c, lds, L.nfr[k], &L.grid[k]))
-                           : (k < 2 ? pk_grid<0, false>(c, lds, L.nfr[k], &L.grid[k])
-                                    : pk_grid<1, false>(c, lds, L.nfr[k], &L.grid[k]));
+        const size_t lds = pk_lds(L, k);
+        const int rc = pk_grid(c, k / 2, nms, lds, L.nfr[k], &L.grid[k]);
         if (rc)
             return rc;
         L.grid[k] = std::max(1, (int)((float)L.grid[k] * grid_scale));
@@ -1415,10 +759,10 @@ static int pk_enqueue(ldpc_gpu_ctx* c, const PkLayout& L, PkArgs a, const int2* 
                       uint32_t* d_msg, hipStream_t origin, const hipStream_t* aux, hipEvent_t fork,
                       const hipEvent_t* join, const hipEvent_t* tev, bool concurrent, bool nms)
 {
-    hipLaunchKernelGGL(reset_ticket, dim3(1), dim3(NCOUNTERS), 0, origin, d_cnt);
-    HIPCHK(c, hipGetLastError());
-    if (concurrent)
-        HIPCHK(c, hipEventRecord(fork, origin));
+    /* the fork event also starts the decode's timing (launch_pk), so it is
+     * recorded in both modes */
+    HIPCHK(c, hipEventRecord(fork, origin));
+    HIPCHK(c, reset_counters(origin, d_cnt));
     a.msg_stride = (size_t)L.maxw;
     size_t moff = 0;
     for (int k = 0; k < NLAUNCH; k++) {
@@ -1429,24 +773,15 @@ static int pk_enqueue(ldpc_gpu_ctx* c, const PkLayout& L, PkArgs a, const int2* 
         a.cnt = d_cnt + L.seg0[k];
         a.msg = d_msg + moff;
         moff += (size_t)L.grid[k] * L.maxw;
-        const size_t lds = 2 * (size_t)L.maxn[k];
+        a.hoff = pk_hoff(L, k);
+        a.hstride = 360 * L.maxx[k];
+        const size_t lds = pk_lds(L, k);
         hipStream_t st = concurrent ? aux[k] : origin;
         if (concurrent)
             HIPCHK(c, hipStreamWaitEvent(st, fork, 0));
         if (tev)
             HIPCHK(c, hipEventRecord(tev[2 * k], st));
-        if (nms) {
-            if (k < 2)
-                hipLaunchKernelGGL((ldpc_pk_kernel<0, true>), dim3(L.grid[k]), dim3(PK_NT), lds, st, a);
-            else
-                hipLaunchKernelGGL((ldpc_pk_kernel<1, true>), dim3(L.grid[k]), dim3(PK_NT), lds, st, a);
-        } else {
-            if (k < 2)
-                hipLaunchKernelGGL((ldpc_pk_kernel<0, false>), dim3(L.grid[k]), dim3(PK_NT), lds, st, a);
-            else
-                hipLaunchKernelGGL((ldpc_pk_kernel<1, false>), dim3(L.grid[k]), dim3(PK_NT), lds, st, a);
-        }
-        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, pk_launch(k / 2, nms, L.grid[k], lds, st, a));
         if (tev)
             HIPCHK(c, hipEventRecord(tev[2 * k + 1], st));
         if (concurrent) {
@@ -1463,9 +798,10 @@ static PkArgs pk_args(ldpc_gpu_ctx* c, const DFrame* d_frames, const int* d_list
     PkArgs a;
     memset(&a, 0, sizeof(a));
     a.codes = c->tab->codes;
-    a.ent = c->tab->ent;
-    a.lay = c->tab->lay;
-    a.pkoff = c->tab->pkoff;
+    a.ent = c->tab->xent;
+    a.pos = c->tab->pos;
+    a.pkl = c->tab->pkl;
+    a.adt = c->tab->adt;
     a.meta = c->tab->meta;
     a.frames = d_frames;
     a.list = d_list;
@@ -1507,7 +843,7 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
     for (int k = 0; k < NLAUNCH; k++) {
         join[k] = c->ev_cls[2 * k + 1];
         c->cls_frames[k] = L.nfr[k];
-        c->cls_di[k] = CLASS_DI[k / 2];
+        c->cls_di[k] = PK_DM[k / 2];
     }
     if ((rc = pk_enqueue(c, L, a, c->d_seg, c->ticket, c->msg, c->stream, c->aux, c->ev0, join, c->ev_cls, !c->serial,
                          nms)))

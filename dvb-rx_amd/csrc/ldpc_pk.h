@@ -15,14 +15,22 @@
  * except after a positive saturation (0xff); such values only occur at
  * |y| = 127 where they cannot change a result (see pk_out).
  *
- * Messages: one int8 per edge per slot, two edges a word
- * [e0 slot0, e0 slot1, e1 slot0, e1 slot1], edge e in word e / 2.
+ * Edges of a node by position: 0 its own parity bit, 1 the previous parity
+ * bit (the IRA staircase, ldpc.c:277-284), 2 + e information edge e of the
+ * layer.  Messages: one int8 per edge per slot, two positions a word
+ * [p0 slot0, p0 slot1, p1 slot0, p1 slot1], position p in word p / 2.
+ *
+ * The node routines are templates on DM, the largest information degree
+ * a kernel instance handles; a layer's own degree D <= DM is a run-time
+ * value, the same for all nodes of the layer (edges e >= D are skipped by
+ * uniform branches, so they cost no vector instructions).
  */
 #ifndef LDPC_PK_H
 #define LDPC_PK_H
 
 #include <stdint.h>
 
+#include "ldpc_codes.h"
 #include "ldpc_node.h"
 
 typedef short lg_pk __attribute__((ext_vector_type(2)));
@@ -82,32 +90,34 @@ LG_PK_HD uint32_t pk_msg_sel(int odd, int first0, int first1)
 
 #define LG_PK_WORDS(deg) (((deg) + 2 + 1) / 2)
 
-template <int DEG>
+template <int DM>
 struct PkNode {
-    int ad[DEG];       /* posterior index (variable) of each information edge */
-    lg_pk x[DEG + 2];  /* extrinsic inputs, both slots */
-    lg_pk a[DEG + 2];  /* their magnitudes */
-    int apc, app, has_pp;
+    int ad[DM];       /* posterior index (variable) of each information edge */
+    lg_pk x[DM + 2];  /* extrinsic inputs by position, both slots */
+    lg_pk a[DM + 2];  /* their magnitudes */
+    int apc, app;
+    uint32_t nopp;    /* all ones for node (0, 0), which has no previous parity */
     lg_pk min0, min1, s;
 };
 
-/* posterior index of node j's edge on table entry en (base << 16 | shift):
- * base + (j - shift) mod 360, the wrap as an unsigned minimum */
+/* posterior index of node j's edge on table entry en (base << 16 | shift
+ * in the low 9 bits): base + (j - shift) mod 360, the wrap as an unsigned
+ * minimum */
 LG_PK_HD int pk_addr(uint32_t en, int j)
 {
-    const uint32_t t = (uint32_t)j - (en & 0xffff);
+    const uint32_t t = (uint32_t)j - (en & 0x1ffu);
     const uint32_t u = t + 360u;
     return (int)(en >> 16) + (int)(t < u ? t : u);
 }
 
 LG_PK_HD lg_pk pk_abs(lg_pk x) { return pk_max(x, pk_subs(pk_dup(0), x)); }
 
-template <int DEG>
-LG_PK_HD void pk_fold(PkNode<DEG>& n, int e, lg_pk x)
+template <int DM>
+LG_PK_HD void pk_fold(PkNode<DM>& n, int p, lg_pk x)
 {
     const lg_pk a = pk_abs(x);
-    n.x[e] = x;
-    n.a[e] = a;
+    n.x[p] = x;
+    n.a[p] = a;
     n.min1 = pk_minu(n.min1, pk_maxu(n.min0, a));
     n.min0 = pk_minu(n.min0, a);
     n.s = n.s ^ x;
@@ -117,80 +127,98 @@ LG_PK_HD void pk_fold(PkNode<DEG>& n, int e, lg_pk x)
  * the minimums and the sign as they were (|x| | 0xffff is above any
  * minimum).  Branch-free, so the posterior reads of all edges can be in
  * flight together. */
-template <int DEG>
-LG_PK_HD void pk_fold_skip(PkNode<DEG>& n, int e, lg_pk x, uint32_t skip)
+template <int DM>
+LG_PK_HD void pk_fold_skip(PkNode<DM>& n, int p, lg_pk x, uint32_t skip)
 {
     const lg_pk a = pk_of(pk_bits(pk_abs(x)) | skip);
-    n.x[e] = x;
-    n.a[e] = a;
+    n.x[p] = x;
+    n.a[p] = a;
     n.min1 = pk_minu(n.min1, pk_maxu(n.min0, a));
     n.min0 = pk_minu(n.min0, a);
     n.s = pk_of(pk_bits(n.s) ^ (pk_bits(x) & ~skip));
 }
 
-LG_PK_HD lg_pk pk_in(const uint16_t* post, int v, const uint32_t* mw, int e, uint32_t sel_even, uint32_t sel_odd)
+/* x = sat(L - m) of position p from the two posterior bytes w */
+LG_PK_HD lg_pk pk_in_w(uint32_t w, const uint32_t* mw, int p, uint32_t sel_even, uint32_t sel_odd)
 {
-    const lg_pk L = pk_of(lg_perm(0, post[v], PK_UNPACK_LO));
-    const lg_pk M = pk_of(lg_perm(0, mw[e >> 1], (e & 1) ? sel_odd : sel_even));
+    const lg_pk L = pk_of(lg_perm(0, w, PK_UNPACK_LO));
+    const lg_pk M = pk_of(lg_perm(0, mw[p >> 1], (p & 1) ? sel_odd : sel_even));
     return pk_subs(L, M);
 }
 
-/* TAIL: the layer has dependent nodes (dep masks); without, every edge is
- * read in phase A and no compile-time-unknown mask is tested */
-template <int DEG, bool TAIL>
-LG_PK_HD void pk_phase_a(PkNode<DEG>& n, const uint16_t* post, const uint32_t* ent, int i, int j, int K, int q,
-                         uint32_t dep, const uint32_t* mw, uint32_t sel_even, uint32_t sel_odd)
+LG_PK_HD lg_pk pk_in(const uint16_t* post, int v, const uint32_t* mw, int p, uint32_t sel_even, uint32_t sel_odd)
+{
+    return pk_in_w(post[v], mw, p, sel_even, sel_odd);
+}
+
+/* Phase A of node (i, j) in a layer of D information edges: all edges
+ * read, minimums and sign formed.  TAIL (the layer has nodes sharing
+ * bits): the edges in dep (read after an earlier node of the layer) stay
+ * out of the minimums; the hand-over rounds fold them. */
+/* information-edge posterior indices of node j in a layer from its table
+ * entries (the kernel reads them precomputed, pk_addr_words) */
+template <int DM>
+LG_PK_HD void pk_addrs(int* ad, const uint32_t* ent, int D, int j)
 {
 #pragma unroll
-    for (int e = 0; e < DEG; e++)
-        n.ad[e] = pk_addr(ent[e], j);
-    n.apc = K + 360 * i + j;
-    n.has_pp = (i | j) != 0;
-    n.app = i ? n.apc - 360 : K + 360 * (q - 1) + j - 1;
-    n.min0 = pk_dup(-1);
-    n.min1 = pk_dup(-1);
-    n.s = pk_dup(0);
-#pragma unroll
-    for (int e = 0; e < DEG; e++) {
-        const lg_pk x = pk_in(post, n.ad[e], mw, e, sel_even, sel_odd);
-        if (TAIL) /* dependent edges are folded by pk_step */
-            pk_fold_skip(n, e, x, 0u - ((dep >> e) & 1));
-        else
-            pk_fold(n, e, x);
-    }
-    pk_fold(n, DEG, pk_in(post, n.apc, mw, DEG, sel_even, sel_odd));
-    if (n.has_pp) {
-        pk_fold(n, DEG + 1, pk_in(post, n.app, mw, DEG + 1, sel_even, sel_odd));
-    } else {
-        n.x[DEG + 1] = pk_dup(0);
-        n.a[DEG + 1] = pk_dup(0);
+    for (int e = 0; e < DM; e++) {
+        if (e >= D)
+            continue;
+        ad[e] = pk_addr(ent[e], j);
     }
 }
 
-template <int DEG>
-LG_PK_HD void pk_step(PkNode<DEG>& n, const uint16_t* post, uint32_t dep, const uint32_t* mw, uint32_t sel_even,
-                      uint32_t sel_odd)
+/* the same from the precomputed rows: two indices a word, edge 2w in the
+ * low half of word w */
+template <int DM>
+LG_PK_HD void pk_addr_words(int* ad, const uint32_t* aw, int D)
 {
 #pragma unroll
-    for (int e = 0; e < DEG; e++)
-        if ((dep >> e) & 1)
-            pk_fold(n, e, pk_in(post, n.ad[e], mw, e, sel_even, sel_odd));
+    for (int e = 0; e < DM; e++) {
+        if (e >= D)
+            continue;
+        ad[e] = (int)((e & 1) ? aw[e >> 1] >> 16 : aw[e >> 1] & 0xffffu);
+    }
+}
+
+template <int DM, bool TAIL>
+LG_PK_HD void pk_phase_a(PkNode<DM>& n, const uint16_t* post, int D, int i, int j, int K, int q, uint32_t dep,
+                         const uint32_t* mw, uint32_t sel_even, uint32_t sel_odd)
+{
+    n.apc = K + 360 * i + j;
+    n.nopp = (i | j) ? 0u : ~0u;
+    /* node (0, 0) reads its own parity bit as a stand-in, never used */
+    n.app = i ? n.apc - 360 : j ? K + 360 * (q - 1) + j - 1 : n.apc;
+    n.min0 = pk_dup(-1);
+    n.min1 = pk_dup(-1);
+    n.s = pk_dup(0);
+    pk_fold(n, 0, pk_in(post, n.apc, mw, 0, sel_even, sel_odd));
+    pk_fold_skip(n, 1, pk_in(post, n.app, mw, 1, sel_even, sel_odd), n.nopp);
+#pragma unroll
+    for (int e = 0; e < DM; e++) {
+        if (!(e >= D)) {
+            const lg_pk x = pk_in(post, n.ad[e], mw, e + 2, sel_even, sel_odd);
+            if (TAIL)
+                pk_fold_skip(n, e + 2, x, 0u - ((dep >> e) & 1));
+            else
+                pk_fold(n, e + 2, x);
+        }
+    }
 }
 
 /* The offset magnitudes of a complete node, for both slots: max(min(m, 127)
  * - 1, 0) in units of 256 (ldpc.c:353).  min(m, 127*256) also removes a
  * 0xff low byte, so mn0 / mn1 are exact multiples of 256. */
-template <int DEG>
 struct PkOut {
     lg_pk mn1, dmn;
 };
 
-template <int DEG>
-LG_PK_HD PkOut<DEG> pk_mags(const PkNode<DEG>& n)
+template <int DM>
+LG_PK_HD PkOut pk_mags(const PkNode<DM>& n)
 {
     const lg_pk mn0 = pk_max(pk_minu(n.min0, pk_dup(0x7F00)) - pk_dup(0x0100), pk_dup(0));
     const lg_pk mn1 = pk_max(pk_minu(n.min1, pk_dup(0x7F00)) - pk_dup(0x0100), pk_dup(0));
-    return PkOut<DEG>{ mn1, mn0 - mn1 };
+    return PkOut{ mn1, mn0 - mn1 };
 }
 
 /* Learned magnitude of one minimum, per lane: clamp((min(m, 127) * S - B)
@@ -204,12 +232,12 @@ LG_PK_HD lg_pk pk_nms_mag(lg_pk m, lg_pk ps, lg_pk pb)
     return pk_min(pk_max(p, pk_dup(0)), pk_dup(127)) << (short)8;
 }
 
-template <int DEG, bool NMS>
-LG_PK_HD PkOut<DEG> pk_mags_p(const PkNode<DEG>& n, lg_pk ps, lg_pk pb)
+template <int DM, bool NMS>
+LG_PK_HD PkOut pk_mags_p(const PkNode<DM>& n, lg_pk ps, lg_pk pb)
 {
     if constexpr (NMS) {
         const lg_pk mn0 = pk_nms_mag(n.min0, ps, pb), mn1 = pk_nms_mag(n.min1, ps, pb);
-        return PkOut<DEG>{ mn1, mn0 - mn1 };
+        return PkOut{ mn1, mn0 - mn1 };
     } else {
         (void)ps;
         (void)pb;
@@ -217,10 +245,6 @@ LG_PK_HD PkOut<DEG> pk_mags_p(const PkNode<DEG>& n, lg_pk ps, lg_pk pb)
     }
 }
 
-/* o_e: |o| = mn1 if |x_e| is the smallest input, else mn0; sign = product
- * of the other signs.  a >= min0 always, so a - min0 is 0 exactly for the
- * smallest input.  Where a 0xff low byte makes |x_e| differ from min0 by
- * less than 256, both are 127 and mn0 == mn1 (both minimums >= 127). */
 /* min(d, 1) * dmn + mn1 per lane as two instructions: written out so that
  * the compiler does not turn them into per-lane compares and selects */
 LG_PK_HD lg_pk pk_sel_mag(lg_pk d, lg_pk dmn, lg_pk mn1)
@@ -236,64 +260,70 @@ LG_PK_HD lg_pk pk_sel_mag(lg_pk d, lg_pk dmn, lg_pk mn1)
 #endif
 }
 
-template <int DEG>
-LG_PK_HD lg_pk pk_out(const PkNode<DEG>& n, const PkOut<DEG>& m, int e)
+/* o_p: |o| = mn1 if |x_p| is the smallest input, else mn0; sign = product
+ * of the other signs.  a >= min0 always, so a - min0 is 0 exactly for the
+ * smallest input.  Where a 0xff low byte makes |x_p| differ from min0 by
+ * less than 256, both are 127 and mn0 == mn1 (both minimums >= 127). */
+template <int DM>
+LG_PK_HD lg_pk pk_out(const PkNode<DM>& n, const PkOut& m, int p)
 {
-    const lg_pk mag = pk_sel_mag(n.a[e] - n.min0, m.dmn, m.mn1);
-    const lg_pk t = (n.s ^ n.x[e]) >> 15;
+    const lg_pk mag = pk_sel_mag(n.a[p] - n.min0, m.dmn, m.mn1);
+    const lg_pk t = (n.s ^ n.x[p]) >> 15;
     return (mag ^ t) - t;
 }
 
 LG_PK_HD uint16_t pk_post_bytes(lg_pk y) { return (uint16_t)lg_perm(0, pk_bits(y), PK_PACK_POST); }
 
-template <int DEG, bool NMS = false>
-LG_PK_HD void pk_write_succ(const PkNode<DEG>& n, uint16_t* post, uint32_t succ, lg_pk ps = lg_pk{ 16, 16 },
-                            lg_pk pb = lg_pk{ 16, 16 })
-{
-    const PkOut<DEG> m = pk_mags_p<DEG, NMS>(n, ps, pb);
-#pragma unroll
-    for (int e = 0; e < DEG; e++)
-        if ((succ >> e) & 1)
-            post[n.ad[e]] = pk_post_bytes(pk_adds(n.x[e], pk_out(n, m, e)));
-}
+LG_PK_HD lg_pk pk_msg_clamp(lg_pk o) { return pk_min(pk_max(o, pk_dup(-32 * 256)), pk_dup(31 * 256)); }
 
-/* final posteriors (bits this node touches last) and the new messages,
- * clamped to [-32, 31] (ldpc.c:343) */
-template <int DEG, bool TAIL, bool NMS = false>
-LG_PK_HD void pk_phase_c(const PkNode<DEG>& n, uint16_t* post, uint32_t succ, uint32_t* mw_out,
-                         lg_pk ps = lg_pk{ 16, 16 }, lg_pk pb = lg_pk{ 16, 16 })
+/* Phase C: final posteriors (the bits this node touches last; TAIL: the
+ * edges in succ are written by a later node of the layer) and the new
+ * messages, clamped to [-32, 31] (ldpc.c:343): word w of the node's
+ * LG_PK_WORDS(D) goes to mw_out[stride * w] as soon as both its positions
+ * are known */
+template <int DM, bool TAIL, bool NMS>
+LG_PK_HD void pk_phase_c(const PkNode<DM>& n, uint16_t* post, int D, uint32_t succ, uint32_t* mw_out, int stride,
+                         lg_pk ps, lg_pk pb)
 {
-    const PkOut<DEG> m = pk_mags_p<DEG, NMS>(n, ps, pb);
-    lg_pk mo[DEG + 2];
-#pragma unroll
-    for (int e = 0; e < DEG + 2; e++) {
-        if (e == DEG + 1 && !n.has_pp) {
-            mo[e] = pk_dup(0);
-            continue;
-        }
-        const lg_pk o = pk_out(n, m, e);
-        if (e >= DEG || !TAIL || !((succ >> e) & 1)) {
-            const int v = e < DEG ? n.ad[e] : e == DEG ? n.apc : n.app;
-            post[v] = pk_post_bytes(pk_adds(n.x[e], o));
-        }
-        mo[e] = pk_min(pk_max(o, pk_dup(-32 * 256)), pk_dup(31 * 256));
+    const PkOut m = pk_mags_p<DM, NMS>(n, ps, pb);
+    {
+        const lg_pk o0 = pk_out(n, m, 0);
+        post[n.apc] = pk_post_bytes(pk_adds(n.x[0], o0));
+        const lg_pk o1 = pk_out(n, m, 1);
+        if (!n.nopp)
+            post[n.app] = pk_post_bytes(pk_adds(n.x[1], o1));
+        mw_out[0] = lg_perm(pk_bits(pk_msg_clamp(o1)) & ~n.nopp, pk_bits(pk_msg_clamp(o0)), PK_PACK_MSG);
     }
+    uint32_t even = 0;
 #pragma unroll
-    for (int w = 0; w < LG_PK_WORDS(DEG); w++) {
-        const lg_pk odd = 2 * w + 1 < DEG + 2 ? mo[2 * w + 1] : pk_dup(0);
-        mw_out[w] = lg_perm(pk_bits(odd), pk_bits(mo[2 * w]), PK_PACK_MSG);
+    for (int e = 0; e < DM; e++) {
+        if (e >= D)
+            continue;
+        const lg_pk o = pk_out(n, m, e + 2);
+        if (!TAIL || !((succ >> e) & 1))
+            post[n.ad[e]] = pk_post_bytes(pk_adds(n.x[e + 2], o));
+        const uint32_t c = pk_bits(pk_msg_clamp(o));
+        if (!(e & 1)) {
+            even = c;
+            if (e + 1 == D)
+                mw_out[stride * (e / 2 + 1)] = lg_perm(0, even, PK_PACK_MSG);
+        } else {
+            mw_out[stride * (e / 2 + 1)] = lg_perm(c, even, PK_PACK_MSG);
+        }
     }
 }
 
 /* parity check of node (i, j) in both slots: bit k set if slot k holds it
  * (no zero posterior, even number of negative ones; bad(), ldpc.c:286-305).
  * A byte is non-zero iff bit 7 of ((b & 0x7f) + 0x7f) | b is set. */
-template <int DEG>
-LG_PK_HD uint32_t pk_check(const uint16_t* post, const uint32_t* ent, int i, int j, int K, int q)
+template <int DM>
+LG_PK_HD uint32_t pk_check(const uint16_t* post, const uint32_t* ent, int D, int i, int j, int K, int q)
 {
     uint32_t par = 0, nz = 0x8080;
 #pragma unroll
-    for (int e = 0; e < DEG; e++) {
+    for (int e = 0; e < DM; e++) {
+        if (e >= D)
+            continue;
         const uint32_t r = post[pk_addr(ent[e], j)];
         par ^= r;
         nz &= ((r & 0x7F7F) + 0x7F7F) | r;
@@ -309,6 +339,99 @@ LG_PK_HD uint32_t pk_check(const uint16_t* post, const uint32_t* ent, int i, int
     }
     const uint32_t good = nz & ~par & 0x8080;
     return ((good >> 7) & 1) | ((good >> 14) & 2);
+}
+
+/* ---- layers with nodes sharing a bit: rounds to a fixed point ----------
+ *
+ * The reference updates the nodes of a layer one after another
+ * (ldpc.c:474-480): a node that shares an information bit with earlier
+ * nodes of the layer reads it as the latest of them left it.  Here all
+ * nodes of such a layer run at once, in rounds.  Round 0 takes every bit
+ * as the layer found it; in round r a node reads each shared bit as its
+ * predecessor on that bit wrote it in round r - 1 (hand-over slots,
+ * double-buffered) and writes what it hands on to its successors.  A node
+ * whose inputs are exact after round r - 1 is exact after round r, so all
+ * are exact after (levels - 1) rounds; and once a round reads the same
+ * inputs as the round before, every later round would too: that state is
+ * the sequential result (the hand-over graph has no cycle, so its fixed
+ * point is unique).  The rounds stop at the first of the two.
+ *
+ * Hand-over slot of edge e at rank r (its node is the r-th of the layer
+ * touching the bit, 0-based): the bit is handed from rank r to r + 1
+ * through slot (t0 + r) * 360 + m, m the bit's index in its group. */
+struct PkPart {
+    lg_pk min0, min1, s;
+};
+
+template <int DM>
+LG_PK_HD PkPart pk_part(const PkNode<DM>& n)
+{
+    return PkPart{ n.min0, n.min1, n.s };
+}
+
+template <int DM>
+LG_PK_HD void pk_restore(PkNode<DM>& n, const PkPart& p)
+{
+    n.min0 = p.min0;
+    n.min1 = p.min1;
+    n.s = p.s;
+}
+
+/* round 0: the dependent edges with the values phase A read */
+template <int DM>
+LG_PK_HD void pk_fold_dep0(PkNode<DM>& n, int D, uint32_t dep)
+{
+#pragma unroll
+    for (int e = 0; e < DM; e++) {
+        if (e >= D)
+            continue;
+        if ((dep >> e) & 1)
+            pk_fold(n, e + 2, n.x[e + 2]);
+    }
+}
+
+LG_PK_HD int pk_slot(uint32_t xe, int ad, int r) { return (LG_XENT_T0(xe) + r) * 360 + ad - (int)(xe >> 16); }
+
+/* write the values handed to the successors (edges in succ) */
+template <int DM, bool NMS>
+LG_PK_HD void pk_put_hand(const PkNode<DM>& n, uint16_t* hb, const uint32_t* xent, int D, uint32_t dupm, uint32_t succ,
+                          uint64_t pos, lg_pk ps, lg_pk pb)
+{
+    const PkOut m = pk_mags_p<DM, NMS>(n, ps, pb);
+#pragma unroll
+    for (int e = 0; e < DM; e++) {
+        if (e >= D)
+            continue;
+        if (!((dupm >> e) & 1)) /* the same for every node of the layer */
+            continue;
+        if ((succ >> e) & 1)
+            hb[pk_slot(xent[e], n.ad[e], (int)((pos >> (2 * e)) & 3))] =
+                pk_post_bytes(pk_adds(n.x[e + 2], pk_out(n, m, e + 2)));
+    }
+}
+
+/* take the values handed over by the predecessors (edges in dep), fold
+ * them into the minimums restored from the other edges; returns non-zero
+ * if an input differs from the previous round's */
+template <int DM>
+LG_PK_HD uint32_t pk_take_hand(PkNode<DM>& n, const uint16_t* hb, const uint32_t* xent, int D, uint32_t dupm,
+                               uint32_t dep, uint64_t pos, const uint32_t* mw, uint32_t sel_even, uint32_t sel_odd)
+{
+    uint32_t chg = 0;
+#pragma unroll
+    for (int e = 0; e < DM; e++) {
+        if (e >= D)
+            continue;
+        if (!((dupm >> e) & 1))
+            continue;
+        if ((dep >> e) & 1) {
+            const int r = (int)((pos >> (2 * e)) & 3);
+            const lg_pk x = pk_in_w(hb[pk_slot(xent[e], n.ad[e], r - 1)], mw, e + 2, sel_even, sel_odd);
+            chg |= pk_bits(x) ^ pk_bits(n.x[e + 2]);
+            pk_fold(n, e + 2, x);
+        }
+    }
+    return chg;
 }
 
 #endif

@@ -89,6 +89,7 @@ def gpu_lib():
         L.ldpc_gpu_last_kernel_ms.restype = C.c_float
         L.ldpc_gpu_last_launches.argtypes = [vp, ip, ip, C.POINTER(C.c_float), i32]
         L.ldpc_gpu_code_max_degree.argtypes = [i32]
+        L.ldpc_gpu_code_kernel_degree.argtypes = [i32]
         L.ldpc_gpu_set_kernel.argtypes = [vp, i32]
         L.ldpc_gpu_set_nms.argtypes = [vp, i32, i32, vp, vp]
         L.ldpc_gpu_plan_create.argtypes = [vp, vp, i32, vp, vp, vp, vp, C.POINTER(vp)]
@@ -172,7 +173,7 @@ def code_params(code):
     n, k, q, lt, steps = [x.value for x in v]
     dmax = gpu_lib().ldpc_gpu_code_max_degree(code)
     return dict(n=n, k=k, q=q, r=n - k, links_total=lt, level_steps=steps, max_degree=dmax,
-                degree_class=12 if dmax <= 12 else 28)
+                degree_class=gpu_lib().ldpc_gpu_code_kernel_degree(code))
 
 
 def pls_lookup(pls):
@@ -334,11 +335,11 @@ class Decoder:
         return self.L.ldpc_gpu_last_kernel_ms(self.ctx)
 
     def last_launches(self):
-        """[(degree class bound, frames, ms)] of the last decode's launches."""
-        di = (C.c_int * 8)()
-        nf = (C.c_int * 8)()
-        ms = (C.c_float * 8)()
-        n = self.L.ldpc_gpu_last_launches(self.ctx, di, nf, ms, 8)
+        """[(kernel instance degree bound, frames, ms)] of the last decode's launches."""
+        di = (C.c_int * 16)()
+        nf = (C.c_int * 16)()
+        ms = (C.c_float * 16)()
+        n = self.L.ldpc_gpu_last_launches(self.ctx, di, nf, ms, 16)
         return [(di[k], nf[k], ms[k]) for k in range(n)]
 
     def decode_host(self, frames, llr, out_bytes):

@@ -61,6 +61,11 @@ int ldpc_gpu_code_lookup(int short_frame, const char* rate);
 int ldpc_gpu_code_params(int code, int* n, int* k, int* q, int* links_total, int* level_steps);
 /* Largest information degree of a layer of the code (check degree - 2). */
 int ldpc_gpu_code_max_degree(int code);
+/* Layer degree bound of the kernel instance that decodes the code with
+ * per-frame early stop (4, 8, 12, 16, 20 or 28; the key of the launches
+ * ldpc_gpu_last_launches reports); LDPC_GPU_EINVAL for an unknown code.
+ * Diagnostics only, no counterpart in the reference. */
+int ldpc_gpu_code_kernel_degree(int code);
 int ldpc_gpu_fec_lookup(int short_frame, const char* rate, int* nldpc, int* kldpc, int* kbch, int* t);
 
 /* Device memory helpers (synchronous copies on the context's stream). */
@@ -261,10 +266,11 @@ int ldpc_gpu_set_concurrent(ldpc_gpu_ctx* ctx, int concurrent);
 float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* ctx);
 
 /* Per-launch detail of the last ldpc_gpu_decode_batch: frames are decoded
- * by one persistent kernel launch per degree class (largest information
- * degree of a layer <= 12 or <= 28) and frame size.  Fills up to max
- * entries with the class bound, the frames of the launch and its time in
- * ms; returns the count. */
+ * by one persistent kernel launch per kernel instance (the layer degree
+ * bound of ldpc_gpu_code_kernel_degree; for frames in groups, the one-frame
+ * kernel's classes 12 and 28) and frame size.  Fills up to max entries
+ * with that bound, the frames of the launch and its time in ms; returns
+ * the count. */
 int ldpc_gpu_last_launches(ldpc_gpu_ctx* ctx, int* di, int* nframes, float* ms, int max);
 
 #ifdef __cplusplus

@@ -145,15 +145,25 @@ struct PkRows {
     const int16_t* B[2];
 };
 
+/* hand-over rounds run and level steps they replace (statistics) */
+static long g_pk_rounds, g_pk_levels;
+
+extern "C" void kmodel_pk_rounds(long* rounds, long* levels)
+{
+    *rounds = g_pk_rounds;
+    *levels = g_pk_levels;
+}
+
+/* DEG: the kernel instance's layer degree bound (DM), the layer's own
+ * degree D <= DEG is a run-time value as in the kernel */
 template <int DEG, bool NMS>
 static void pk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t sel_even,
                            uint32_t sel_odd, const PkRows& rows)
 {
     const int K = c->k, q = c->q;
-    const uint32_t* ent = c->ent + c->layer_start[i];
     const uint64_t* meta = c->meta + (size_t)i * 360;
     static PkNode<DEG> nodes[360];
-    const int W = LG_PK_WORDS(DEG);
+    const int W = LG_PK_WORDS(c->layer_start[i + 1] - c->layer_start[i]);
     static uint32_t mw[360][LG_PK_WORDS(28)];
     uint32_t* m = msg + (size_t)c->pk_msg_off[i] * 360;
     lg_pk ps[360], pb[360];
@@ -163,41 +173,77 @@ static void pk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
         ps[j] = lg_pk{ 16, 16 };
         pb[j] = lg_pk{ 16, 16 };
         if (NMS) {
-            const int dc = DEG + 2 - (i == 0 && j == 0);
+            const int dc = c->layer_start[i + 1] - c->layer_start[i] + 2 - (i == 0 && j == 0);
             ps[j] = lg_pk{ (short)rows.S[0][dc], (short)rows.S[1][dc] };
             pb[j] = lg_pk{ (short)rows.B[0][dc], (short)rows.B[1][dc] };
         }
     }
+    const uint32_t* xent = c->xent + c->layer_start[i];
+    const uint64_t* pos = c->pos + (size_t)i * 360;
+    const uint32_t dupm = c->dupm[i];
+    const int D = c->layer_start[i + 1] - c->layer_start[i];
     for (int j = 359; j >= 0; j--) {
+        pk_addrs<DEG>(nodes[j].ad, xent, D, j);
         if (c->nlev[i] > 1)
-            pk_phase_a<DEG, true>(nodes[j], post, ent, i, j, K, q, LG_META_DEP(meta[j]), mw[j], sel_even, sel_odd);
+            pk_phase_a<DEG, true>(nodes[j], post, D, i, j, K, q, LG_META_DEP(meta[j]), mw[j], sel_even, sel_odd);
         else
-            pk_phase_a<DEG, false>(nodes[j], post, ent, i, j, K, q, 0, mw[j], sel_even, sel_odd);
-        if (LG_META_LEVEL(meta[j]) == 0)
-            pk_write_succ<DEG, NMS>(nodes[j], post, LG_META_SUCC(meta[j]), ps[j], pb[j]);
+            pk_phase_a<DEG, false>(nodes[j], post, D, i, j, K, q, 0, mw[j], sel_even, sel_odd);
     }
-    for (int l = 1; l < c->nlev[i]; l++)
-        for (int j = 359; j >= 0; j--)
-            if (LG_META_LEVEL(meta[j]) == l) {
-                pk_step<DEG>(nodes[j], post, LG_META_DEP(meta[j]), mw[j], sel_even, sel_odd);
-                pk_write_succ<DEG, NMS>(nodes[j], post, LG_META_SUCC(meta[j]), ps[j], pb[j]);
+    /* hand-over rounds as the kernel runs them: all nodes of a round
+     * together (here in decreasing j), two buffers, stop at the first
+     * round that reads what the round before read, or after levels - 1 */
+    if (c->nlev[i] > 1) {
+        static PkPart part[360];
+        static uint16_t hb[2][LG_MAX_XFER * 360];
+        for (int j = 359; j >= 0; j--) {
+            part[j] = pk_part(nodes[j]);
+            pk_fold_dep0<DEG>(nodes[j], D, LG_META_DEP(meta[j]));
+            pk_put_hand<DEG, NMS>(nodes[j], hb[0], xent, D, dupm, LG_META_SUCC(meta[j]), pos[j], ps[j], pb[j]);
+        }
+        for (int r = 1; r < c->nlev[i]; r++) {
+            uint32_t chg = 0;
+            for (int j = 359; j >= 0; j--) {
+                pk_restore(nodes[j], part[j]);
+                chg |= pk_take_hand<DEG>(nodes[j], hb[(r - 1) & 1], xent, D, dupm, LG_META_DEP(meta[j]), pos[j],
+                                         mw[j], sel_even, sel_odd);
             }
+            g_pk_rounds++;
+            if (r == c->nlev[i] - 1)
+                break;
+            for (int j = 359; j >= 0; j--)
+                pk_put_hand<DEG, NMS>(nodes[j], hb[r & 1], xent, D, dupm, LG_META_SUCC(meta[j]), pos[j], ps[j],
+                                      pb[j]);
+            if (!chg)
+                break;
+        }
+        g_pk_levels += c->nlev[i] - 1;
+    }
     for (int j = 359; j >= 0; j--) {
-        uint32_t o[LG_PK_WORDS(DEG)];
         if (c->nlev[i] > 1)
-            pk_phase_c<DEG, true, NMS>(nodes[j], post, LG_META_SUCC(meta[j]), o, ps[j], pb[j]);
+            pk_phase_c<DEG, true, NMS>(nodes[j], post, D, LG_META_SUCC(meta[j]), m + j, 360, ps[j], pb[j]);
         else
-            pk_phase_c<DEG, false, NMS>(nodes[j], post, 0, o, ps[j], pb[j]);
-        for (int w = 0; w < W; w++)
-            m[w * 360 + j] = o[w];
+            pk_phase_c<DEG, false, NMS>(nodes[j], post, D, 0, m + j, 360, ps[j], pb[j]);
     }
 }
+
+/* the kernel instance of the code: the smallest layer degree bound that
+ * covers its layers (the library's choice, 4 8 12 16 20 28) */
+static int pk_dm(const lg_code* c)
+{
+    static const int dms[] = { 4, 8, 12, 16, 20, 28 };
+    for (int d : dms)
+        if (c->max_layer_deg <= d)
+            return d;
+    return -1;
+}
+
+#define PK_DM_CASES(X) X(4) X(8) X(12) X(16) X(20) X(28)
 
 static int pk_layer_any(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t se, uint32_t so,
                         const PkRows* rows)
 {
     static const PkRows none = { { nullptr, nullptr }, { nullptr, nullptr } };
-    switch (c->layer_start[i + 1] - c->layer_start[i]) {
+    switch (pk_dm(c)) {
 #define X(D)                                                         \
     case D:                                                          \
         if (rows)                                                    \
@@ -205,7 +251,7 @@ static int pk_layer_any(const lg_code* c, int i, uint16_t* post, uint32_t* msg, 
         else                                                         \
             pk_model_layer<D, false>(c, i, post, msg, se, so, none); \
         return 0;
-        DEG_CASES(X)
+        PK_DM_CASES(X)
 #undef X
     }
     return -1;
@@ -213,12 +259,13 @@ static int pk_layer_any(const lg_code* c, int i, uint16_t* post, uint32_t* msg, 
 
 static uint32_t pk_check_any(const lg_code* c, int i, int j, const uint16_t* post)
 {
-    const uint32_t* e = c->ent + c->layer_start[i];
-    switch (c->layer_start[i + 1] - c->layer_start[i]) {
-#define X(D) \
-    case D:  \
-        return pk_check<D>(post, e, i, j, c->k, c->q);
-        DEG_CASES(X)
+    const uint32_t* e = c->xent + c->layer_start[i];
+    const int D = c->layer_start[i + 1] - c->layer_start[i];
+    switch (pk_dm(c)) {
+#define X(DMX) \
+    case DMX:  \
+        return pk_check<DMX>(post, e, D, i, j, c->k, c->q);
+        PK_DM_CASES(X)
 #undef X
     }
     return 0;
