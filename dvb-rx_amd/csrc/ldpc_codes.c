@@ -16,6 +16,86 @@ static lg_code g_codes[sizeof(LG_CODES) / sizeof(*LG_CODES)];
 static pthread_once_t g_once = PTHREAD_ONCE_INIT;
 static int g_build_failed;
 
+/* Split-kernel tables (lg_code.sk_*), from the expanded code: slot
+ * addresses per lane, and for layers with nodes sharing bits the slot
+ * masks and hand-over cells of the two-frame kernel's rounds. */
+static int build_split(lg_code* c)
+{
+    const int q = c->q, K = c->k;
+    int dmax = 0, dmin = 1 << 30;
+    for (int i = 0; i < q; i++) {
+        const int d = c->layer_start[i + 1] - c->layer_start[i];
+        if (d > dmax)
+            dmax = d;
+        if (d < dmin)
+            dmin = d;
+    }
+    const int ks = (dmax + 3) / 2, sk = (ks + 1) / 2;
+    if (ks > 16)
+        return -1;
+    c->sk_ks = ks;
+    c->sk_wide = (dmin + 2) / 2 < ks - 1;
+    int rows = 0, mrows = 0;
+    for (int i = 0; i < q; i++) {
+        rows += sk + (c->nlev[i] > 1 ? 1 + sk : 0);
+        mrows += sk;
+    }
+    c->sk_rows = rows;
+    c->sk_msg_rows = mrows;
+    c->sk_lay = calloc((size_t)q * 4, sizeof(uint32_t));
+    c->sk_tab = calloc((size_t)rows * 720, sizeof(uint32_t));
+    if (!c->sk_lay || !c->sk_tab)
+        return -1;
+    int row = 0, mrow = 0;
+    for (int i = 0; i < q; i++) {
+        const int s = c->layer_start[i], d = c->layer_start[i + 1] - s, T = d + 2;
+        const int tail = c->nlev[i] > 1;
+        uint32_t* L = c->sk_lay + 4 * i;
+        L[0] = (uint32_t)T | (uint32_t)c->nlev[i] << 8;
+        L[1] = (uint32_t)mrow;
+        L[2] = (uint32_t)row;
+        L[3] = tail ? (uint32_t)(row + sk) : 0u;
+        for (int t = 0; t < 720; t++) {
+            const int j = t >> 1, h = t & 1;
+            const uint64_t mt = c->meta[(size_t)i * LG_M + j], ps = c->pos[(size_t)i * LG_M + j];
+            uint32_t dep = 0, succ = 0;
+            for (int k = 0; k < ks; k++) {
+                const int p = 2 * k + h;
+                uint32_t ad = (uint32_t)c->n, cell = 0;
+                if (p == 0) {
+                    ad = (uint32_t)(K + 360 * i + j);
+                } else if (p == 1) {
+                    if (i)
+                        ad = (uint32_t)(K + 360 * (i - 1) + j);
+                    else if (j)
+                        ad = (uint32_t)(K + 360 * (q - 1) + j - 1);
+                } else if (p < T) {
+                    const int e = p - 2;
+                    const uint32_t xe = c->xent[s + e];
+                    const int m = (j - LG_XENT_SHIFT(xe) + LG_M) % LG_M;
+                    ad = (xe >> 16) + (uint32_t)m;
+                    if (tail && ((c->dupm[i] >> e) & 1)) {
+                        const int r = (int)((ps >> (2 * e)) & 3);
+                        cell = (uint32_t)((LG_XENT_T0(xe) + r) * LG_M + m);
+                        dep |= ((LG_META_DEP(mt) >> e) & 1u) << k;
+                        succ |= ((LG_META_SUCC(mt) >> e) & 1u) << k;
+                    }
+                }
+                if (ad > 0xffffu || cell > 0xffffu)
+                    return -1;
+                c->sk_tab[(size_t)(row + k / 2) * 720 + t] |= ad << (16 * (k & 1));
+                if (tail)
+                    c->sk_tab[(size_t)(row + sk + 1 + k / 2) * 720 + t] |= cell << (16 * (k & 1));
+            }
+            if (tail)
+                c->sk_tab[(size_t)(row + sk) * 720 + t] = dep | succ << 16;
+        }
+        row += sk + (tail ? 1 + sk : 0);
+        mrow += sk;
+    }
+    return 0;
+}
+
 /* Layer i holds the table addresses x with x mod q == i; node j of that
  * layer is check q*j + i, and the bit 360*row + m reaches check
  * (x + m*q) mod R (ldpc.c:240-249), i.e. node (x div q + m) mod 360. */
@@ -218,6 +298,8 @@ static int build_code(lg_code* c, int id)
         }
     }
     free(owner);
+    if (build_split(c))
+        return -1;
     /* every edge counted: info edges plus the 2R-1 of the staircase */
     if ((long)naddr * LG_M + 2L * c->r - 1 != c->links_total)
         return -1;

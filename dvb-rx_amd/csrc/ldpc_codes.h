@@ -70,6 +70,21 @@ typedef struct lg_code {
     uint32_t* dupm;        /* q: information edges of the layer in a group
                               with k > 1 */
     int max_xfer;          /* largest number of hand-overs of one layer */
+    /* Tables of the split kernel (a lane pair per node, ldpc_sk.h): lane
+     * t = 2j + h takes the node's positions p = 2k + h (0 own parity, 1
+     * previous parity, 2 + e information edge e), k < sk_ks.  Rows of 720
+     * words, word t for lane t: per layer SK rows of slot posterior
+     * indices (two u16 a word, slot 2w in the low half; an empty slot
+     * points at index n, a spare cell), and for layers with nodes sharing
+     * bits a row of slot masks (dep in bits 0-15, succ in 16-31) and SK
+     * rows of hand-over cells; SK = (sk_ks + 1) / 2. */
+    int sk_ks;             /* slots per lane: (largest degree + 2) / 2 rounded up */
+    int sk_wide;           /* 1 if a layer has fewer slots than sk_ks - 1 */
+    uint32_t* sk_lay;      /* q x 4: positions | levels << 8, first message row,
+                              first address row, mask row (0: none) */
+    uint32_t* sk_tab;      /* sk_rows x 720 */
+    int sk_rows;
+    int sk_msg_rows;       /* message rows (of 720 words) per frame pair */
 } lg_code;
 
 #define LG_MAX_XFER 8      /* hand-overs per layer the kernels provide for */

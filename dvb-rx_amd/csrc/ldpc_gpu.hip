@@ -53,13 +53,13 @@
 using namespace lgk;
 
 static const int CLASS_DI[NCLASS] = { 12, 28 }; /* largest layer degree of each class (one-frame kernel) */
-static const int PK_DM[PK_NDM] = PK_DMS;        /* largest layer degree of each two-slot instance */
+static const int SK_INST[SK_NINST][2] = SK_INSTS; /* split kernel instances: slots per lane, wide */
 
-/* two-slot kernel instance of a code: the smallest whose degree covers its layers */
-static int pk_instance(int dmax)
+/* split kernel instance of a code (lg_code.sk_ks, sk_wide), -1 if none */
+static int sk_instance(const lg_code* cd)
 {
-    for (int k = 0; k < PK_NDM; k++)
-        if (dmax <= PK_DM[k])
+    for (int k = 0; k < SK_NINST; k++)
+        if (SK_INST[k][0] == cd->sk_ks && SK_INST[k][1] == cd->sk_wide)
             return k;
     return -1;
 }
@@ -72,12 +72,9 @@ struct DeviceTables {
     int ready = 0;
     DCode* codes = nullptr;
     uint32_t* ent = nullptr;
-    uint32_t* xent = nullptr;
-    uint64_t* pos = nullptr;
-    uint32_t* dupm = nullptr;
     uint2* lay = nullptr;
-    uint4* pkl = nullptr;    /* per layer of every code: the two-slot kernel's descriptor */
-    uint32_t* adt = nullptr; /* its address rows */
+    uint4* pkl = nullptr;    /* per layer of every code: the split kernel's descriptor */
+    uint32_t* adt = nullptr; /* its table rows (lg_code.sk_tab of every code) */
     uint64_t* meta = nullptr;
     std::vector<DCode> h_codes;
 };
@@ -90,7 +87,7 @@ DeviceTables g_tables[64];
 struct ldpc_gpu_ctx {
     int device;
     hipStream_t stream;
-    hipStream_t aux[NLAUNCH]; /* forked streams of concurrent launches */
+    hipStream_t aux[NAUX]; /* forked streams of concurrent launches (launch k on aux[k % NAUX]) */
     char err[256];
     DeviceTables* tab;
     int ncu;
@@ -146,7 +143,7 @@ struct ldpc_gpu_plan {
     unsigned* d_cnt;
     uint32_t* d_msg;
     size_t list_cap, seg_cap, msg_cap;
-    hipStream_t aux[NLAUNCH];
+    hipStream_t aux[NAUX];
     hipEvent_t fork, join[NLAUNCH], t0, t1;
     hipGraph_t graph;
     hipGraphExec_t exec;
@@ -191,11 +188,11 @@ static int upload_tables(ldpc_gpu_ctx* c)
         return LDPC_GPU_OK;
     const int nc = lg_code_count();
     std::vector<DCode> codes(nc);
-    std::vector<uint32_t> ent, xent, dupm;
+    std::vector<uint32_t> ent;
     std::vector<uint2> lay;
     std::vector<uint4> pkl;
     std::vector<uint32_t> adt;
-    std::vector<uint64_t> meta, pos;
+    std::vector<uint64_t> meta;
     for (int id = 0; id < nc; id++) {
         const lg_code* cd = lg_code_get(id);
         if (!cd)
@@ -212,36 +209,24 @@ static int upload_tables(ldpc_gpu_ctx* c)
         d.dmax = cd->max_layer_deg;
         d.dmin = 255;
         d.msg_words = cd->msg_words;
-        d.pk_msg_words = cd->pk_msg_words;
+        d.pk_msg_words = cd->sk_msg_rows * 720;
         d.code_id = id;
         d.max_xfer = cd->max_xfer;
+        d.sk_inst = sk_instance(cd);
+        if (d.sk_inst < 0)
+            return set_err(c, LDPC_GPU_EINVAL, "code %d: no split kernel for %d slots per lane", id, cd->sk_ks);
+        const uint32_t base_row = (uint32_t)(adt.size() / 720);
+        adt.insert(adt.end(), cd->sk_tab, cd->sk_tab + (size_t)cd->sk_rows * 720);
         ent.insert(ent.end(), cd->ent, cd->ent + cd->nent);
-        xent.insert(xent.end(), cd->xent, cd->xent + cd->nent);
-        dupm.insert(dupm.end(), cd->dupm, cd->dupm + cd->q);
         for (int i = 0; i < cd->q; i++) {
             const int st = cd->layer_start[i], deg = cd->layer_start[i + 1] - st;
             d.dmin = std::min(d.dmin, deg);
             lay.push_back(make_uint2((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24,
                                      (uint32_t)cd->msg_off[i]));
-            /* address rows of the layer: node j's edges 2w, 2w + 1 in word j
-             * of row w (posterior indices, < 2^16) */
-            const uint32_t arow = (uint32_t)(adt.size() / 360);
-            for (int w = 0; w < (deg + 1) / 2; w++) {
-                for (int j = 0; j < 360; j++) {
-                    uint32_t v = 0;
-                    for (int h = 0; h < 2 && 2 * w + h < deg; h++) {
-                        const uint32_t en = cd->ent[st + 2 * w + h];
-                        const int m = (j - (int)(en & 0xffff) + 360) % 360;
-                        v |= ((en >> 16) + (uint32_t)m) << (16 * h);
-                    }
-                    adt.push_back(v);
-                }
-            }
-            pkl.push_back(make_uint4((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24,
-                                     (uint32_t)cd->pk_msg_off[i], cd->dupm[i], arow));
+            const uint32_t* sl = cd->sk_lay + 4 * i;
+            pkl.push_back(make_uint4(sl[0], sl[1], sl[2] + base_row, sl[3] ? sl[3] + base_row : 0u));
         }
         meta.insert(meta.end(), cd->meta, cd->meta + cd->r);
-        pos.insert(pos.end(), cd->pos, cd->pos + cd->r);
         /* every layer degree must be one the code's class kernel handles */
         const int cls = d.dmax <= LOW_MAX_DEG ? 0 : 1;
         if (cd->nent > MAX_ENT || cd->q > MAX_LAYERS || d.dmax > 28 || (cls == 1 && d.dmin < HIGH_MIN_DEG) ||
@@ -255,12 +240,6 @@ static int upload_tables(ldpc_gpu_ctx* c)
     HIPCHK(c, hipMalloc((void**)&t.pkl, sizeof(uint4) * pkl.size()));
     HIPCHK(c, hipMalloc((void**)&t.adt, sizeof(uint32_t) * adt.size()));
     HIPCHK(c, hipMemcpy(t.adt, adt.data(), sizeof(uint32_t) * adt.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMalloc((void**)&t.xent, sizeof(uint32_t) * xent.size()));
-    HIPCHK(c, hipMalloc((void**)&t.dupm, sizeof(uint32_t) * dupm.size()));
-    HIPCHK(c, hipMalloc((void**)&t.pos, sizeof(uint64_t) * pos.size()));
-    HIPCHK(c, hipMemcpy(t.xent, xent.data(), sizeof(uint32_t) * xent.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(t.dupm, dupm.data(), sizeof(uint32_t) * dupm.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(t.pos, pos.data(), sizeof(uint64_t) * pos.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.pkl, pkl.data(), sizeof(uint4) * pkl.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.codes, codes.data(), sizeof(DCode) * codes.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(t.ent, ent.data(), sizeof(uint32_t) * ent.size(), hipMemcpyHostToDevice));
@@ -291,7 +270,7 @@ int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
     int rc;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         [&] {
-            for (int k = 0; k < NLAUNCH; k++)
+            for (int k = 0; k < NAUX; k++)
                 if (hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking) != hipSuccess)
                     return true;
             return false;
@@ -303,7 +282,8 @@ int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
                     return true;
             return false;
         }()) {
-        free(c);
+        /* releases the streams and events made so far (the rest are NULL) */
+        ldpc_gpu_destroy(c);
         return LDPC_GPU_EDEVICE;
     }
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
@@ -346,7 +326,7 @@ void ldpc_gpu_destroy(ldpc_gpu_ctx* c)
     free(c->h_nB);
     free(c->h_nrow0);
     free(c->h_nT);
-    for (int k = 0; k < NLAUNCH; k++)
+    for (int k = 0; k < NAUX; k++)
         if (c->aux[k]) {
             (void)hipStreamSynchronize(c->aux[k]);
             (void)hipStreamDestroy(c->aux[k]);
@@ -389,8 +369,8 @@ int ldpc_gpu_code_max_degree(int code)
 int ldpc_gpu_code_kernel_degree(int code)
 {
     const lg_code* c = lg_code_get(code);
-    const int k = c ? pk_instance(c->max_layer_deg) : -1;
-    return k >= 0 ? PK_DM[k] : LDPC_GPU_EINVAL;
+    const int k = c ? sk_instance(c) : -1;
+    return k >= 0 ? 2 * SK_INST[k][0] - 2 + 100 * SK_INST[k][1] : LDPC_GPU_EINVAL;
 }
 
 int ldpc_gpu_fec_lookup(int short_frame, const char* rate, int* nldpc, int* kldpc, int* kbch, int* t)
@@ -519,25 +499,25 @@ float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* c)
 } // extern "C"
 
 namespace lgk {
-#define LGK_PK_CASE_L(D, N)                                                                         \
-    if (PK_DM[k] == D && nms == (N != 0))                                                             \
-        return pk_launch_##D##_##N(grid, lds, st, a);
-#define LGK_PK_CASE_O(D, N)                                                                         \
-    if (PK_DM[k] == D && nms == (N != 0))                                                             \
-        return pk_occupancy_##D##_##N(lds, per_cu);
+#define SK_CASE_L(KS, W, N)                                                                       \
+    if (SK_INST[k][0] == KS && SK_INST[k][1] == W && nms == (N != 0))                                 \
+        return sk_launch_##KS##_##W##_##N(grid, lds, st, a);
+#define SK_CASE_O(KS, W, N)                                                                       \
+    if (SK_INST[k][0] == KS && SK_INST[k][1] == W && nms == (N != 0))                                 \
+        return sk_occupancy_##KS##_##W##_##N(lds, per_cu);
 
 hipError_t pk_launch(int k, bool nms, int grid, size_t lds, hipStream_t st, const PkArgs& a)
 {
-    if (k >= 0 && k < PK_NDM) {
-        LGK_PK_ALL(LGK_PK_CASE_L)
+    if (k >= 0 && k < SK_NINST) {
+        SK_ALL(SK_CASE_L)
     }
     return hipErrorInvalidValue;
 }
 
 hipError_t pk_occupancy(int k, bool nms, size_t lds, int* per_cu)
 {
-    if (k >= 0 && k < PK_NDM) {
-        LGK_PK_ALL(LGK_PK_CASE_O)
+    if (k >= 0 && k < SK_NINST) {
+        SK_ALL(SK_CASE_O)
     }
     return hipErrorInvalidValue;
 }
@@ -697,7 +677,7 @@ struct PkLayout {
 
 /* LDS of a launch: both slots' posteriors (2 bytes per bit of its longest
  * code), then two hand-over buffers of maxx * 360 cells */
-static size_t pk_hoff(const PkLayout& L, int k) { return (2 * (size_t)L.maxn[k] + 15) & ~(size_t)15; }
+static size_t pk_hoff(const PkLayout& L, int k) { return (2 * ((size_t)L.maxn[k] + 1) + 15) & ~(size_t)15; }
 static size_t pk_lds(const PkLayout& L, int k) { return pk_hoff(L, k) + 2 * sizeof(uint16_t) * 360 * (size_t)L.maxx[k]; }
 
 static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, float grid_scale, PkLayout& L)
@@ -710,7 +690,7 @@ static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, f
         L.maxn[k] = L.maxx[k] = 0;
     for (int f = 0; f < (int)hf.size(); f++) {
         const DCode& cd = hc[hf[f].code];
-        const int k = pk_instance(cd.dmax) * 2 + (cd.n < 64800);
+        const int k = cd.sk_inst * 2 + (cd.n < 64800);
         auto it = std::find(gcodes[k].begin(), gcodes[k].end(), hf[f].code);
         size_t g = it - gcodes[k].begin();
         if (it == gcodes[k].end()) {
@@ -776,7 +756,7 @@ static int pk_enqueue(ldpc_gpu_ctx* c, const PkLayout& L, PkArgs a, const int2* 
         a.hoff = pk_hoff(L, k);
         a.hstride = 360 * L.maxx[k];
         const size_t lds = pk_lds(L, k);
-        hipStream_t st = concurrent ? aux[k] : origin;
+        hipStream_t st = concurrent ? aux[k % NAUX] : origin;
         if (concurrent)
             HIPCHK(c, hipStreamWaitEvent(st, fork, 0));
         if (tev)
@@ -798,11 +778,8 @@ static PkArgs pk_args(ldpc_gpu_ctx* c, const DFrame* d_frames, const int* d_list
     PkArgs a;
     memset(&a, 0, sizeof(a));
     a.codes = c->tab->codes;
-    a.ent = c->tab->xent;
-    a.pos = c->tab->pos;
     a.pkl = c->tab->pkl;
     a.adt = c->tab->adt;
-    a.meta = c->tab->meta;
     a.frames = d_frames;
     a.list = d_list;
     a.llr = d_llr;
@@ -843,7 +820,7 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
     for (int k = 0; k < NLAUNCH; k++) {
         join[k] = c->ev_cls[2 * k + 1];
         c->cls_frames[k] = L.nfr[k];
-        c->cls_di[k] = PK_DM[k / 2];
+        c->cls_di[k] = 2 * SK_INST[k / 2][0] - 2 + 100 * SK_INST[k / 2][1];
     }
     if ((rc = pk_enqueue(c, L, a, c->d_seg, c->ticket, c->msg, c->stream, c->aux, c->ev0, join, c->ev_cls, !c->serial,
                          nms)))
@@ -1132,12 +1109,12 @@ extern "C" void ldpc_gpu_plan_destroy(ldpc_gpu_plan* p)
     (void)hipFree(p->d_seg);
     (void)hipFree(p->d_cnt);
     (void)hipFree(p->d_msg);
-    for (int k = 0; k < NLAUNCH; k++) {
+    for (int k = 0; k < NAUX; k++)
         if (p->aux[k])
             (void)hipStreamDestroy(p->aux[k]);
+    for (int k = 0; k < NLAUNCH; k++)
         if (p->join[k])
             (void)hipEventDestroy(p->join[k]);
-    }
     if (p->fork)
         (void)hipEventDestroy(p->fork);
     if (p->t0)
@@ -1198,9 +1175,11 @@ extern "C" int ldpc_gpu_plan_create(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frame
     if (hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&p->t0) != hipSuccess || hipEventCreate(&p->t1) != hipSuccess)
         return fail(LDPC_GPU_EDEVICE);
+    for (int k = 0; k < NAUX; k++)
+        if (hipStreamCreateWithFlags(&p->aux[k], hipStreamNonBlocking) != hipSuccess)
+            return fail(LDPC_GPU_EDEVICE);
     for (int k = 0; k < NLAUNCH; k++)
-        if (hipStreamCreateWithFlags(&p->aux[k], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&p->join[k], hipEventDisableTiming) != hipSuccess)
+        if (hipEventCreateWithFlags(&p->join[k], hipEventDisableTiming) != hipSuccess)
             return fail(LDPC_GPU_EDEVICE);
     if (hipMemcpy(p->d_frames, df.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice) != hipSuccess)
         return fail(LDPC_GPU_EDEVICE);

@@ -13,17 +13,15 @@
 #include <stdint.h>
 
 #define NT 384 /* threads per workgroup of the one-frame kernel: 360 check nodes of a layer */
-/* the two-slot kernel: exactly one thread per check node, so no thread of
- * a layer is idle and the node code needs no lane test (the sixth wave has
- * 40 lanes) */
-#define PK_NT 360
+/* the split two-frame kernel: a lane pair per check node */
+#define SK_NT 720
 #define NCLASS 2 /* degree classes of the one-frame kernel: one instance each (registers grow with the degree) */
-/* the two-slot kernel: one instance per largest layer degree PK_DMS[k]; a
- * code goes to the smallest that covers its layers */
-#define PK_NDM 6
-#define PK_DMS { 4, 8, 12, 16, 20, 28 }
-#define NLAUNCH (2 * PK_NDM) /* launches of one decode: kernel instance x frame size */
-
+/* instances of the split kernel: (slots per lane, wide) */
+#define SK_NINST 15
+#define SK_INSTS { { 2, 0 }, { 3, 0 }, { 4, 0 }, { 4, 1 }, { 5, 0 }, { 6, 0 }, { 7, 0 }, { 7, 1 }, { 8, 0 }, { 9, 0 }, \
+                   { 10, 0 }, { 10, 1 }, { 11, 0 }, { 14, 0 }, { 15, 0 } }
+#define NLAUNCH (2 * SK_NINST) /* launches of one decode: kernel instance x frame size */
+#define NAUX 4                 /* streams the concurrent launches of a decode share */
 
 namespace lgk {
 
@@ -37,9 +35,10 @@ struct DCode {
     int dmax;      /* largest information degree of a layer */
     int dmin;      /* smallest */
     int msg_words; /* 32-bit message words per frame */
-    int pk_msg_words; /* the same for the two-frame layout */
+    int pk_msg_words; /* message words of a frame pair in the split kernel's layout */
     int code_id;   /* index in the registry */
     int max_xfer;  /* hand-overs of its busiest layer */
+    int sk_inst;   /* split kernel instance (SK_INSTS index) */
 };
 
 struct DFrame {
@@ -93,19 +92,14 @@ __device__ __forceinline__ int lay_start(uint2 ly) { return ly.x & 0xffff; }
 
 /* ---- two frames per workgroup: packed 16-bit arithmetic (ldpc_pk.h) ----- */
 
-#define PK_WMAX 15 /* message words of a node: (28 + 2 + 1) / 2 */
 
 struct PkArgs {
     const DCode* codes;
-    const uint32_t* ent;   /* entries with their hand-over fields (lg_code.xent) */
-    const uint64_t* pos;   /* per node: rank on each shared bit (lg_code.pos) */
     size_t hoff;           /* byte offset of the hand-over buffers in LDS */
     int hstride;           /* u16 cells per hand-over buffer */
-    const uint4* pkl;      /* per layer: first entry | degree << 16 | levels << 24, first message row
-                              of the two-frame layout, edges on shared bits, first address row */
-    const uint32_t* adt;   /* address rows: per layer ceil(degree / 2) rows of 360 words, word j
-                              holding the posterior indices of node j's edges 2w (low) and 2w + 1 */
-    const uint64_t* meta;
+    const uint4* pkl;      /* per layer: positions | levels << 8, first message row, first address
+                              row, mask row (ldpc_sk_kern.hip) */
+    const uint32_t* adt;   /* rows of 720 words: slot addresses, masks, hand-over cells */
     const DFrame* frames;
     const int* list;       /* frame indices, grouped by code into segments */
     const int2* seg;       /* per segment: first list entry, length */
@@ -142,14 +136,18 @@ struct PkRowPtrs {
 hipError_t layered_launch(int cls, int grid, size_t lds, hipStream_t st, const KArgs& a);
 hipError_t layered_occupancy(int cls, size_t lds, int* per_cu);
 hipError_t reset_counters(hipStream_t st, unsigned* cnt);
-/* the two-slot kernel instance k (PK_DMS[k]), plain or learned */
+/* the split kernel instance k (SK_INSTS[k]), plain or learned */
 hipError_t pk_launch(int k, bool nms, int grid, size_t lds, hipStream_t st, const PkArgs& a);
 hipError_t pk_occupancy(int k, bool nms, size_t lds, int* per_cu);
-#define LGK_PK_DECL(D, N)                                                                          \
-    hipError_t pk_launch_##D##_##N(int grid, size_t lds, hipStream_t st, const PkArgs& a);          \
-    hipError_t pk_occupancy_##D##_##N(size_t lds, int* per_cu);
-#define LGK_PK_ALL(X) X(4, 0) X(4, 1) X(8, 0) X(8, 1) X(12, 0) X(12, 1) X(16, 0) X(16, 1) X(20, 0) X(20, 1) X(28, 0) X(28, 1)
-LGK_PK_ALL(LGK_PK_DECL)
+#define SK_DECL(KS, W, N)                                                                          \
+    hipError_t sk_launch_##KS##_##W##_##N(int grid, size_t lds, hipStream_t st, const PkArgs& a);   \
+    hipError_t sk_occupancy_##KS##_##W##_##N(size_t lds, int* per_cu);
+#define SK_ALL(X)                                                                                   \
+    X(2, 0, 0) X(3, 0, 0) X(4, 0, 0) X(4, 1, 0) X(5, 0, 0) X(6, 0, 0) X(7, 0, 0) X(7, 1, 0) X(8, 0, 0) X(9, 0, 0)  \
+    X(10, 0, 0) X(10, 1, 0) X(11, 0, 0) X(14, 0, 0) X(15, 0, 0) X(2, 0, 1) X(3, 0, 1) X(4, 0, 1) X(4, 1, 1)       \
+    X(5, 0, 1) X(6, 0, 1) X(7, 0, 1) X(7, 1, 1) X(8, 0, 1) X(9, 0, 1) X(10, 0, 1) X(10, 1, 1) X(11, 0, 1)         \
+    X(14, 0, 1) X(15, 0, 1)
+SK_ALL(SK_DECL)
 
 } // namespace lgk
 

@@ -1,0 +1,247 @@
+/*
+ * Check nodes split over two lanes: the arithmetic of ldpc_pk.h (two
+ * frames in the 16-bit halves of each register) with each node's edges
+ * shared by a lane pair, shared by the HIP kernel (ldpc_sk_kern.hip) and
+ * the CPU model of it (built with clang, same code).
+ *
+ * Node j of a layer is served by lanes 2j (half h = 0) and 2j + 1 (h = 1).
+ * The node's edges by position p: 0 its own parity bit, 1 the previous
+ * parity bit (the IRA staircase, ldpc.c:277-284), 2 + e information edge
+ * e; half h takes the positions p = 2k + h, its "slots" k = 0..KS-1.  A
+ * layer of T = degree + 2 positions leaves slot k of half h empty when
+ * 2k + h >= T (only slots k >= KA can be, KA a template bound), and node
+ * (0, 0) has no previous parity (slot 0 of half 1 in layer 0).  Empty
+ * slots point at a spare posterior cell; they are left out of the
+ * minimums, the sign and the syndrome by all-ones skip masks, and what
+ * they write is never read.
+ *
+ * Each half forms the minimums and the sign of its own slots; the pair
+ * exchanges them (sk_combine; the two smallest of a union are the two
+ * smallest of the two halves' two smallest) and both continue with the
+ * node's: the results are those of the whole node, bit for bit.
+ *
+ * Messages: slot k in word k / 2 of the half, bytes [k even: 0, 1; odd:
+ * 2, 3] (slot 0 of the two frames, then slot 1), as the selectors of
+ * pk_msg_sel give them.
+ */
+#ifndef LDPC_SK_H
+#define LDPC_SK_H
+
+#include <stdint.h>
+
+#include "ldpc_pk.h"
+
+#define SK_WORDS(ks) (((ks) + 1) / 2)
+
+template <int KS>
+struct SkHalf {
+    int ad[KS];      /* posterior index of each slot */
+    lg_pk x[KS];     /* extrinsic inputs, both frames */
+    lg_pk a[KS];     /* their magnitudes */
+    lg_pk min0, min1, s;
+};
+
+/* slot posterior indices from the address words (two u16 a word, slot 2w
+ * in the low half) */
+template <int KS>
+LG_PK_HD void sk_addrs(SkHalf<KS>& n, const uint32_t* aw)
+{
+#pragma unroll
+    for (int k = 0; k < KS; k++)
+        n.ad[k] = (int)((k & 1) ? aw[k >> 1] >> 16 : aw[k >> 1] & 0xffffu);
+}
+
+/* skip mask of slot k: all ones if it is empty (2k + h >= T, or node
+ * (0, 0)'s missing previous parity: skip0 on slot 0) */
+template <int KA>
+LG_PK_HD uint32_t sk_skip(int k, int T, int h, uint32_t skip0)
+{
+    uint32_t m = k == 0 ? skip0 : 0u;
+    if (k >= KA)
+        m |= 2 * k + h >= T ? ~0u : 0u;
+    return m;
+}
+
+/* Phase A of a half: every slot read, minimums and sign of the present
+ * ones.  TAIL: the slots in dep (read after an earlier node of the layer)
+ * stay out too; the hand-over rounds fold them. */
+template <int KS, int KA, bool TAIL>
+LG_PK_HD void sk_phase_a(SkHalf<KS>& n, const uint16_t* post, int T, int h, uint32_t skip0, uint32_t dep,
+                         const uint32_t* mw, uint32_t sel_even, uint32_t sel_odd)
+{
+    n.min0 = pk_dup(-1);
+    n.min1 = pk_dup(-1);
+    n.s = pk_dup(0);
+#pragma unroll
+    for (int k = 0; k < KS; k++) {
+        const lg_pk x = pk_in(post, n.ad[k], mw, k, sel_even, sel_odd);
+        uint32_t skip = sk_skip<KA>(k, T, h, skip0);
+        if (TAIL)
+            skip |= 0u - ((dep >> k) & 1);
+        const lg_pk a = pk_of(pk_bits(pk_abs(x)) | skip);
+        n.x[k] = x;
+        n.a[k] = a;
+        n.min1 = pk_minu(n.min1, pk_maxu(n.min0, a));
+        n.min0 = pk_minu(n.min0, a);
+        n.s = pk_of(pk_bits(n.s) ^ (pk_bits(x) & ~skip));
+    }
+}
+
+/* fold slot k with input x into the half's minimums and sign */
+template <int KS>
+LG_PK_HD void sk_fold(SkHalf<KS>& n, int k, lg_pk x)
+{
+    const lg_pk a = pk_abs(x);
+    n.x[k] = x;
+    n.a[k] = a;
+    n.min1 = pk_minu(n.min1, pk_maxu(n.min0, a));
+    n.min0 = pk_minu(n.min0, a);
+    n.s = n.s ^ x;
+}
+
+/* merge the partner half's minimums and sign (p0, p1, ps) */
+template <int KS>
+LG_PK_HD void sk_combine(SkHalf<KS>& n, lg_pk p0, lg_pk p1, lg_pk ps)
+{
+    const lg_pk hi = pk_maxu(n.min0, p0);
+    n.min0 = pk_minu(n.min0, p0);
+    n.min1 = pk_minu(pk_minu(n.min1, p1), hi);
+    n.s = n.s ^ ps;
+}
+
+template <int KS, bool NMS>
+LG_PK_HD PkOut sk_mags(const SkHalf<KS>& n, lg_pk ps, lg_pk pb)
+{
+    if constexpr (NMS) {
+        const lg_pk mn0 = pk_nms_mag(n.min0, ps, pb), mn1 = pk_nms_mag(n.min1, ps, pb);
+        return PkOut{ mn1, mn0 - mn1 };
+    } else {
+        (void)ps;
+        (void)pb;
+        const lg_pk mn0 = pk_max(pk_minu(n.min0, pk_dup(0x7F00)) - pk_dup(0x0100), pk_dup(0));
+        const lg_pk mn1 = pk_max(pk_minu(n.min1, pk_dup(0x7F00)) - pk_dup(0x0100), pk_dup(0));
+        return PkOut{ mn1, mn0 - mn1 };
+    }
+}
+
+/* o_k: |o| = mn1 if |x_k| is the node's smallest input, else mn0; sign =
+ * product of the other signs (see pk_out) */
+template <int KS>
+LG_PK_HD lg_pk sk_out(const SkHalf<KS>& n, const PkOut& m, int k)
+{
+    const lg_pk mag = pk_sel_mag(n.a[k] - n.min0, m.dmn, m.mn1);
+    const lg_pk t = (n.s ^ n.x[k]) >> 15;
+    return (mag ^ t) - t;
+}
+
+/* Phase C of a half: final posteriors (TAIL: the slots in succ are written
+ * by a later node of the layer) and the new messages, clamped to [-32,
+ * 31] (ldpc.c:343), word w to mw_out[stride * w] */
+template <int KS, bool TAIL, bool NMS>
+LG_PK_HD void sk_phase_c(const SkHalf<KS>& n, uint16_t* post, uint32_t succ, uint32_t* mw_out, int stride, lg_pk ps,
+                         lg_pk pb)
+{
+    const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
+    uint32_t even = 0;
+#pragma unroll
+    for (int k = 0; k < KS; k++) {
+        const lg_pk o = sk_out(n, m, k);
+        if (!TAIL || !((succ >> k) & 1))
+            post[n.ad[k]] = pk_post_bytes(pk_adds(n.x[k], o));
+        const uint32_t c = pk_bits(pk_msg_clamp(o));
+        if (!(k & 1)) {
+            even = c;
+            if (k + 1 == KS)
+                mw_out[stride * (k / 2)] = lg_perm(0, even, PK_PACK_MSG);
+        } else {
+            mw_out[stride * (k / 2)] = lg_perm(c, even, PK_PACK_MSG);
+        }
+    }
+}
+
+/* parity and non-zero flags of a half's present slots (bad(),
+ * ldpc.c:286-305): par collects the sign bits (bit 7 / 15 of the two
+ * frames), nz bit 7 / 15 stays set while no posterior is zero */
+template <int KS, int KA>
+LG_PK_HD void sk_check(const uint16_t* post, const int* ad, int T, int h, uint32_t skip0, uint32_t& par, uint32_t& nz)
+{
+    par = 0;
+    nz = 0x8080;
+#pragma unroll
+    for (int k = 0; k < KS; k++) {
+        const uint32_t r = post[ad[k]];
+        const uint32_t skip = sk_skip<KA>(k, T, h, skip0);
+        par ^= r & ~skip;
+        nz &= ((r & 0x7F7F) + 0x7F7F) | r | skip;
+    }
+}
+
+/* node result of the pair's flags: bit f set if frame f holds the check */
+LG_PK_HD uint32_t sk_check_bits(uint32_t par, uint32_t nz)
+{
+    const uint32_t good = nz & ~par & 0x8080;
+    return ((good >> 7) & 1) | ((good >> 14) & 2);
+}
+
+/* ---- layers with nodes sharing a bit: hand-over rounds (ldpc_pk.h) ------
+ *
+ * hw: the slots' hand-over cells, two u16 a word as the addresses; a slot
+ * at rank r on its bit hands over through cell c = (t0 + r) * 360 + m and
+ * reads its predecessor's at c - 360. */
+LG_PK_HD int sk_cell(const uint32_t* hw, int k) { return (int)((k & 1) ? hw[k >> 1] >> 16 : hw[k >> 1] & 0xffffu); }
+
+template <int KS>
+LG_PK_HD PkPart sk_part(const SkHalf<KS>& n)
+{
+    return PkPart{ n.min0, n.min1, n.s };
+}
+
+template <int KS>
+LG_PK_HD void sk_restore(SkHalf<KS>& n, const PkPart& p)
+{
+    n.min0 = p.min0;
+    n.min1 = p.min1;
+    n.s = p.s;
+}
+
+/* round 0: the dependent slots with the values phase A read */
+template <int KS>
+LG_PK_HD void sk_fold_dep0(SkHalf<KS>& n, uint32_t dep)
+{
+#pragma unroll
+    for (int k = 0; k < KS; k++)
+        if ((dep >> k) & 1)
+            sk_fold(n, k, n.x[k]);
+}
+
+/* write the values handed to the successors (slots in succ); n holds the
+ * node's combined minimums */
+template <int KS, bool NMS>
+LG_PK_HD void sk_put_hand(const SkHalf<KS>& n, uint16_t* hb, const uint32_t* hw, uint32_t succ, lg_pk ps, lg_pk pb)
+{
+    const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
+#pragma unroll
+    for (int k = 0; k < KS; k++)
+        if ((succ >> k) & 1)
+            hb[sk_cell(hw, k)] = pk_post_bytes(pk_adds(n.x[k], sk_out(n, m, k)));
+}
+
+/* take the values handed over by the predecessors (slots in dep) into the
+ * half's minimums; non-zero if an input differs from the previous round */
+template <int KS>
+LG_PK_HD uint32_t sk_take_hand(SkHalf<KS>& n, const uint16_t* hb, const uint32_t* hw, uint32_t dep, const uint32_t* mw,
+                               uint32_t sel_even, uint32_t sel_odd)
+{
+    uint32_t chg = 0;
+#pragma unroll
+    for (int k = 0; k < KS; k++) {
+        if ((dep >> k) & 1) {
+            const lg_pk x = pk_in_w(hb[sk_cell(hw, k) - 360], mw, k, sel_even, sel_odd);
+            chg |= pk_bits(x) ^ pk_bits(n.x[k]);
+            sk_fold(n, k, x);
+        }
+    }
+    return chg;
+}
+
+#endif

@@ -1,34 +1,54 @@
 /*
- * The two-slot decoder kernel (ldpc_pk_kernel, see ldpc_gpu.hip): one
- * instance per build of this file, chosen by LGK_DM (the largest layer
- * information degree the instance handles: 4, 8, 12, 16, 20 or 28) and
- * LGK_NMS (0 plain, 1 learned), with its launch entry points.  One layer
- * body of degree <= LGK_DM serves every layer of the codes sent to it.
+ * The split decoder kernel (ldpc_sk_kernel): two frames per workgroup in
+ * the 16-bit halves of each register (ldpc_pk.h), each check node on a
+ * lane pair (ldpc_sk.h), 720 threads = 360 nodes x 2.  One instance per
+ * build of this file, chosen by SK_KS (slots per lane: the code's largest
+ * layer degree + 2, halved, rounded up), SK_WIDE (1: layers of the code
+ * differ by more than one slot, every slot may be empty) and SK_NMS (0
+ * plain, 1 learned), with its launch entry points.
+ *
+ * Tables (PkArgs.adt, rows of 720 words, word t for lane t): per layer the
+ * slot addresses (SK_WORDS(KS) rows), and for layers with nodes sharing
+ * bits the slots' dependency masks (one row: dep in bits 0-15, succ in
+ * 16-31) and hand-over cells (SK_WORDS(KS) rows).  Layer descriptor
+ * (PkArgs.pkl): x = positions T | level count << 8, y = first message row
+ * (rows of 720 words in the workgroup's message block), z = first
+ * address row, w = the mask row (0 when the layer has none).
  */
 #include <hip/hip_runtime.h>
 
 #include "ldpc_codes.h"
 #include "ldpc_kern.h"
 #include "ldpc_nms.h"
-#include "ldpc_pk.h"
+#include "ldpc_sk.h"
 
-#if !defined(LGK_DM) || !defined(LGK_NMS)
-#error "build with -DLGK_DM=4|8|12|16|20|28 -DLGK_NMS=0|1"
+#if !defined(SK_KS) || !defined(SK_WIDE) || !defined(SK_NMS)
+#error "build with -DSK_KS=2..15 -DSK_WIDE=0|1 -DSK_NMS=0|1"
 #endif
 
 namespace lgk {
 
 namespace {
 
-constexpr int DM = LGK_DM;
-constexpr bool NMS = LGK_NMS != 0;
-constexpr int WM = LG_PK_WORDS(DM);
+constexpr int KS = SK_KS;
+constexpr int KA = SK_WIDE ? 1 : KS - 1; /* slots k >= KA may be empty */
+constexpr bool NMS = SK_NMS != 0;
+constexpr int WA = SK_WORDS(KS);
 
-/* layer descriptor (PkArgs.pkl): x = first table entry | degree << 16 |
- * level count << 24, y = first message row, z = edges on shared bits */
-__device__ __forceinline__ int ld_start(uint32_t x) { return (int)(x & 0xffff); }
-__device__ __forceinline__ int ld_deg(uint32_t x) { return (int)((x >> 16) & 0xff); }
-__device__ __forceinline__ int ld_nlev(uint32_t x) { return (int)(x >> 24); }
+__device__ __forceinline__ int sl_T(uint32_t x) { return (int)(x & 0xff); }
+__device__ __forceinline__ int sl_nlev(uint32_t x) { return (int)((x >> 8) & 0xff); }
+
+/* the partner lane's value (lanes 2j and 2j + 1 swap: quad permutation
+ * [1, 0, 3, 2]) */
+__device__ __forceinline__ uint32_t lane_swap(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false); }
+
+template <int N>
+__device__ __forceinline__ void sk_pair(SkHalf<N>& n)
+{
+    const lg_pk p0 = pk_of(lane_swap(pk_bits(n.min0))), p1 = pk_of(lane_swap(pk_bits(n.min1)));
+    const lg_pk ps = pk_of(lane_swap(pk_bits(n.s)));
+    sk_combine(n, p0, p1, ps);
+}
 
 /* AND of a 2-bit value over the workgroup, one barrier: three vote words
  * in turn, the next one reset before the barrier of the current vote */
@@ -45,19 +65,42 @@ __device__ __forceinline__ uint32_t wg_and2(uint32_t v, unsigned* s_vote, int& n
     return s_vote[slot];
 }
 
-/* syndrome of both slots, chunks of layers with an early exit once every
+/* syndrome of both frames, chunks of layers with an early exit once every
  * slot that is still decoding has failed a check; bit k: slot k holds */
-__device__ __forceinline__ uint32_t pk_syndrome(const uint16_t* sm, const uint32_t* ent, const uint4* pkl, int j, int K,
-                                                int q, uint32_t live, unsigned* s_vote, int& nvote)
+__device__ __forceinline__ uint32_t sk_syndrome(const uint16_t* sm, const uint32_t* adt, const uint4* skl, int t, int q,
+                                                int N, uint32_t live, unsigned* s_vote, int& nvote)
 {
+    const int h = t & 1;
     uint32_t good = 3;
     for (int i0 = 0; i0 < q; i0 += SYN_CHUNK) {
         uint32_t ok = 3;
         const int i1 = min(i0 + SYN_CHUNK, q);
-        for (int i = i0; i < i1; i++) {
-            const uint32_t lx = pkl[i].x;
-            ok &= pk_check<DM>(sm, ent + ld_start(lx), ld_deg(lx), i, j, K, q);
+        uint32_t aw[SYN_CHUNK][WA];
+        int Ts[SYN_CHUNK];
+#pragma unroll
+        for (int c = 0; c < SYN_CHUNK; c++) {
+            if (i0 + c >= i1)
+                continue;
+            const uint4 ld = skl[i0 + c];
+            Ts[c] = sl_T(ld.x);
+            const uint32_t* ar = adt + (size_t)720 * ld.z + t;
+#pragma unroll
+            for (int w = 0; w < WA; w++)
+                aw[c][w] = ar[720 * w];
         }
+#pragma unroll
+        for (int c = 0; c < SYN_CHUNK; c++) {
+            if (i0 + c >= i1)
+                continue;
+            SkHalf<KS> n;
+            sk_addrs(n, aw[c]);
+            uint32_t par, nz;
+            sk_check<KS, KA>(sm, n.ad, Ts[c], h, (i0 + c == 0 && t == 1) ? ~0u : 0u, par, nz);
+            par ^= lane_swap(par);
+            nz &= lane_swap(nz);
+            ok &= sk_check_bits(par, nz);
+        }
+        (void)N;
         good &= wg_and2(ok, s_vote, nvote);
         if (!(good & live))
             break;
@@ -65,113 +108,109 @@ __device__ __forceinline__ uint32_t pk_syndrome(const uint16_t* sm, const uint32
     return good;
 }
 
-/* One layer of D <= DM information edges by the workgroup.  TAIL (the
- * layer has nodes sharing bits): after phase A the nodes run hand-over
- * rounds (ldpc_pk.h) until their inputs stop changing or the layer's level
- * count is reached; hb: the two hand-over buffers, hstride u16 cells
- * apart. */
+/* One layer by the workgroup.  TAIL (the layer has nodes sharing bits):
+ * after phase A the pairs run hand-over rounds (ldpc_pk.h) until their
+ * inputs stop changing or the layer's level count is reached; hb: the two
+ * hand-over buffers, hstride u16 cells apart. */
 template <bool TAIL>
-__device__ __forceinline__ void pk_layer(uint16_t* sm, const uint32_t* ent, const uint32_t* aw, int D, int i, int j,
-                                         int K, int q, int nlev, const uint32_t* mw, uint64_t meta, uint64_t pos,
-                                         uint32_t dupm, uint32_t* mrow, uint32_t se, uint32_t so, lg_pk ps, lg_pk pb,
-                                         uint16_t* hb, int hstride, unsigned* s_vote, int& nvote)
+__device__ __forceinline__ void sk_layer(uint16_t* sm, const uint32_t* aw, int T, int h, uint32_t skip0, int nlev,
+                                         const uint32_t* mw, uint32_t tm, const uint32_t* hw, uint32_t* mrow,
+                                         uint32_t se, uint32_t so, lg_pk ps, lg_pk pb, uint16_t* hb, int hstride,
+                                         unsigned* s_vote, int& nvote)
 {
-    const uint32_t dep = TAIL ? LG_META_DEP(meta) : 0, succ = TAIL ? LG_META_SUCC(meta) : 0;
-    PkNode<DM> n;
-    pk_addr_words<DM>(n.ad, aw, D);
-    pk_phase_a<DM, TAIL>(n, sm, D, i, j, K, q, dep, mw, se, so);
+    const uint32_t dep = TAIL ? (tm & 0xffffu) : 0, succ = TAIL ? (tm >> 16) : 0;
+    SkHalf<KS> n;
+    sk_addrs(n, aw);
+    sk_phase_a<KS, KA, TAIL>(n, sm, T, h, skip0, dep, mw, se, so);
     if (TAIL) {
-        const PkPart part = pk_part(n);
-        pk_fold_dep0<DM>(n, D, dep);
-        pk_put_hand<DM, NMS>(n, hb, ent, D, dupm, succ, pos, ps, pb);
+        const PkPart part = sk_part(n);
+        sk_fold_dep0(n, dep);
+        sk_pair(n);
+        sk_put_hand<KS, NMS>(n, hb, hw, succ, ps, pb);
         __syncthreads();
 #pragma unroll 1
         for (int r = 1; r < nlev; r++) {
-            pk_restore(n, part);
-            const uint32_t chg = pk_take_hand<DM>(n, hb + ((r - 1) & 1) * hstride, ent, D, dupm, dep, pos, mw, se, so);
+            sk_restore(n, part);
+            const uint32_t chg = sk_take_hand(n, hb + ((r - 1) & 1) * hstride, hw, dep, mw, se, so);
+            sk_pair(n);
             if (r == nlev - 1)
                 break;
-            pk_put_hand<DM, NMS>(n, hb + (r & 1) * hstride, ent, D, dupm, succ, pos, ps, pb);
+            sk_put_hand<KS, NMS>(n, hb + (r & 1) * hstride, hw, succ, ps, pb);
             if (wg_and2(chg ? 0u : 3u, s_vote, nvote) & 1)
                 break;
         }
+    } else {
+        sk_pair(n);
     }
-    pk_phase_c<DM, TAIL, NMS>(n, sm, D, succ, mrow + j, 360, ps, pb);
+    sk_phase_c<KS, TAIL, NMS>(n, sm, succ, mrow, 720, ps, pb);
 }
 
-/* one iteration (all layers) on both slots; the code's tables are read
- * through uniform pointers (scalar loads); the messages, node masks and
- * descriptor of layer i+1 are fetched while layer i runs */
-__device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, const uint4* pkl, const uint32_t* adt,
-                                             const uint64_t* meta, const uint64_t* pos, uint32_t* msg, int K, int q,
-                                             int first0, int first1, PkRowPtrs rows, uint16_t* hb, int hstride,
+/* one iteration (all layers) on both frames; the addresses, masks and
+ * messages of layer i+1 are fetched while layer i runs */
+__device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, const uint4* skl, uint32_t* msg, int q,
+                                             int N, int first0, int first1, PkRowPtrs rows, uint16_t* hb, int hstride,
                                              unsigned* s_vote, int& nvote)
 {
-    const int j = threadIdx.x;
+    const int t = threadIdx.x, h = t & 1, j = t >> 1;
     const uint32_t se = pk_msg_sel(0, first0, first1), so = pk_msg_sel(1, first0, first1);
     const bool load = !(first0 && first1);
-    constexpr int WA = (DM + 1) / 2;
-    uint32_t nx[WM], na[WA];
-    uint64_t nm = 0, np = 0;
+    uint32_t nx[WA], na[WA], nh[WA], ntm = 0;
 #pragma unroll
-    for (int w = 0; w < WM; w++)
+    for (int w = 0; w < WA; w++) {
         nx[w] = 0;
-    auto fetch = [&](uint32_t lx, uint32_t row0, uint32_t arow, int l) {
-        const int W = LG_PK_WORDS(ld_deg(lx)), A = (ld_deg(lx) + 1) / 2;
-        const uint32_t* ar = adt + (size_t)360 * arow + j;
-#pragma unroll
-        for (int w = 0; w < WA; w++) {
-            if (w >= A)
-                continue;
-            na[w] = ar[360 * w];
-        }
-        if (load) {
-            const uint32_t* row = msg + 360 * row0 + j;
-#pragma unroll
-            for (int w = 0; w < WM; w++) {
-                if (w >= W)
-                    continue;
-                nx[w] = row[360 * w];
-            }
-        }
-        if (ld_nlev(lx) > 1) {
-            nm = meta[360 * l + j];
-            np = pos[360 * l + j];
-        }
-    };
-    uint4 ldn = pkl[0];
-    fetch(ldn.x, ldn.y, ldn.w, 0);
-    for (int i = 0; i < q; i++) {
-        const uint4 ld = ldn;
-        uint32_t mw[WM], aw[WA];
-#pragma unroll
-        for (int w = 0; w < WM; w++)
-            mw[w] = nx[w];
+        nh[w] = 0;
+    }
+    auto fetch = [&](const uint4& ld) {
+        const uint32_t* ar = adt + (size_t)720 * ld.z + t;
 #pragma unroll
         for (int w = 0; w < WA; w++)
-            aw[w] = na[w];
-        const uint64_t mt = nm, pt = np;
-        if (i + 1 < q) {
-            ldn = pkl[i + 1];
-            fetch(ldn.x, ldn.y, ldn.w, i + 1);
+            na[w] = ar[720 * w];
+        if (load) {
+            const uint32_t* row = msg + (size_t)720 * ld.y + t;
+#pragma unroll
+            for (int w = 0; w < WA; w++)
+                nx[w] = row[720 * w];
         }
-        const int D = ld_deg(ld.x), nlev = ld_nlev(ld.x);
-        const uint32_t* e = ent + ld_start(ld.x);
-        uint32_t* mrow = msg + 360 * ld.y;
-        /* learned scale / offset of the layer's check degree, per slot;
+        if (sl_nlev(ld.x) > 1) {
+            const uint32_t* tr = adt + (size_t)720 * ld.w + t;
+            ntm = tr[0];
+#pragma unroll
+            for (int w = 0; w < WA; w++)
+                nh[w] = tr[720 * (w + 1)];
+        }
+    };
+    uint4 ldn = skl[0];
+    fetch(ldn);
+    for (int i = 0; i < q; i++) {
+        const uint4 ld = ldn;
+        uint32_t mw[WA], aw[WA], hw[WA];
+#pragma unroll
+        for (int w = 0; w < WA; w++) {
+            mw[w] = nx[w];
+            aw[w] = na[w];
+            hw[w] = nh[w];
+        }
+        const uint32_t tm = ntm;
+        if (i + 1 < q) {
+            ldn = skl[i + 1];
+            fetch(ldn);
+        }
+        const int T = sl_T(ld.x), nlev = sl_nlev(ld.x);
+        uint32_t* mrow = msg + (size_t)720 * ld.y + t;
+        const uint32_t skip0 = (i == 0 && t == 1) ? ~0u : 0u;
+        /* learned scale / offset of the layer's check degree, per frame;
          * node (0, 0) has no previous parity, one edge less */
         lg_pk ps = pk_dup(16), pb = pk_dup(16);
         if constexpr (NMS) {
-            const int dc = D + 2 - (i == 0 && j == 0);
+            const int dc = T - (i == 0 && j == 0);
             ps = lg_pk{ (short)rows.S0[dc], (short)rows.S1[dc] };
             pb = lg_pk{ (short)rows.B0[dc], (short)rows.B1[dc] };
         }
+        (void)N;
         if (nlev > 1)
-            pk_layer<true>(sm, e, aw, D, i, j, K, q, nlev, mw, mt, pt, ld.z, mrow, se, so, ps, pb, hb, hstride,
-                           s_vote, nvote);
+            sk_layer<true>(sm, aw, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote, nvote);
         else
-            pk_layer<false>(sm, e, aw, D, i, j, K, q, nlev, mw, mt, pt, 0, mrow, se, so, ps, pb, hb, hstride, s_vote,
-                            nvote);
+            sk_layer<false>(sm, aw, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote, nvote);
         __syncthreads();
     }
 }
@@ -184,7 +223,7 @@ __device__ __forceinline__ void pk_iteration(uint16_t* sm, const uint32_t* ent, 
  * continues.  When the segment is exhausted and both slots are empty, the
  * workgroup moves to another segment with frames left, or exits.
  */
-__global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
+__global__ void __launch_bounds__(SK_NT) ldpc_sk_kernel(PkArgs a)
 {
     extern __shared__ __align__(16) uint16_t sm16[];
     __shared__ unsigned s_vote[3];
@@ -200,8 +239,6 @@ __global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
     uint32_t* msg = a.msg + a.msg_stride * (size_t)blockIdx.x;
     uint16_t* hb = (uint16_t*)((uint8_t*)sm16 + a.hoff);
     DCode cd;
-    const uint64_t* meta = nullptr;
-    const uint64_t* pos = nullptr;
 
     for (;;) {
         /* ---- claim frames for the empty slots ---- */
@@ -251,11 +288,8 @@ __global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
         const int fany = frame[0] >= 0 ? frame[0] : frame[1];
         if (newseg) {
             cd = a.codes[a.frames[fany].code];
-            meta = a.meta + cd.meta_off;
-            pos = a.pos + cd.meta_off;
         }
-        const uint32_t* ent = a.ent + cd.ent_off;
-        const uint4* pkl = a.pkl + cd.lay_off;
+        const uint4* skl = a.pkl + cd.lay_off;
         const int K = cd.k, R = cd.r, q = cd.q;
         /* a frame's LLRs into its slot bytes: information bits in order,
          * parity bit K + q*jj + ii at K + 360*ii + jj (layer order) */
@@ -263,13 +297,13 @@ __global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
             const DFrame fr = a.frames[f];
             const uint32_t* src = (const uint32_t*)(a.llr + fr.llr_off);
             uint8_t* sb = (uint8_t*)sm16 + s;
-            for (int v4 = tid; v4 < K / 4; v4 += PK_NT) {
+            for (int v4 = tid; v4 < K / 4; v4 += SK_NT) {
                 const uint32_t w = src[v4];
                 for (int b = 0; b < 4; b++)
                     sb[2 * (4 * v4 + b)] = (uint8_t)(w >> (8 * b));
             }
             const uint8_t* par = (const uint8_t*)(a.llr + fr.llr_off + K);
-            for (int t = tid; t < R; t += PK_NT) {
+            for (int t = tid; t < R; t += SK_NT) {
                 const int jj = t / q, ii = t - jj * q;
                 sb[2 * (K + 360 * ii + jj)] = par[t];
             }
@@ -290,7 +324,7 @@ __global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
         /* ---- iterate until a slot is done ---- */
         for (;;) {
             const uint32_t live = (frame[0] >= 0 ? 1u : 0u) | (frame[1] >= 0 ? 2u : 0u);
-            const uint32_t good = pk_syndrome(sm16, ent, pkl, tid, K, q, live, s_vote, nvote);
+            const uint32_t good = sk_syndrome(sm16, a.adt, skl, tid, q, cd.n, live, s_vote, nvote);
             uint32_t done = 0;
             for (int s = 0; s < 2; s++) {
                 const int cap = NMS && phase[s] ? a.fallback : maxit[s];
@@ -314,7 +348,7 @@ __global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
                     const int nbytes = (fr.nbits + 7) / 8;
                     uint8_t* o = a.out + fr.out_off;
                     const uint8_t* sb = (const uint8_t*)sm16 + s;
-                    for (int b4 = 4 * tid; b4 < nbytes; b4 += 4 * PK_NT) {
+                    for (int b4 = 4 * tid; b4 < nbytes; b4 += 4 * SK_NT) {
                         uint32_t v = 0;
                         for (int k = 0; k < 32; k++) {
                             const int bit = 8 * b4 + k;
@@ -362,8 +396,8 @@ __global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
             } else {
                 rows = PkRowPtrs{ nullptr, nullptr, nullptr, nullptr };
             }
-            pk_iteration(sm16, ent, pkl, a.adt, meta, pos, msg, K, q, frame[0] < 0 || it[0] == 0, frame[1] < 0 || it[1] == 0,
-                         rows, hb, a.hstride, s_vote, nvote);
+            sk_iteration(sm16, a.adt, skl, msg, q, cd.n, frame[0] < 0 || it[0] == 0, frame[1] < 0 || it[1] == 0, rows, hb,
+                         a.hstride, s_vote, nvote);
             for (int s = 0; s < 2; s++)
                 it[s] += frame[s] >= 0;
         }
@@ -373,18 +407,18 @@ __global__ void __launch_bounds__(PK_NT) ldpc_pk_kernel(PkArgs a)
 
 } // namespace
 
-#define LGK_CAT(a, b, c, d) a##b##_##c##d
-#define LGK_NAME(a, b, c, d) LGK_CAT(a, b, c, d)
+#define SK_CAT(a, b, c, d) a##b##_##c##_##d
+#define SK_NAME(a, b, c, d) SK_CAT(a, b, c, d)
 
-hipError_t LGK_NAME(pk_launch_, LGK_DM, LGK_NMS, )(int grid, size_t lds, hipStream_t st, const PkArgs& a)
+hipError_t SK_NAME(sk_launch_, SK_KS, SK_WIDE, SK_NMS)(int grid, size_t lds, hipStream_t st, const PkArgs& a)
 {
-    hipLaunchKernelGGL(ldpc_pk_kernel, dim3(grid), dim3(PK_NT), lds, st, a);
+    hipLaunchKernelGGL(ldpc_sk_kernel, dim3(grid), dim3(SK_NT), lds, st, a);
     return hipGetLastError();
 }
 
-hipError_t LGK_NAME(pk_occupancy_, LGK_DM, LGK_NMS, )(size_t lds, int* per_cu)
+hipError_t SK_NAME(sk_occupancy_, SK_KS, SK_WIDE, SK_NMS)(size_t lds, int* per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ldpc_pk_kernel, PK_NT, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ldpc_sk_kernel, SK_NT, lds);
 }
 
 } // namespace lgk

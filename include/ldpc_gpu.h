@@ -61,10 +61,12 @@ int ldpc_gpu_code_lookup(int short_frame, const char* rate);
 int ldpc_gpu_code_params(int code, int* n, int* k, int* q, int* links_total, int* level_steps);
 /* Largest information degree of a layer of the code (check degree - 2). */
 int ldpc_gpu_code_max_degree(int code);
-/* Layer degree bound of the kernel instance that decodes the code with
- * per-frame early stop (4, 8, 12, 16, 20 or 28; the key of the launches
- * ldpc_gpu_last_launches reports); LDPC_GPU_EINVAL for an unknown code.
- * Diagnostics only, no counterpart in the reference. */
+/* Kernel instance that decodes the code with per-frame early stop, as the
+ * largest layer information degree it covers (2 x slots per lane - 2),
+ * plus 100 for the instance that also takes layers several slots shorter;
+ * the key of the launches ldpc_gpu_last_launches reports.  LDPC_GPU_EINVAL
+ * for an unknown code.  Diagnostics only, no counterpart in the
+ * reference. */
 int ldpc_gpu_code_kernel_degree(int code);
 int ldpc_gpu_fec_lookup(int short_frame, const char* rate, int* nldpc, int* kldpc, int* kbch, int* t);
 

@@ -15,6 +15,7 @@
 #include "ldpc_codes.h"
 #include "ldpc_node.h"
 #include "ldpc_pk.h"
+#include "ldpc_sk.h"
 
 extern "C" {
 int kmodel_decode(int code, const int8_t* llr, int max_iter, int nbits, uint8_t* out, int* ok);
@@ -136,14 +137,7 @@ int kmodel_decode(int code, const int8_t* llr, int max_iter, int nbits, uint8_t*
     return it;
 }
 
-/* ---- the two-frame packed path (dvb-rx_amd/csrc/ldpc_pk.h) --------------- */
-
-/* learned parameters of the two slots for this layer: rows (S, B) by
- * check degree; NULL rows: the plain decoder's arithmetic (pk_mags) */
-struct PkRows {
-    const uint8_t* S[2];
-    const int16_t* B[2];
-};
+/* ---- the two-frame split path (dvb-rx_amd/csrc/ldpc_pk.h, ldpc_sk.h) ----- */
 
 /* hand-over rounds run and level steps they replace (statistics) */
 static long g_pk_rounds, g_pk_levels;
@@ -154,120 +148,154 @@ extern "C" void kmodel_pk_rounds(long* rounds, long* levels)
     *levels = g_pk_levels;
 }
 
-/* DEG: the kernel instance's layer degree bound (DM), the layer's own
- * degree D <= DEG is a run-time value as in the kernel */
-template <int DEG, bool NMS>
-static void pk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t sel_even,
-                           uint32_t sel_odd, const PkRows& rows)
+/* learned parameters of the two slots for this layer: rows (S, B) by
+ * check degree; NULL rows: the plain decoder's arithmetic */
+struct PkRows {
+    const uint8_t* S[2];
+    const int16_t* B[2];
+};
+
+/* the lane pairs exchange minimums and sign (the kernel's DPP swap) */
+template <int KS>
+static void sk_model_pair(SkHalf<KS>* n)
 {
-    const int K = c->k, q = c->q;
-    const uint64_t* meta = c->meta + (size_t)i * 360;
-    static PkNode<DEG> nodes[360];
-    const int W = LG_PK_WORDS(c->layer_start[i + 1] - c->layer_start[i]);
-    static uint32_t mw[360][LG_PK_WORDS(28)];
-    uint32_t* m = msg + (size_t)c->pk_msg_off[i] * 360;
-    lg_pk ps[360], pb[360];
     for (int j = 0; j < 360; j++) {
-        for (int w = 0; w < W; w++)
-            mw[j][w] = m[w * 360 + j];
-        ps[j] = lg_pk{ 16, 16 };
-        pb[j] = lg_pk{ 16, 16 };
+        SkHalf<KS>& a = n[2 * j];
+        SkHalf<KS>& b = n[2 * j + 1];
+        const lg_pk a0 = a.min0, a1 = a.min1, as = a.s;
+        sk_combine(a, b.min0, b.min1, b.s);
+        sk_combine(b, a0, a1, as);
+    }
+}
+
+/* One layer of the split kernel (ldpc_sk_kern.hip) on the CPU: 720 lanes,
+ * each phase over all lanes (in decreasing lane order: the kernel runs
+ * them at once, the result must not depend on the order), the code's own
+ * split tables. */
+template <int KS, int KA, bool NMS>
+static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t se, uint32_t so,
+                           const PkRows& rows)
+{
+    const uint32_t* L = c->sk_lay + 4 * i;
+    const int T = (int)(L[0] & 0xff), nlev = (int)((L[0] >> 8) & 0xff);
+    const int WA = SK_WORDS(KS);
+    const bool tail = nlev > 1;
+    static SkHalf<KS> n[720];
+    static uint32_t mw[720][8], tm[720], hw[720][8];
+    static lg_pk ps[720], pb[720];
+    static PkPart part[720];
+    uint32_t* m = msg + (size_t)L[1] * 720;
+    for (int t = 0; t < 720; t++) {
+        uint32_t aw[8];
+        for (int w = 0; w < WA; w++) {
+            aw[w] = c->sk_tab[(size_t)(L[2] + w) * 720 + t];
+            mw[t][w] = m[w * 720 + t];
+            hw[t][w] = tail ? c->sk_tab[(size_t)(L[3] + 1 + w) * 720 + t] : 0;
+        }
+        tm[t] = tail ? c->sk_tab[(size_t)L[3] * 720 + t] : 0;
+        sk_addrs(n[t], aw);
+        const int j = t >> 1;
+        ps[t] = lg_pk{ 16, 16 };
+        pb[t] = lg_pk{ 16, 16 };
         if (NMS) {
-            const int dc = c->layer_start[i + 1] - c->layer_start[i] + 2 - (i == 0 && j == 0);
-            ps[j] = lg_pk{ (short)rows.S[0][dc], (short)rows.S[1][dc] };
-            pb[j] = lg_pk{ (short)rows.B[0][dc], (short)rows.B[1][dc] };
+            const int dc = T - (i == 0 && j == 0);
+            ps[t] = lg_pk{ (short)rows.S[0][dc], (short)rows.S[1][dc] };
+            pb[t] = lg_pk{ (short)rows.B[0][dc], (short)rows.B[1][dc] };
         }
     }
-    const uint32_t* xent = c->xent + c->layer_start[i];
-    const uint64_t* pos = c->pos + (size_t)i * 360;
-    const uint32_t dupm = c->dupm[i];
-    const int D = c->layer_start[i + 1] - c->layer_start[i];
-    for (int j = 359; j >= 0; j--) {
-        pk_addrs<DEG>(nodes[j].ad, xent, D, j);
-        if (c->nlev[i] > 1)
-            pk_phase_a<DEG, true>(nodes[j], post, D, i, j, K, q, LG_META_DEP(meta[j]), mw[j], sel_even, sel_odd);
+    for (int t = 719; t >= 0; t--) {
+        const uint32_t skip0 = (i == 0 && t == 1) ? ~0u : 0u;
+        if (tail)
+            sk_phase_a<KS, KA, true>(n[t], post, T, t & 1, skip0, tm[t] & 0xffff, mw[t], se, so);
         else
-            pk_phase_a<DEG, false>(nodes[j], post, D, i, j, K, q, 0, mw[j], sel_even, sel_odd);
+            sk_phase_a<KS, KA, false>(n[t], post, T, t & 1, skip0, 0, mw[t], se, so);
     }
-    /* hand-over rounds as the kernel runs them: all nodes of a round
-     * together (here in decreasing j), two buffers, stop at the first
-     * round that reads what the round before read, or after levels - 1 */
-    if (c->nlev[i] > 1) {
-        static PkPart part[360];
+    if (tail) {
         static uint16_t hb[2][LG_MAX_XFER * 360];
-        for (int j = 359; j >= 0; j--) {
-            part[j] = pk_part(nodes[j]);
-            pk_fold_dep0<DEG>(nodes[j], D, LG_META_DEP(meta[j]));
-            pk_put_hand<DEG, NMS>(nodes[j], hb[0], xent, D, dupm, LG_META_SUCC(meta[j]), pos[j], ps[j], pb[j]);
+        for (int t = 719; t >= 0; t--) {
+            part[t] = sk_part(n[t]);
+            sk_fold_dep0(n[t], tm[t] & 0xffff);
         }
-        for (int r = 1; r < c->nlev[i]; r++) {
+        sk_model_pair(n);
+        for (int t = 719; t >= 0; t--)
+            sk_put_hand<KS, NMS>(n[t], hb[0], hw[t], tm[t] >> 16, ps[t], pb[t]);
+        for (int r = 1; r < nlev; r++) {
             uint32_t chg = 0;
-            for (int j = 359; j >= 0; j--) {
-                pk_restore(nodes[j], part[j]);
-                chg |= pk_take_hand<DEG>(nodes[j], hb[(r - 1) & 1], xent, D, dupm, LG_META_DEP(meta[j]), pos[j],
-                                         mw[j], sel_even, sel_odd);
+            for (int t = 719; t >= 0; t--) {
+                sk_restore(n[t], part[t]);
+                chg |= sk_take_hand(n[t], hb[(r - 1) & 1], hw[t], tm[t] & 0xffff, mw[t], se, so);
             }
+            sk_model_pair(n);
             g_pk_rounds++;
-            if (r == c->nlev[i] - 1)
+            if (r == nlev - 1)
                 break;
-            for (int j = 359; j >= 0; j--)
-                pk_put_hand<DEG, NMS>(nodes[j], hb[r & 1], xent, D, dupm, LG_META_SUCC(meta[j]), pos[j], ps[j],
-                                      pb[j]);
+            for (int t = 719; t >= 0; t--)
+                sk_put_hand<KS, NMS>(n[t], hb[r & 1], hw[t], tm[t] >> 16, ps[t], pb[t]);
             if (!chg)
                 break;
         }
-        g_pk_levels += c->nlev[i] - 1;
+        g_pk_levels += nlev - 1;
+    } else {
+        sk_model_pair(n);
     }
-    for (int j = 359; j >= 0; j--) {
-        if (c->nlev[i] > 1)
-            pk_phase_c<DEG, true, NMS>(nodes[j], post, D, LG_META_SUCC(meta[j]), m + j, 360, ps[j], pb[j]);
+    for (int t = 719; t >= 0; t--) {
+        if (tail)
+            sk_phase_c<KS, true, NMS>(n[t], post, tm[t] >> 16, m + t, 720, ps[t], pb[t]);
         else
-            pk_phase_c<DEG, false, NMS>(nodes[j], post, D, 0, m + j, 360, ps[j], pb[j]);
+            sk_phase_c<KS, false, NMS>(n[t], post, 0, m + t, 720, ps[t], pb[t]);
     }
 }
 
-/* the kernel instance of the code: the smallest layer degree bound that
- * covers its layers (the library's choice, 4 8 12 16 20 28) */
-static int pk_dm(const lg_code* c)
-{
-    static const int dms[] = { 4, 8, 12, 16, 20, 28 };
-    for (int d : dms)
-        if (c->max_layer_deg <= d)
-            return d;
-    return -1;
-}
-
-#define PK_DM_CASES(X) X(4) X(8) X(12) X(16) X(20) X(28)
+#define SK_MODEL_INSTS(X) X(2, 0) X(3, 0) X(4, 0) X(4, 1) X(5, 0) X(6, 0) X(7, 0) X(7, 1) X(8, 0) X(9, 0) \
+    X(10, 0) X(10, 1) X(11, 0) X(14, 0) X(15, 0)
 
 static int pk_layer_any(const lg_code* c, int i, uint16_t* post, uint32_t* msg, uint32_t se, uint32_t so,
                         const PkRows* rows)
 {
     static const PkRows none = { { nullptr, nullptr }, { nullptr, nullptr } };
-    switch (pk_dm(c)) {
-#define X(D)                                                         \
-    case D:                                                          \
-        if (rows)                                                    \
-            pk_model_layer<D, true>(c, i, post, msg, se, so, *rows); \
-        else                                                         \
-            pk_model_layer<D, false>(c, i, post, msg, se, so, none); \
-        return 0;
-        PK_DM_CASES(X)
-#undef X
+#define X(KSX, W)                                                                                     \
+    if (c->sk_ks == KSX && c->sk_wide == W) {                                                       \
+        if (rows)                                                                                   \
+            sk_model_layer<KSX, W ? 1 : KSX - 1, true>(c, i, post, msg, se, so, *rows);             \
+        else                                                                                        \
+            sk_model_layer<KSX, W ? 1 : KSX - 1, false>(c, i, post, msg, se, so, none);            \
+        return 0;                                                                                   \
     }
+    SK_MODEL_INSTS(X)
+#undef X
     return -1;
 }
 
-static uint32_t pk_check_any(const lg_code* c, int i, int j, const uint16_t* post)
+/* syndrome of layer i for both slots (bit f: frame f holds every check) */
+template <int KS, int KA>
+static uint32_t sk_model_check(const lg_code* c, int i, const uint16_t* post)
 {
-    const uint32_t* e = c->xent + c->layer_start[i];
-    const int D = c->layer_start[i + 1] - c->layer_start[i];
-    switch (pk_dm(c)) {
-#define X(DMX) \
-    case DMX:  \
-        return pk_check<DMX>(post, e, D, i, j, c->k, c->q);
-        PK_DM_CASES(X)
-#undef X
+    const uint32_t* L = c->sk_lay + 4 * i;
+    const int T = (int)(L[0] & 0xff);
+    uint32_t good = 3;
+    for (int j = 0; j < 360; j++) {
+        uint32_t par[2], nz[2];
+        for (int h = 0; h < 2; h++) {
+            const int t = 2 * j + h;
+            uint32_t aw[8];
+            for (int w = 0; w < SK_WORDS(KS); w++)
+                aw[w] = c->sk_tab[(size_t)(L[2] + w) * 720 + t];
+            SkHalf<KS> n;
+            sk_addrs(n, aw);
+            sk_check<KS, KA>(post, n.ad, T, h, (i == 0 && t == 1) ? ~0u : 0u, par[h], nz[h]);
+        }
+        good &= sk_check_bits(par[0] ^ par[1], nz[0] & nz[1]);
     }
+    return good;
+}
+
+static uint32_t pk_check_layer(const lg_code* c, int i, const uint16_t* post)
+{
+#define X(KSX, W)                                                        \
+    if (c->sk_ks == KSX && c->sk_wide == W)                            \
+        return sk_model_check<KSX, W ? 1 : KSX - 1>(c, i, post);
+    SK_MODEL_INSTS(X)
+#undef X
     return 0;
 }
 
@@ -287,8 +315,9 @@ static int pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay
     if (!c)
         return -1;
     const int K = c->k, q = c->q, N = c->n;
-    uint16_t* post = (uint16_t*)calloc((size_t)N, sizeof(uint16_t));
-    uint32_t* msg = (uint32_t*)calloc((size_t)c->pk_msg_words, sizeof(uint32_t));
+    /* posteriors and the spare cell of empty slots */
+    uint16_t* post = (uint16_t*)calloc((size_t)N + 1, sizeof(uint16_t));
+    uint32_t* msg = (uint32_t*)calloc((size_t)c->sk_msg_rows * 720, sizeof(uint32_t));
     if (!post || !msg) {
         free(post);
         free(msg);
@@ -321,8 +350,7 @@ static int pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay
         }
         uint32_t good = 3;
         for (int i = 0; i < q; i++)
-            for (int j = 0; j < 360; j++)
-                good &= pk_check_any(c, i, j, post);
+            good &= pk_check_layer(c, i, post);
         for (int s = 0; s < 2; s++) {
             if (!active[s])
                 continue;
