@@ -250,6 +250,8 @@ static int upload_tables(ldpc_gpu_ctx* c)
     return LDPC_GPU_OK;
 }
 
+static int static_init(ldpc_gpu_ctx* c);
+
 extern "C" {
 
 int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
@@ -288,7 +290,8 @@ int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
     }
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         c->ncu = 256;
-    if ((rc = upload_tables(c)) != LDPC_GPU_OK || hipMalloc((void**)&c->ticket, NCOUNTERS * sizeof(unsigned)) != hipSuccess) {
+    if ((rc = upload_tables(c)) != LDPC_GPU_OK || (rc = static_init(c)) != LDPC_GPU_OK ||
+        hipMalloc((void**)&c->ticket, NCOUNTERS * sizeof(unsigned)) != hipSuccess) {
         ldpc_gpu_destroy(c);
         return rc ? rc : LDPC_GPU_EDEVICE;
     }
@@ -1042,6 +1045,152 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
         }
         if (!more)
             break;
+    }
+    return LDPC_GPU_OK;
+}
+
+/* ---- decoding without allocation (ldpc_gpu_decode_batch_static) --------
+ *
+ * All device scratch of the call (frame table, frame list, segments, claim
+ * counters, message blocks) lives in `scratch`, a fixed array of this
+ * library on every device it runs on, and the host layout in fixed host
+ * arrays; the call makes no allocation.  The array's device address is
+ * taken once per device when a context is created (static_init) and used
+ * for every launch.  Calls on one device share the array (and the host
+ * staging, under g_static_lock): they are serialised on the host, and the
+ * launches of one call run one after the other on the context's stream. */
+#define STATIC_SCRATCH_BYTES ((size_t)96 << 20)
+#define STATIC_MAX_FRAMES 65536
+
+static __device__ __align__(256) uint8_t scratch[STATIC_SCRATCH_BYTES];
+
+static std::mutex g_static_lock;
+static uint8_t* g_static_ptr[64];          /* scratch's address on each device */
+static DFrame g_static_frames[STATIC_MAX_FRAMES];
+static int g_static_list[STATIC_MAX_FRAMES];
+static int2 g_static_segs[NCOUNTERS];
+
+static int static_init(ldpc_gpu_ctx* c)
+{
+    std::lock_guard<std::mutex> lk(g_static_lock);
+    if (g_static_ptr[c->device])
+        return LDPC_GPU_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_err(c, LDPC_GPU_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_static_ptr[c->device] = (uint8_t*)p;
+    return LDPC_GPU_OK;
+}
+
+extern "C" int ldpc_gpu_decode_batch_static(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes,
+                                            const int8_t* d_llr, uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    if (nframes == 0)
+        return LDPC_GPU_OK;
+    if (nframes < 0 || !frames || !d_llr || !d_out || !d_iters || !d_synd)
+        return set_err(c, LDPC_GPU_EINVAL, "bad arguments");
+    if (nframes > STATIC_MAX_FRAMES)
+        return set_err(c, LDPC_GPU_EINVAL, "static decode: %d frames, at most %d", nframes, STATIC_MAX_FRAMES);
+    HIPCHK(c, hipSetDevice(c->device));
+    std::lock_guard<std::mutex> lk(g_static_lock);
+    uint8_t* base = g_static_ptr[c->device];
+    if (!base)
+        return set_err(c, LDPC_GPU_EDEVICE, "static scratch not set up on device %d", c->device);
+    const DCode* hc = c->tab->h_codes.data();
+    const int ncode = (int)c->tab->h_codes.size();
+    /* frame table (as check_frames) and the launch / code of each frame */
+    int maxw = 0, maxn[NLAUNCH] = { 0 }, maxx[NLAUNCH] = { 0 }, nfr[NLAUNCH] = { 0 };
+    for (int f = 0; f < nframes; f++) {
+        const ldpc_gpu_frame& s = frames[f];
+        if (s.code < 0 || s.code >= ncode)
+            return set_err(c, LDPC_GPU_EINVAL, "frame %d: unknown code %d", f, s.code);
+        const DCode& cd = hc[s.code];
+        if (s.max_iter < 0 || s.nbits < 0 || s.nbits > cd.k || (s.llr_off & 7))
+            return set_err(c, LDPC_GPU_EINVAL, "frame %d: bad max_iter/nbits/llr_off", f);
+        if (s.group >= 0)
+            return set_err(c, LDPC_GPU_EINVAL, "static decode takes frames with their own early stop (group -1)");
+        DFrame& d = g_static_frames[f];
+        d.code = s.code;
+        d.max_iter = s.max_iter;
+        d.nbits = s.nbits ? s.nbits : cd.k;
+        d.pad = 0;
+        d.llr_off = s.llr_off;
+        d.out_off = s.out_off;
+        const int k = cd.sk_inst * 2 + (cd.n < 64800);
+        nfr[k]++;
+        maxn[k] = std::max(maxn[k], cd.n);
+        maxx[k] = std::max(maxx[k], cd.max_xfer);
+        maxw = std::max(maxw, cd.pk_msg_words);
+    }
+    /* frame list grouped by launch, then by code (one segment per code) */
+    int nseg = 0, seg0[NLAUNCH], nsegk[NLAUNCH], pos = 0;
+    for (int k = 0; k < NLAUNCH; k++) {
+        seg0[k] = nseg;
+        nsegk[k] = 0;
+        if (!nfr[k])
+            continue;
+        for (int code = 0; code < ncode; code++) {
+            if (hc[code].sk_inst * 2 + (hc[code].n < 64800) != k)
+                continue;
+            const int start = pos;
+            for (int f = 0; f < nframes; f++)
+                if (g_static_frames[f].code == code)
+                    g_static_list[pos++] = f;
+            if (pos == start)
+                continue;
+            if (nseg >= NCOUNTERS)
+                return set_err(c, LDPC_GPU_EINVAL, "too many codes in one batch");
+            g_static_segs[nseg++] = make_int2(start, pos - start);
+            nsegk[k]++;
+        }
+    }
+    /* scratch: frames | list | segments | counters | message blocks */
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_list = up(sizeof(DFrame) * (size_t)nframes);
+    const size_t o_seg = o_list + up(sizeof(int) * (size_t)nframes);
+    const size_t o_cnt = o_seg + up(sizeof(int2) * NCOUNTERS);
+    const size_t o_msg = o_cnt + up(sizeof(unsigned) * NCOUNTERS);
+    const size_t msg_blocks = (STATIC_SCRATCH_BYTES - o_msg) / (sizeof(uint32_t) * (size_t)maxw);
+    if (msg_blocks < 1)
+        return set_err(c, LDPC_GPU_EINVAL, "static decode: scratch of %zu bytes too small", STATIC_SCRATCH_BYTES);
+    DFrame* d_frames = (DFrame*)base;
+    int* d_list = (int*)(base + o_list);
+    int2* d_seg = (int2*)(base + o_seg);
+    unsigned* d_cnt = (unsigned*)(base + o_cnt);
+    uint32_t* d_msg = (uint32_t*)(base + o_msg);
+    /* the host staging is reused by the next call: the copies complete
+     * before this call returns */
+    HIPCHK(c, hipMemcpyAsync(d_frames, g_static_frames, sizeof(DFrame) * (size_t)nframes, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_list, g_static_list, sizeof(int) * (size_t)nframes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_seg, g_static_segs, sizeof(int2) * (size_t)nseg, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, reset_counters(c->stream, d_cnt));
+    PkArgs a = pk_args(c, d_frames, d_list, d_llr, d_out, d_iters, d_synd, false, 0, nullptr);
+    a.msg = d_msg;
+    a.msg_stride = (size_t)maxw;
+    c->timed = 0;
+    for (int k = 0; k < NLAUNCH; k++) {
+        if (!nfr[k])
+            continue;
+        PkLayout L;
+        L.maxn[k] = maxn[k];
+        L.maxx[k] = maxx[k];
+        const size_t lds = pk_lds(L, k);
+        int grid = 0;
+        int rc = pk_grid(c, k / 2, false, lds, nfr[k], &grid);
+        if (rc)
+            return rc;
+        grid = (int)std::min<size_t>((size_t)grid, msg_blocks);
+        a.seg = d_seg + seg0[k];
+        a.nseg = nsegk[k];
+        a.cnt = d_cnt + seg0[k];
+        a.hoff = pk_hoff(L, k);
+        a.hstride = 360 * maxx[k];
+        HIPCHK(c, pk_launch(k / 2, false, grid, lds, c->stream, a));
     }
     return LDPC_GPU_OK;
 }

@@ -84,6 +84,7 @@ def gpu_lib():
         L.ldpc_gpu_t2_decode_host.argtypes = [vp, vp, i32, vp, vp, i32, i32, vp, vp, vp, vp]
         L.ldpc_gpu_t2_decode.argtypes = [vp, vp, i32, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp]
         L.ldpc_gpu_decode_batch.argtypes = [vp, vp, i32, vp, vp, vp, vp]
+        L.ldpc_gpu_decode_batch_static.argtypes = [vp, vp, i32, vp, vp, vp, vp]
         L.ldpc_gpu_decode_host.argtypes = [vp, vp, i32, vp, sz, vp, sz, vp, vp]
         L.ldpc_gpu_last_kernel_ms.argtypes = [vp]
         L.ldpc_gpu_last_kernel_ms.restype = C.c_float

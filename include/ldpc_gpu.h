@@ -97,6 +97,28 @@ int ldpc_gpu_query(ldpc_gpu_ctx* ctx);
 int ldpc_gpu_decode_batch(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nframes,
                           const int8_t* d_llr, uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd);
 
+/*
+ * ldpc_gpu_decode_batch without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device or host memory
+ * during decoding (tight serving loops, harnesses that forbid it).  Same
+ * arguments and results as ldpc_gpu_decode_batch for frames with their own
+ * early stop (every group -1; groups are rejected with LDPC_GPU_EINVAL).
+ *
+ * The call's device scratch (frame table, lists, counters and the
+ * decoder's message blocks) is a fixed 96 MiB array of the library on each
+ * device, set up when a context is created there and held until the
+ * process ends; at most 65536 frames per call (LDPC_GPU_EINVAL beyond).
+ * The launches run one after the other on the context's stream with as
+ * many workgroups as that scratch holds, so large batches decode more
+ * slowly than with ldpc_gpu_decode_batch.  Calls on one device share the
+ * array: two calls whose work overlaps on the device (different contexts
+ * or streams of one GPU) race; calls on one context are ordered.  The
+ * frame table is copied before the call returns; the decode is enqueued on
+ * the context's stream (results valid after ldpc_gpu_synchronize).
+ */
+int ldpc_gpu_decode_batch_static(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
+                                 uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd);
+
 /* Same on host buffers: copies in, decodes, copies out (synchronous). */
 int ldpc_gpu_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nframes,
                          const int8_t* llr, size_t llr_bytes, uint8_t* out, size_t out_bytes,
