@@ -300,35 +300,22 @@ extern "C" int ldpc_gpu_demap_symbols(int pls)
     return (nldpc + row->bits - 1) / row->bits;
 }
 
-extern "C" int ldpc_gpu_demap(ldpc_gpu_ctx* ctx, int pls, int nframes, const float* d_sym, const float* d_n0,
-                              int8_t* d_llr)
+/* the demap kernel of MODCOD pls over nframes frames on stream st of
+ * device dev (shared by ldpc_gpu_demap and the GPU channel) */
+int lg_demap_launch(int dev, int pls, int nframes, const float* d_sym, const float* d_n0, int8_t* d_llr,
+                    hipStream_t st)
 {
-    if (!ctx)
-        return LDPC_GPU_EINVAL;
-    const lg_modcod_row* row = lg_modcod_get(pls);
-    if (!row)
-        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "demap: unknown PLS code");
-    if (nframes < 0 || (nframes > 0 && (!d_sym || !d_n0 || !d_llr)))
-        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "demap: bad arguments");
-    if (nframes == 0)
-        return LDPC_GPU_OK;
-    if (nframes > 65535)
-        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "demap: at most 65535 frames per call");
-    const int dev = lg_ctx_device(ctx);
     if (dev < 0 || dev >= DM_MAX_DEV)
-        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "demap: device index");
-    if (hipSetDevice(dev) != hipSuccess)
-        return lg_ctx_fail(ctx, LDPC_GPU_EDEVICE, "demap: hipSetDevice");
+        return LDPC_GPU_EINVAL;
     DemapTables* t = nullptr;
     int rc = get_tables(dev, &t);
     if (rc)
-        return lg_ctx_fail(ctx, rc, "demap: point tables");
+        return rc;
     const int idx = t->index[(pls & 0xFE) >> 1];
     if (idx < 0)
-        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "demap: no constellation for this PLS code");
+        return LDPC_GPU_EINVAL;
     const DemapTab& h = t->host[idx];
     const dim3 grid((h.nsym + DM_BLOCK - 1) / DM_BLOCK, nframes);
-    const hipStream_t st = (hipStream_t)ldpc_gpu_stream(ctx);
     const DemapTab* tab = t->d_tab + idx;
     const float2* sym = (const float2*)d_sym;
     switch (h.qpsk ? 2 : h.bits) {
@@ -354,9 +341,30 @@ extern "C" int ldpc_gpu_demap(ldpc_gpu_ctx* ctx, int pls, int nframes, const flo
         hipLaunchKernelGGL(demap_kernel<8>, grid, dim3(DM_BLOCK), 0, st, tab, sym, d_n0, d_llr);
         break;
     default:
-        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "demap: unsupported constellation");
+        return LDPC_GPU_EINVAL;
     }
-    if (hipGetLastError() != hipSuccess)
-        return lg_ctx_fail(ctx, LDPC_GPU_EDEVICE, "demap: launch failed");
+    return hipGetLastError() == hipSuccess ? LDPC_GPU_OK : LDPC_GPU_EDEVICE;
+}
+
+extern "C" int ldpc_gpu_demap(ldpc_gpu_ctx* ctx, int pls, int nframes, const float* d_sym, const float* d_n0,
+                              int8_t* d_llr)
+{
+    if (!ctx)
+        return LDPC_GPU_EINVAL;
+    const lg_modcod_row* row = lg_modcod_get(pls);
+    if (!row)
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "demap: unknown PLS code");
+    if (nframes < 0 || (nframes > 0 && (!d_sym || !d_n0 || !d_llr)))
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "demap: bad arguments");
+    if (nframes == 0)
+        return LDPC_GPU_OK;
+    if (nframes > 65535)
+        return lg_ctx_fail(ctx, LDPC_GPU_EINVAL, "demap: at most 65535 frames per call");
+    const int dev = lg_ctx_device(ctx);
+    if (hipSetDevice(dev) != hipSuccess)
+        return lg_ctx_fail(ctx, LDPC_GPU_EDEVICE, "demap: hipSetDevice");
+    const int rc = lg_demap_launch(dev, pls, nframes, d_sym, d_n0, d_llr, (hipStream_t)ldpc_gpu_stream(ctx));
+    if (rc)
+        return lg_ctx_fail(ctx, rc, rc == LDPC_GPU_EDEVICE ? "demap: launch failed" : "demap: no constellation for this PLS code");
     return LDPC_GPU_OK;
 }

@@ -12,4 +12,9 @@ int lg_ctx_device(ldpc_gpu_ctx* c);
 /* record msg as the context's last error; returns code */
 int lg_ctx_fail(ldpc_gpu_ctx* c, int code, const char* msg);
 
+/* the max-log demap kernel (ldpc_demap.hip) of MODCOD pls over nframes
+ * frames on stream st (a hipStream_t) of device dev; LDPC_GPU_* code */
+int lg_demap_launch(int dev, int pls, int nframes, const float* d_sym, const float* d_n0, int8_t* d_llr,
+                    struct ihipStream_t* st);
+
 #endif

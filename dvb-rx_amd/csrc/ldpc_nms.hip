@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "dvbs2_table_types.h"
+#include "ldpc_internal.h"
 #include "ldpc_codes.h"
 #include "ldpc_gpu.h"
 #include "ldpc_nms.h"
@@ -382,46 +383,43 @@ __device__ __forceinline__ int8_t quant8(float v)
     return (int8_t)rintf(v);
 }
 
+/* the unit channel of tests/test_ldpc.c:64-73: +-1 per bit, AWGN of
+ * variance 1 / Es/N0, llr = sat127(rint(4 y / sigma^2)) */
 __global__ void channel_kernel(ChanArgs a, const uint8_t* cw, const int32_t* row, const float* esn0_db, uint64_t seed,
                                int8_t* llr)
 {
-    const size_t per = a.unit ? (size_t)a.n : (size_t)a.nsym;
+    const size_t per = (size_t)a.n;
     const size_t total = per * a.nframes;
     for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
         const int f = (int)(g / per), j = (int)(g % per);
         const uint8_t* bits = cw + (size_t)row[f] * a.n;
-        int8_t* out = llr + (size_t)f * a.n;
-        const float esn0 = exp10f(esn0_db[f] * 0.1f);
+        const float s2 = 1.0f / exp10f(esn0_db[f] * 0.1f);
         const float2 z = gauss2(seed, (uint64_t)f, (uint64_t)j);
-        if (a.unit) {
-            const float s2 = 1.0f / esn0;
-            const float y = (bits[j] ? -1.0f : 1.0f) + sqrtf(s2) * z.x;
-            out[j] = quant8(4.0f * y / s2);
-            continue;
-        }
-        const float n0 = 1.0f / esn0, sn = sqrtf(n0 * 0.5f), k = 2.0f / n0;
-        int label = 0, pos[8];
+        const float y = (bits[j] ? -1.0f : 1.0f) + sqrtf(s2) * z.x;
+        llr[(size_t)f * a.n + j] = quant8(4.0f * y / s2);
+    }
+}
+
+/* the channel up to the receiver's symbols: frame f's bits mapped (bit
+ * interleaver as the demapper undoes it) to unit-energy points plus
+ * complex AWGN of variance N0 = 1 / Es/N0; n0[f] = N0 */
+__global__ void symbol_kernel(ChanArgs a, const uint8_t* cw, const int32_t* row, const float* esn0_db, uint64_t seed,
+                              int f0, float2* sym, float* n0)
+{
+    const size_t per = (size_t)a.nsym, total = per * a.nframes;
+    for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        const int fl = (int)(g / per), j = (int)(g % per), f = f0 + fl;
+        const uint8_t* bits = cw + (size_t)row[f] * a.n;
+        const float n0f = 1.0f / exp10f(esn0_db[f] * 0.1f), sn = sqrtf(n0f * 0.5f);
+        const float2 z = gauss2(seed, (uint64_t)f, (uint64_t)j);
+        int label = 0;
         for (int b = 0; b < a.bits; b++) {
-            pos[b] = a.order[0] < 0 ? a.bits * j + b : a.order[b] * a.nsym + j;
-            label = label << 1 | (pos[b] < a.n ? bits[pos[b]] : 0);
+            const int pos = a.order[0] < 0 ? a.bits * j + b : a.order[b] * a.nsym + j;
+            label = label << 1 | (pos < a.n ? bits[pos] : 0);
         }
-        const float x = a.pts[2 * label] + sn * z.x, y = a.pts[2 * label + 1] + sn * z.y;
-        float m0[8], m1[8];
-        for (int b = 0; b < a.bits; b++)
-            m0[b] = m1[b] = 3.0e38f;
-        for (int p = 0; p < a.np; p++) {
-            const float dx = x - a.pts[2 * p], dy = y - a.pts[2 * p + 1];
-            const float d = dx * dx + dy * dy;
-            for (int b = 0; b < a.bits; b++) {
-                if ((p >> (a.bits - 1 - b)) & 1)
-                    m1[b] = fminf(m1[b], d);
-                else
-                    m0[b] = fminf(m0[b], d);
-            }
-        }
-        for (int b = 0; b < a.bits; b++)
-            if (pos[b] < a.n)
-                out[pos[b]] = quant8(k * (m1[b] - m0[b]));
+        sym[g] = make_float2(a.pts[2 * label] + sn * z.x, a.pts[2 * label + 1] + sn * z.y);
+        if (j == 0)
+            n0[fl] = n0f;
     }
 }
 
@@ -662,42 +660,94 @@ int ldpc_nms_adam(ldpc_nms_trainer* t, float* d_param, const float* d_grad, floa
     return LDPC_GPU_OK;
 }
 
-int ldpc_gpu_channel(int pls, int n, const uint8_t* d_cw, const int32_t* d_row, int nframes, const float* d_esn0_db,
-                     uint64_t seed, int8_t* d_llr, void* stream)
+/* channel arguments of MODCOD pls (pls < 0: the unit channel) */
+static int chan_args(int pls, int n, int nframes, ChanArgs& a)
 {
-    if (n < 1 || nframes < 1 || !d_cw || !d_row || !d_esn0_db || !d_llr)
-        return LDPC_GPU_EINVAL;
-    ChanArgs a;
     memset(&a, 0, sizeof(a));
     a.n = n;
     a.nframes = nframes;
     if (pls < 0) {
         a.unit = 1;
         a.bits = 1;
-    } else {
-        const lg_modcod_row* m = lg_modcod_get(pls);
-        if (!m || m->bits < 1 || m->bits > 8)
-            return LDPC_GPU_EINVAL;
-        a.bits = m->bits;
-        a.np = 1 << m->bits;
-        if (a.np > 256)
-            return LDPC_GPU_EINVAL;
-        a.nsym = (n + a.bits - 1) / a.bits;
-        for (int b = 0; b < 8; b++)
-            a.order[b] = m->order[b];
-        const float* pts = lg_points() + 2 * m->point_off;
-        double e = 0;
-        for (int p = 0; p < a.np; p++)
-            e += (double)pts[2 * p] * pts[2 * p] + (double)pts[2 * p + 1] * pts[2 * p + 1];
-        const double gn = 1.0 / sqrt(e / a.np);
-        for (int p = 0; p < 2 * a.np; p++)
-            a.pts[p] = (float)(pts[p] * gn);
+        return LDPC_GPU_OK;
     }
-    const size_t total = (size_t)(a.unit ? n : a.nsym) * nframes;
+    const lg_modcod_row* m = lg_modcod_get(pls);
+    if (!m || m->bits < 1 || m->bits > 8)
+        return LDPC_GPU_EINVAL;
+    a.bits = m->bits;
+    a.np = 1 << m->bits;
+    a.nsym = (n + a.bits - 1) / a.bits;
+    for (int b = 0; b < 8; b++)
+        a.order[b] = m->order[b];
+    const float* pts = lg_points() + 2 * m->point_off;
+    double e = 0;
+    for (int p = 0; p < a.np; p++)
+        e += (double)pts[2 * p] * pts[2 * p] + (double)pts[2 * p + 1] * pts[2 * p + 1];
+    const double gn = 1.0 / sqrt(e / a.np);
+    for (int p = 0; p < 2 * a.np; p++)
+        a.pts[p] = (float)(pts[p] * gn);
+    return LDPC_GPU_OK;
+}
+
+int ldpc_gpu_channel_symbols(int pls, int n, const uint8_t* d_cw, const int32_t* d_row, int nframes,
+                             const float* d_esn0_db, uint64_t seed, float* d_sym, float* d_n0, void* stream)
+{
+    if (pls < 0 || n < 1 || nframes < 1 || !d_cw || !d_row || !d_esn0_db || !d_sym || !d_n0)
+        return LDPC_GPU_EINVAL;
+    ChanArgs a;
+    if (chan_args(pls, n, nframes, a))
+        return LDPC_GPU_EINVAL;
+    const size_t total = (size_t)a.nsym * nframes;
     const int grid = (int)std::min<size_t>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(channel_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, d_cw, d_row, d_esn0_db, seed,
-                       d_llr);
+    hipLaunchKernelGGL(symbol_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, d_cw, d_row, d_esn0_db, seed, 0,
+                       (float2*)d_sym, d_n0);
     return hipGetLastError() == hipSuccess ? LDPC_GPU_OK : LDPC_GPU_EDEVICE;
+}
+
+int ldpc_gpu_channel(int pls, int n, const uint8_t* d_cw, const int32_t* d_row, int nframes, const float* d_esn0_db,
+                     uint64_t seed, int8_t* d_llr, void* stream)
+{
+    if (n < 1 || nframes < 1 || !d_cw || !d_row || !d_esn0_db || !d_llr)
+        return LDPC_GPU_EINVAL;
+    ChanArgs a;
+    if (chan_args(pls, n, nframes, a))
+        return LDPC_GPU_EINVAL;
+    const hipStream_t st = (hipStream_t)stream;
+    if (a.unit) {
+        const size_t total = (size_t)n * nframes;
+        const int grid = (int)std::min<size_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(channel_kernel, dim3(grid), dim3(256), 0, st, a, d_cw, d_row, d_esn0_db, seed, d_llr);
+        return hipGetLastError() == hipSuccess ? LDPC_GPU_OK : LDPC_GPU_EDEVICE;
+    }
+    /* MODCOD frames: symbols, then the library's demapper (ldpc_gpu_demap's
+     * kernel, the reference's LLRs for these symbols), in chunks */
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return LDPC_GPU_EDEVICE;
+    const int chunk = std::min(nframes, 2048);
+    float2* sym = nullptr;
+    float* n0 = nullptr;
+    if (hipMalloc((void**)&sym, sizeof(float2) * (size_t)a.nsym * chunk) != hipSuccess ||
+        hipMalloc((void**)&n0, sizeof(float) * (size_t)chunk) != hipSuccess) {
+        (void)hipFree(sym);
+        return LDPC_GPU_ENOMEM;
+    }
+    int rc = LDPC_GPU_OK;
+    for (int f0 = 0; f0 < nframes && !rc; f0 += chunk) {
+        a.nframes = std::min(chunk, nframes - f0);
+        const size_t total = (size_t)a.nsym * a.nframes;
+        const int grid = (int)std::min<size_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(symbol_kernel, dim3(grid), dim3(256), 0, st, a, d_cw, d_row, d_esn0_db, seed, f0, sym, n0);
+        if (hipGetLastError() != hipSuccess)
+            rc = LDPC_GPU_EDEVICE;
+        else
+            rc = lg_demap_launch(dev, pls, a.nframes, (const float*)sym, n0, d_llr + (size_t)f0 * n, st);
+    }
+    if (hipStreamSynchronize(st) != hipSuccess && !rc)
+        rc = LDPC_GPU_EDEVICE;
+    (void)hipFree(sym);
+    (void)hipFree(n0);
+    return rc;
 }
 
 } // extern "C"

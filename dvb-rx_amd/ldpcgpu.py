@@ -119,6 +119,7 @@ def gpu_lib():
         L.ldpc_nms_adam.argtypes = [vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, i32]
         L.ldpc_nms_quantize.argtypes = [vp, vp, i32, vp, vp]
         L.ldpc_gpu_channel.argtypes = [i32, i32, vp, vp, i32, vp, C.c_uint64, vp, vp]
+        L.ldpc_gpu_channel_symbols.argtypes = [i32, i32, vp, vp, i32, vp, C.c_uint64, vp, vp, vp]
         _gpu = L
     return _gpu
 

@@ -70,16 +70,25 @@ void ldpc_nms_quantize(const float* alpha, const float* beta, int T, uint8_t* S,
 
 /*
  * GPU channel for training and evaluation data: codeword rows (one byte
- * per bit) -> the MODCOD's constellation (PLS code, as drx_pls_lookup;
- * pls < 0: +-1 per bit) with its bit interleaver -> complex AWGN at
- * d_esn0_db[f] (dB, per frame) -> max-log demapping -> int8 LLRs
- * quantise(2/N0 x (d1 - d0)), saturated to +-127 (positive for bit 0).
- * Noise from a counter-based generator keyed by (seed, frame, symbol).
- * n: codeword length of the rows.  Enqueued on stream (a hipStream_t, or
- * NULL for the default stream).
+ * per bit) -> the MODCOD's constellation (PLS code, as drx_pls_lookup)
+ * with its bit interleaver -> complex AWGN at d_esn0_db[f] (dB, per
+ * frame) -> the library's max-log demapper (ldpc_gpu_demap's kernel: the
+ * reference's int8 LLRs for those symbols), positive for bit 0.  pls < 0:
+ * +-1 per bit, llr = sat127(rint(4 y / sigma^2)).  Noise from a
+ * counter-based generator keyed by (seed, frame, symbol).  n: codeword
+ * length of the rows.  Runs on stream (a hipStream_t, or NULL for the
+ * default stream); MODCOD frames go through a symbol buffer of the call
+ * and the call returns when the LLRs are written.
  */
 int ldpc_gpu_channel(int pls, int n, const uint8_t* d_cw, const int32_t* d_row, int nframes,
                      const float* d_esn0_db, uint64_t seed, int8_t* d_llr, void* stream);
+
+/* The received symbols of ldpc_gpu_channel's MODCOD frames (pls >= 0), the
+ * input of its demapping: d_sym nframes x ldpc_gpu_demap_symbols(pls)
+ * (re, im) float pairs at unit average energy, d_n0 the frames' noise
+ * variances, as ldpc_gpu_demap takes them.  Enqueued on stream. */
+int ldpc_gpu_channel_symbols(int pls, int n, const uint8_t* d_cw, const int32_t* d_row, int nframes,
+                             const float* d_esn0_db, uint64_t seed, float* d_sym, float* d_n0, void* stream);
 
 #ifdef __cplusplus
 }
