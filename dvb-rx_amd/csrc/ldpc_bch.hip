@@ -11,7 +11,9 @@
  *     the generator (the minimal polynomials of alpha^1, alpha^3, ...,
  *     alpha^(2t-1)) divides c, the reference's remainder test; the even
  *     ones are squares of the odd ones;
- *   - Berlekamp-Massey (bch.c:229-254); more than t errors: -1;
+ *   - the error locator by the inversionless Berlekamp-Massey algorithm
+ *     (same register length and roots as bch.c:229-254's); more than t
+ *     errors: -1;
  *   - Chien search over the nbch positions: a root at alpha^-p is an error
  *     at degree p (bch.c:256-289); a root count other than the locator's
  *     degree: -1, the codeword unchanged; else the bits are flipped and
@@ -22,7 +24,7 @@
  * alpha^(j p0) times its own polynomial at alpha^j, whose log comes from a
  * 256-entry table per j in LDS (exponents reduced mod 2^m - 1 by folding),
  * then a workgroup XOR reduction.  Codewords with a
- * non-zero syndrome (few, after LDPC decoding) run Berlekamp-Massey on one
+ * non-zero syndrome (few, after LDPC decoding) find the locator on one
  * thread and the Chien search on all, positions strided by 256 with
  * incremental exponents.  The field tables (exp, log) are read from
  * device memory (256 KB for GF(2^16), L2-resident).
@@ -188,8 +190,20 @@ __global__ void __launch_bounds__(BCH_NT) bch_kernel(BchArgs a)
     }
     __syncthreads();
 
-    /* ---- Berlekamp-Massey on one thread ---- */
+    /* ---- error locator on one thread: the inversionless
+     * Berlekamp-Massey algorithm (Reed, Shih and Truong, IEE Proc. E 138,
+     * 1991; the "iBM" recursion of Sarwate and Shanbhag, IEEE Trans. VLSI
+     * 9(5), 2001).  With S_1 .. S_2t, for r = 0 .. 2t - 1:
+     *   delta = sum_i lam_i S_(r+1-i)
+     *   lam'  = gam lam + delta x bb
+     *   if delta != 0 and kk >= 0: bb = lam, gam = delta, kk = -kk - 1
+     *   else:                      bb = x bb,            kk = kk + 1
+     * from lam = bb = 1, gam = 1, kk = 0.  kk = r - 2 L (L the register
+     * length), so L = (2t - kk) / 2 at the end; lam is a non-zero multiple
+     * of the classical locator, with the same roots.  L > t: more errors
+     * than the code corrects. ---- */
     if (tid == 0) {
+        constexpr int PL = 2 * BCH_MAX_T + 2; /* polynomial coefficients kept */
         uint16_t S[2 * BCH_MAX_T + 1];
         uint32_t any = 0;
         for (int k = 0; k < t; k++) {
@@ -200,44 +214,44 @@ __global__ void __launch_bounds__(BCH_NT) bch_kernel(BchArgs a)
         if (any) {
             for (int j = 2; j <= 2 * t; j += 2)
                 S[j] = gf_mul(a.exp, a.log, S[j / 2], S[j / 2]);
-            uint16_t L[BCH_MAX_T + 2], B[BCH_MAX_T + 2], T[BCH_MAX_T + 2];
-            for (int i = 0; i < BCH_MAX_T + 2; i++)
-                L[i] = B[i] = 0;
-            L[0] = B[0] = 1;
-            int len = 0, shift = 1;
-            uint16_t bd = 1;
-            for (int n = 0; n < 2 * t; n++) {
-                uint16_t d = S[n + 1];
-                for (int i = 1; i <= len; i++)
-                    d ^= gf_mul(a.exp, a.log, L[i], S[n + 1 - i]);
-                if (!d) {
-                    shift++;
-                    continue;
-                }
-                const uint16_t c = gf_mul(a.exp, a.log, d, a.exp[q - a.log[bd]]);
-                for (int i = 0; i < BCH_MAX_T + 2; i++)
-                    T[i] = L[i];
-                for (int i = 0; i + shift <= t + 1; i++)
-                    L[i + shift] ^= gf_mul(a.exp, a.log, c, B[i]);
-                if (2 * len <= n) {
-                    len = n + 1 - len;
-                    for (int i = 0; i < BCH_MAX_T + 2; i++)
-                        B[i] = T[i];
-                    bd = d;
-                    shift = 1;
+            uint16_t lam[PL], bb[PL];
+            for (int i = 0; i < PL; i++)
+                lam[i] = bb[i] = 0;
+            lam[0] = bb[0] = 1;
+            uint16_t gam = 1;
+            int kk = 0;
+            for (int r = 0; r < 2 * t; r++) {
+                uint16_t delta = 0;
+                for (int i = 0; i <= r && i < PL; i++)
+                    delta ^= gf_mul(a.exp, a.log, lam[i], S[r + 1 - i]);
+                uint16_t nl[PL];
+                nl[0] = gf_mul(a.exp, a.log, gam, lam[0]);
+                for (int i = 1; i < PL; i++)
+                    nl[i] = gf_mul(a.exp, a.log, gam, lam[i]) ^ gf_mul(a.exp, a.log, delta, bb[i - 1]);
+                if (delta && kk >= 0) {
+                    for (int i = 0; i < PL; i++)
+                        bb[i] = lam[i];
+                    gam = delta;
+                    kk = -kk - 1;
                 } else {
-                    shift++;
+                    for (int i = PL - 1; i > 0; i--)
+                        bb[i] = bb[i - 1];
+                    bb[0] = 0;
+                    kk++;
                 }
+                for (int i = 0; i < PL; i++)
+                    lam[i] = nl[i];
             }
+            const int len = (2 * t - kk) / 2;
             if (len > t) {
                 s_len = -1;
             } else {
-                /* Chien terms: L_k alpha^(-k p) as exponents log L_k + (q - k) p */
-                s_L0 = L[0];
+                /* Chien terms: lam_k alpha^(-k p) as exponents log lam_k + (q - k) p */
+                s_L0 = lam[0];
                 int nt = 0;
                 for (int k = 1; k <= len; k++)
-                    if (L[k]) {
-                        s_lg[nt] = a.log[L[k]];
+                    if (lam[k]) {
+                        s_lg[nt] = a.log[lam[k]];
                         s_step[nt] = q - (uint32_t)k;
                         nt++;
                     }
