@@ -46,12 +46,27 @@ static int build_split(lg_code* c)
     c->sk_tab = calloc((size_t)rows * 720, sizeof(uint32_t));
     if (!c->sk_lay || !c->sk_tab)
         return -1;
+    /* barriers: layer i must wait for the workgroup (bit 16) unless it
+     * shares no information group with the layers since the last barrier
+     * (a parity bit passes between the two lanes of one node, in one
+     * wave); the first layer of an iteration and the layers with nodes
+     * sharing bits, and the one after them, always wait */
+    uint8_t* used = calloc((size_t)(K / LG_M) + 1, 1);
+    if (!used)
+        return -1;
     int row = 0, mrow = 0;
     for (int i = 0; i < q; i++) {
         const int s = c->layer_start[i], d = c->layer_start[i + 1] - s, T = d + 2;
         const int tail = c->nlev[i] > 1;
+        int bar = i == 0 || tail || c->nlev[i - 1] > 1;
+        for (int e = 0; e < d && !bar; e++)
+            bar = used[(c->ent[s + e] >> 16) / LG_M];
+        if (bar)
+            memset(used, 0, (size_t)(K / LG_M) + 1);
+        for (int e = 0; e < d; e++)
+            used[(c->ent[s + e] >> 16) / LG_M] = 1;
         uint32_t* L = c->sk_lay + 4 * i;
-        L[0] = (uint32_t)T | (uint32_t)c->nlev[i] << 8;
+        L[0] = (uint32_t)T | (uint32_t)c->nlev[i] << 8 | (uint32_t)bar << 16;
         L[1] = (uint32_t)mrow;
         L[2] = (uint32_t)row;
         L[3] = tail ? (uint32_t)(row + sk) : 0u;
@@ -81,8 +96,10 @@ static int build_split(lg_code* c)
                         succ |= ((LG_META_SUCC(mt) >> e) & 1u) << k;
                     }
                 }
-                if (ad > 0xffffu || cell > 0xffffu)
+                if (ad > 0xffffu || cell > 0xffffu) {
+                    free(used);
                     return -1;
+                }
                 c->sk_tab[(size_t)(row + k / 2) * 720 + t] |= ad << (16 * (k & 1));
                 if (tail)
                     c->sk_tab[(size_t)(row + sk + 1 + k / 2) * 720 + t] |= cell << (16 * (k & 1));
@@ -93,6 +110,7 @@ static int build_split(lg_code* c)
         row += sk + (tail ? 1 + sk : 0);
         mrow += sk;
     }
+    free(used);
     return 0;
 }
 

@@ -80,8 +80,10 @@ typedef struct lg_code {
      * rows of hand-over cells; SK = (sk_ks + 1) / 2. */
     int sk_ks;             /* slots per lane: (largest degree + 2) / 2 rounded up */
     int sk_wide;           /* 1 if a layer has fewer slots than sk_ks - 1 */
-    uint32_t* sk_lay;      /* q x 4: positions | levels << 8, first message row,
-                              first address row, mask row (0: none) */
+    uint32_t* sk_lay;      /* q x 4: positions | levels << 8 | barrier << 16 (the
+                              layer waits for the workgroup's previous layers),
+                              first message row, first address row, mask row
+                              (0: none) */
     uint32_t* sk_tab;      /* sk_rows x 720 */
     int sk_rows;
     int sk_msg_rows;       /* message rows (of 720 words) per frame pair */

@@ -11,7 +11,8 @@
  * slot addresses (SK_WORDS(KS) rows), and for layers with nodes sharing
  * bits the slots' dependency masks (one row: dep in bits 0-15, succ in
  * 16-31) and hand-over cells (SK_WORDS(KS) rows).  Layer descriptor
- * (PkArgs.pkl): x = positions T | level count << 8, y = first message row
+ * (PkArgs.pkl): x = positions T | level count << 8 | barrier << 16, y =
+ * first message row
  * (rows of 720 words in the workgroup's message block), z = first
  * address row, w = the mask row (0 when the layer has none).
  */
@@ -37,6 +38,7 @@ constexpr int WA = SK_WORDS(KS);
 
 __device__ __forceinline__ int sl_T(uint32_t x) { return (int)(x & 0xff); }
 __device__ __forceinline__ int sl_nlev(uint32_t x) { return (int)((x >> 8) & 0xff); }
+__device__ __forceinline__ bool sl_bar(uint32_t x) { return (x >> 16) & 1; }
 
 /* the partner lane's value (lanes 2j and 2j + 1 swap: quad permutation
  * [1, 0, 3, 2]) */
@@ -211,7 +213,10 @@ __device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, 
             sk_layer<true>(sm, aw, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote, nvote);
         else
             sk_layer<false>(sm, aw, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote, nvote);
-        __syncthreads();
+        /* the next layer waits for this one where it reads what another
+         * wave wrote (its barrier flag; the end of the iteration always) */
+        if (i + 1 == q || sl_bar(ldn.x))
+            __syncthreads();
     }
 }
 
