@@ -125,7 +125,10 @@ LG_PK_HD PkOut sk_mags(const SkHalf<KS>& n, lg_pk ps, lg_pk pb)
 }
 
 /* o_k: |o| = mn1 if |x_k| is the node's smallest input, else mn0; sign =
- * product of the other signs (see pk_out) */
+ * product of the other signs.  a >= min0 always, so a - min0 is 0 exactly
+ * for the smallest input.  Where a 0xff low byte makes |x_k| differ from
+ * min0 by less than 256, both are 127 and mn0 == mn1 (both minimums >=
+ * 127). */
 template <int KS>
 LG_PK_HD lg_pk sk_out(const SkHalf<KS>& n, const PkOut& m, int k)
 {
@@ -161,7 +164,8 @@ LG_PK_HD void sk_phase_c(const SkHalf<KS>& n, uint16_t* post, uint32_t succ, uin
 
 /* parity and non-zero flags of a half's present slots (bad(),
  * ldpc.c:286-305): par collects the sign bits (bit 7 / 15 of the two
- * frames), nz bit 7 / 15 stays set while no posterior is zero */
+ * frames), nz bit 7 / 15 stays set while no posterior is zero (a byte b is
+ * non-zero iff bit 7 of ((b & 0x7f) + 0x7f) | b is set) */
 template <int KS, int KA>
 LG_PK_HD void sk_check(const uint16_t* post, const int* ad, int T, int h, uint32_t skip0, uint32_t& par, uint32_t& nz)
 {
@@ -183,11 +187,26 @@ LG_PK_HD uint32_t sk_check_bits(uint32_t par, uint32_t nz)
     return ((good >> 7) & 1) | ((good >> 14) & 2);
 }
 
-/* ---- layers with nodes sharing a bit: hand-over rounds (ldpc_pk.h) ------
+/* ---- layers with nodes sharing a bit: hand-over rounds -------------------
+ *
+ * The reference updates the nodes of a layer one after another
+ * (ldpc.c:474-480): a node that shares an information bit with earlier
+ * nodes of the layer reads it as the latest of them left it.  Here all
+ * nodes of such a layer run at once, in rounds.  Round 0 takes every bit
+ * as the layer found it; in round r a node reads each shared bit as its
+ * predecessor on that bit handed it over in round r - 1 (hand-over cells,
+ * double-buffered) and hands on what it would write to its successors.  A
+ * node whose inputs are exact after round r - 1 is exact after round r, so
+ * all are exact after (levels - 1) rounds; and once a round hands over the
+ * same values as the round before, the next round would read the inputs
+ * of this one, and so would every later round: that state is the
+ * sequential result (the hand-over graph has no cycle, so its fixed point
+ * is unique).  The rounds stop at the first of the two.
  *
  * hw: the slots' hand-over cells, two u16 a word as the addresses; a slot
- * at rank r on its bit hands over through cell c = (t0 + r) * 360 + m and
- * reads its predecessor's at c - 360. */
+ * at rank r on its bit (its node is the r-th of the layer touching the
+ * bit, 0-based) hands over through cell c = (t0 + r) * 360 + m, m the
+ * bit's index in its group, and reads its predecessor's at c - 360. */
 LG_PK_HD int sk_cell(const uint32_t* hw, int k) { return (int)((k & 1) ? hw[k >> 1] >> 16 : hw[k >> 1] & 0xffffu); }
 
 template <int KS>
@@ -226,22 +245,46 @@ LG_PK_HD void sk_put_hand(const SkHalf<KS>& n, uint16_t* hb, const uint32_t* hw,
             hb[sk_cell(hw, k)] = pk_post_bytes(pk_adds(n.x[k], sk_out(n, m, k)));
 }
 
-/* take the values handed over by the predecessors (slots in dep) into the
- * half's minimums; non-zero if an input differs from the previous round */
+/* the values this half handed over in the previous round (slots in succ,
+ * own cells of that round's buffer hb), read before the round writes */
 template <int KS>
-LG_PK_HD uint32_t sk_take_hand(SkHalf<KS>& n, const uint16_t* hb, const uint32_t* hw, uint32_t dep, const uint32_t* mw,
-                               uint32_t sel_even, uint32_t sel_odd)
+LG_PK_HD void sk_prev_hand(uint32_t* old, const uint16_t* hb, const uint32_t* hw, uint32_t succ)
 {
+#pragma unroll
+    for (int k = 0; k < KS; k++)
+        old[k] = ((succ >> k) & 1) ? hb[sk_cell(hw, k)] : 0u;
+}
+
+/* sk_put_hand, and non-zero if a value differs from old (the previous
+ * round's): when no value handed over in the workgroup changed, the next
+ * round would see the inputs of this one, and the layer is at its fixed
+ * point */
+template <int KS, bool NMS>
+LG_PK_HD uint32_t sk_put_hand_chg(const SkHalf<KS>& n, uint16_t* hb, const uint32_t* hw, uint32_t succ,
+                                  const uint32_t* old, lg_pk ps, lg_pk pb)
+{
+    const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
     uint32_t chg = 0;
 #pragma unroll
-    for (int k = 0; k < KS; k++) {
-        if ((dep >> k) & 1) {
-            const lg_pk x = pk_in_w(hb[sk_cell(hw, k) - 360], mw, k, sel_even, sel_odd);
-            chg |= pk_bits(x) ^ pk_bits(n.x[k]);
-            sk_fold(n, k, x);
+    for (int k = 0; k < KS; k++)
+        if ((succ >> k) & 1) {
+            const uint16_t v = pk_post_bytes(pk_adds(n.x[k], sk_out(n, m, k)));
+            chg |= v ^ old[k];
+            hb[sk_cell(hw, k)] = v;
         }
-    }
     return chg;
+}
+
+/* take the values handed over by the predecessors (slots in dep) into the
+ * half's minimums */
+template <int KS>
+LG_PK_HD void sk_take_hand(SkHalf<KS>& n, const uint16_t* hb, const uint32_t* hw, uint32_t dep, const uint32_t* mw,
+                           uint32_t sel_even, uint32_t sel_odd)
+{
+#pragma unroll
+    for (int k = 0; k < KS; k++)
+        if ((dep >> k) & 1)
+            sk_fold(n, k, pk_in_w(hb[sk_cell(hw, k) - 360], mw, k, sel_even, sel_odd));
 }
 
 #endif

@@ -111,9 +111,9 @@ __device__ __forceinline__ uint32_t sk_syndrome(const uint16_t* sm, const uint32
 }
 
 /* One layer by the workgroup.  TAIL (the layer has nodes sharing bits):
- * after phase A the pairs run hand-over rounds (ldpc_pk.h) until their
- * inputs stop changing or the layer's level count is reached; hb: the two
- * hand-over buffers, hstride u16 cells apart. */
+ * after phase A the pairs run hand-over rounds (ldpc_sk.h) until what they
+ * hand over stops changing or the layer's level count is reached; hb: the
+ * two hand-over buffers, hstride u16 cells apart. */
 template <bool TAIL>
 __device__ __forceinline__ void sk_layer(uint16_t* sm, const uint32_t* aw, int T, int h, uint32_t skip0, int nlev,
                                          const uint32_t* mw, uint32_t tm, const uint32_t* hw, uint32_t* mrow,
@@ -132,12 +132,15 @@ __device__ __forceinline__ void sk_layer(uint16_t* sm, const uint32_t* aw, int T
         __syncthreads();
 #pragma unroll 1
         for (int r = 1; r < nlev; r++) {
+            const uint16_t* prv = hb + ((r - 1) & 1) * hstride;
+            uint32_t old[KS];
             sk_restore(n, part);
-            const uint32_t chg = sk_take_hand(n, hb + ((r - 1) & 1) * hstride, hw, dep, mw, se, so);
+            sk_take_hand(n, prv, hw, dep, mw, se, so);
+            sk_prev_hand<KS>(old, prv, hw, succ);
             sk_pair(n);
             if (r == nlev - 1)
                 break;
-            sk_put_hand<KS, NMS>(n, hb + (r & 1) * hstride, hw, succ, ps, pb);
+            const uint32_t chg = sk_put_hand_chg<KS, NMS>(n, hb + (r & 1) * hstride, hw, succ, old, ps, pb);
             if (wg_and2(chg ? 0u : 3u, s_vote, nvote) & 1)
                 break;
         }

@@ -219,18 +219,20 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
         sk_model_pair(n);
         for (int t = 719; t >= 0; t--)
             sk_put_hand<KS, NMS>(n[t], hb[0], hw[t], tm[t] >> 16, ps[t], pb[t]);
+        static uint32_t old[720][KS];
         for (int r = 1; r < nlev; r++) {
             uint32_t chg = 0;
             for (int t = 719; t >= 0; t--) {
                 sk_restore(n[t], part[t]);
-                chg |= sk_take_hand(n[t], hb[(r - 1) & 1], hw[t], tm[t] & 0xffff, mw[t], se, so);
+                sk_take_hand(n[t], hb[(r - 1) & 1], hw[t], tm[t] & 0xffff, mw[t], se, so);
+                sk_prev_hand<KS>(old[t], hb[(r - 1) & 1], hw[t], tm[t] >> 16);
             }
             sk_model_pair(n);
             g_pk_rounds++;
             if (r == nlev - 1)
                 break;
             for (int t = 719; t >= 0; t--)
-                sk_put_hand<KS, NMS>(n[t], hb[r & 1], hw[t], tm[t] >> 16, ps[t], pb[t]);
+                chg |= sk_put_hand_chg<KS, NMS>(n[t], hb[r & 1], hw[t], tm[t] >> 16, old[t], ps[t], pb[t]);
             if (!chg)
                 break;
         }
