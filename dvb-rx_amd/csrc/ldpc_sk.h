@@ -62,11 +62,21 @@ LG_PK_HD uint32_t sk_skip(int k, int T, int h, uint32_t skip0)
     return m;
 }
 
-/* Phase A of a half: every slot read, minimums and sign of the present
- * ones.  TAIL: the slots in dep (read after an earlier node of the layer)
- * stay out too; the hand-over rounds fold them. */
+/* the posteriors of a half's slots (two bytes, both frames), all reads
+ * issued before any is used */
+template <int KS>
+LG_PK_HD void sk_read(uint32_t* raw, const uint16_t* post, const SkHalf<KS>& n)
+{
+#pragma unroll
+    for (int k = 0; k < KS; k++)
+        raw[k] = post[n.ad[k]];
+}
+
+/* Phase A of a half from its posteriors raw: minimums and sign of the
+ * present slots.  TAIL: the slots in dep (read after an earlier node of
+ * the layer) stay out too; the hand-over rounds fold them. */
 template <int KS, int KA, bool TAIL>
-LG_PK_HD void sk_phase_a(SkHalf<KS>& n, const uint16_t* post, int T, int h, uint32_t skip0, uint32_t dep,
+LG_PK_HD void sk_phase_a(SkHalf<KS>& n, const uint32_t* raw, int T, int h, uint32_t skip0, uint32_t dep,
                          const uint32_t* mw, uint32_t sel_even, uint32_t sel_odd)
 {
     n.min0 = pk_dup(-1);
@@ -74,7 +84,7 @@ LG_PK_HD void sk_phase_a(SkHalf<KS>& n, const uint16_t* post, int T, int h, uint
     n.s = pk_dup(0);
 #pragma unroll
     for (int k = 0; k < KS; k++) {
-        const lg_pk x = pk_in(post, n.ad[k], mw, k, sel_even, sel_odd);
+        const lg_pk x = pk_in_w(raw[k], mw, k, sel_even, sel_odd);
         uint32_t skip = sk_skip<KA>(k, T, h, skip0);
         if (TAIL)
             skip |= 0u - ((dep >> k) & 1);

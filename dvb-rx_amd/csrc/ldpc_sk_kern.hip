@@ -36,6 +36,17 @@ constexpr int KA = SK_WIDE ? 1 : KS - 1; /* slots k >= KA may be empty */
 constexpr bool NMS = SK_NMS != 0;
 constexpr int WA = SK_WORDS(KS);
 
+/* layer descriptors through the scalar cache: their index is uniform over
+ * the workgroup, so a constant-address-space view of the table makes them
+ * scalar loads (SGPRs, waited on with the LDS counter) instead of vector
+ * loads whose wait would also wait for the message stores */
+typedef const __attribute__((address_space(4))) uint32_t sl_cu32;
+__device__ __forceinline__ uint4 sl_desc(const uint4* skl, int i)
+{
+    const sl_cu32* p = (const sl_cu32*)(skl + i);
+    return make_uint4(p[0], p[1], p[2], p[3]);
+}
+
 __device__ __forceinline__ int sl_T(uint32_t x) { return (int)(x & 0xff); }
 __device__ __forceinline__ int sl_nlev(uint32_t x) { return (int)((x >> 8) & 0xff); }
 __device__ __forceinline__ bool sl_bar(uint32_t x) { return (x >> 16) & 1; }
@@ -83,7 +94,7 @@ __device__ __forceinline__ uint32_t sk_syndrome(const uint16_t* sm, const uint32
         for (int c = 0; c < SYN_CHUNK; c++) {
             if (i0 + c >= i1)
                 continue;
-            const uint4 ld = skl[i0 + c];
+            const uint4 ld = sl_desc(skl, i0 + c);
             Ts[c] = sl_T(ld.x);
             const uint32_t* ar = adt + (size_t)720 * ld.z + t;
 #pragma unroll
@@ -115,15 +126,13 @@ __device__ __forceinline__ uint32_t sk_syndrome(const uint16_t* sm, const uint32
  * hand over stops changing or the layer's level count is reached; hb: the
  * two hand-over buffers, hstride u16 cells apart. */
 template <bool TAIL>
-__device__ __forceinline__ void sk_layer(uint16_t* sm, const uint32_t* aw, int T, int h, uint32_t skip0, int nlev,
-                                         const uint32_t* mw, uint32_t tm, const uint32_t* hw, uint32_t* mrow,
-                                         uint32_t se, uint32_t so, lg_pk ps, lg_pk pb, uint16_t* hb, int hstride,
-                                         unsigned* s_vote, int& nvote)
+__device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uint16_t* sm, int T, int h,
+                                         uint32_t skip0, int nlev, const uint32_t* mw, uint32_t tm,
+                                         const uint32_t* hw, uint32_t* mrow, uint32_t se, uint32_t so, lg_pk ps,
+                                         lg_pk pb, uint16_t* hb, int hstride, unsigned* s_vote, int& nvote)
 {
     const uint32_t dep = TAIL ? (tm & 0xffffu) : 0, succ = TAIL ? (tm >> 16) : 0;
-    SkHalf<KS> n;
-    sk_addrs(n, aw);
-    sk_phase_a<KS, KA, TAIL>(n, sm, T, h, skip0, dep, mw, se, so);
+    sk_phase_a<KS, KA, TAIL>(n, raw, T, h, skip0, dep, mw, se, so);
     if (TAIL) {
         const PkPart part = sk_part(n);
         sk_fold_dep0(n, dep);
@@ -151,7 +160,8 @@ __device__ __forceinline__ void sk_layer(uint16_t* sm, const uint32_t* aw, int T
 }
 
 /* one iteration (all layers) on both frames; the addresses, masks and
- * messages of layer i+1 are fetched while layer i runs */
+ * messages of layer i+1 are fetched while layer i runs, the descriptor of
+ * layer i+2 too */
 __device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, const uint4* skl, uint32_t* msg, int q,
                                              int N, int first0, int first1, PkRowPtrs rows, uint16_t* hb, int hstride,
                                              unsigned* s_vote, int& nvote)
@@ -184,8 +194,14 @@ __device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, 
                 nh[w] = tr[720 * (w + 1)];
         }
     };
-    uint4 ldn = skl[0];
+    uint4 ldn = sl_desc(skl, 0);
     fetch(ldn);
+    uint4 ldf = sl_desc(skl, min(1, q - 1));
+    /* layer 0's words are needed at once; waiting for them here also tells
+     * the compiler that nothing is outstanding when the loop starts, so the
+     * loop's own waits need not cover the message stores of the layer
+     * before (s_waitcnt 0: every counter) */
+    __builtin_amdgcn_s_waitcnt(0);
     for (int i = 0; i < q; i++) {
         const uint4 ld = ldn;
         uint32_t mw[WA], aw[WA], hw[WA];
@@ -196,9 +212,14 @@ __device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, 
             hw[w] = nh[w];
         }
         const uint32_t tm = ntm;
+        SkHalf<KS> n;
+        sk_addrs(n, aw);
+        uint32_t raw[KS];
+        sk_read<KS>(raw, sm, n);
         if (i + 1 < q) {
-            ldn = skl[i + 1];
+            ldn = ldf;
             fetch(ldn);
+            ldf = sl_desc(skl, min(i + 2, q - 1));
         }
         const int T = sl_T(ld.x), nlev = sl_nlev(ld.x);
         uint32_t* mrow = msg + (size_t)720 * ld.y + t;
@@ -213,9 +234,11 @@ __device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, 
         }
         (void)N;
         if (nlev > 1)
-            sk_layer<true>(sm, aw, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote, nvote);
+            sk_layer<true>(n, raw, sm, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote,
+                           nvote);
         else
-            sk_layer<false>(sm, aw, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote, nvote);
+            sk_layer<false>(n, raw, sm, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote,
+                            nvote);
         /* the next layer waits for this one where it reads what another
          * wave wrote (its barrier flag; the end of the iteration always) */
         if (i + 1 == q || sl_bar(ldn.x))

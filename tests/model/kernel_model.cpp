@@ -205,10 +205,12 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
     }
     for (int t = 719; t >= 0; t--) {
         const uint32_t skip0 = (i == 0 && t == 1) ? ~0u : 0u;
+        uint32_t raw[KS];
+        sk_read<KS>(raw, post, n[t]);
         if (tail)
-            sk_phase_a<KS, KA, true>(n[t], post, T, t & 1, skip0, tm[t] & 0xffff, mw[t], se, so);
+            sk_phase_a<KS, KA, true>(n[t], raw, T, t & 1, skip0, tm[t] & 0xffff, mw[t], se, so);
         else
-            sk_phase_a<KS, KA, false>(n[t], post, T, t & 1, skip0, 0, mw[t], se, so);
+            sk_phase_a<KS, KA, false>(n[t], raw, T, t & 1, skip0, 0, mw[t], se, so);
     }
     if (tail) {
         static uint16_t hb[2][LG_MAX_XFER * 360];
