@@ -58,6 +58,22 @@ static int build_split(lg_code* c)
     for (int i = 0; i < q; i++) {
         const int s = c->layer_start[i], d = c->layer_start[i + 1] - s, T = d + 2;
         const int tail = c->nlev[i] > 1;
+        /* information edges by position: in layers with nodes sharing bits
+         * the edges of the shared groups first (positions 2 .. 1 + S), so
+         * the rounds touch slots 1 .. kde - 1 only (the node's arithmetic
+         * does not depend on the order of its edges) */
+        int order[LG_MAX_CN_DEG], S = 0;
+        for (int e = 0; e < d; e++)
+            if (tail && ((c->dupm[i] >> e) & 1))
+                order[S++] = e;
+        for (int e = 0, o = S; e < d; e++)
+            if (!(tail && ((c->dupm[i] >> e) & 1)))
+                order[o++] = e;
+        const int kde = tail ? 1 + (S + 1) / 2 : 0;
+        if (kde - 1 > SK_KD(ks)) {
+            free(used);
+            return -1;
+        }
         int bar = i == 0 || tail || c->nlev[i - 1] > 1;
         for (int e = 0; e < d && !bar; e++)
             bar = used[(c->ent[s + e] >> 16) / LG_M];
@@ -66,7 +82,7 @@ static int build_split(lg_code* c)
         for (int e = 0; e < d; e++)
             used[(c->ent[s + e] >> 16) / LG_M] = 1;
         uint32_t* L = c->sk_lay + 4 * i;
-        L[0] = (uint32_t)T | (uint32_t)c->nlev[i] << 8 | (uint32_t)bar << 16;
+        L[0] = (uint32_t)T | (uint32_t)c->nlev[i] << 8 | (uint32_t)bar << 16 | (uint32_t)kde << 24;
         L[1] = (uint32_t)mrow;
         L[2] = (uint32_t)row;
         L[3] = tail ? (uint32_t)(row + sk) : 0u;
@@ -85,7 +101,7 @@ static int build_split(lg_code* c)
                     else if (j)
                         ad = (uint32_t)(K + 360 * (q - 1) + j - 1);
                 } else if (p < T) {
-                    const int e = p - 2;
+                    const int e = order[p - 2];
                     const uint32_t xe = c->xent[s + e];
                     const int m = (j - LG_XENT_SHIFT(xe) + LG_M) % LG_M;
                     ad = (xe >> 16) + (uint32_t)m;
@@ -96,7 +112,7 @@ static int build_split(lg_code* c)
                         succ |= ((LG_META_SUCC(mt) >> e) & 1u) << k;
                     }
                 }
-                if (ad > 0xffffu || cell > 0xffffu) {
+                if (ad > 0xffffu || cell > 0xffffu || (cell && (k == 0 || k >= kde))) {
                     free(used);
                     return -1;
                 }

@@ -72,7 +72,8 @@ typedef struct lg_code {
     int max_xfer;          /* largest number of hand-overs of one layer */
     /* Tables of the split kernel (a lane pair per node, ldpc_sk.h): lane
      * t = 2j + h takes the node's positions p = 2k + h (0 own parity, 1
-     * previous parity, 2 + e information edge e), k < sk_ks.  Rows of 720
+     * previous parity, 2 + r the layer's r-th information edge, those of
+     * shared groups first), k < sk_ks.  Rows of 720
      * words, word t for lane t: per layer SK rows of slot posterior
      * indices (two u16 a word, slot 2w in the low half; an empty slot
      * points at index n, a spare cell), and for layers with nodes sharing
@@ -81,15 +82,21 @@ typedef struct lg_code {
     int sk_ks;             /* slots per lane: (largest degree + 2) / 2 rounded up */
     int sk_wide;           /* 1 if a layer has fewer slots than sk_ks - 1 */
     uint32_t* sk_lay;      /* q x 4: positions | levels << 8 | barrier << 16 (the
-                              layer waits for the workgroup's previous layers),
-                              first message row, first address row, mask row
-                              (0: none) */
+                              layer waits for the workgroup's previous layers)
+                              | kde << 24 (slots 1 .. kde - 1 hold the edges
+                              of groups shared by nodes of the layer, the
+                              first positions after the parity bits), first
+                              message row, first address row, mask row (0:
+                              none) */
     uint32_t* sk_tab;      /* sk_rows x 720 */
     int sk_rows;
     int sk_msg_rows;       /* message rows (of 720 words) per frame pair */
 } lg_code;
 
 #define LG_MAX_XFER 8      /* hand-overs per layer the kernels provide for */
+/* slots per lane that may hold shared edges (1 .. SK_KD) in a code whose
+ * split kernel has ks slots per lane: kde - 1 <= SK_KD(ks) */
+#define SK_KD(ks) ((ks) <= 4 ? 1 : (ks) <= 7 ? 3 : (ks) <= 11 ? 5 : 6)
 #define LG_XENT_SHIFT(e) ((int)((e) & 0x1ffu))
 #define LG_XENT_KM1(e) ((int)(((e) >> 9) & 3u))
 #define LG_XENT_T0(e) ((int)(((e) >> 11) & 15u))

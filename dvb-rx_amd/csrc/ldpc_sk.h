@@ -6,8 +6,10 @@
  *
  * Node j of a layer is served by lanes 2j (half h = 0) and 2j + 1 (h = 1).
  * The node's edges by position p: 0 its own parity bit, 1 the previous
- * parity bit (the IRA staircase, ldpc.c:277-284), 2 + e information edge
- * e; half h takes the positions p = 2k + h, its "slots" k = 0..KS-1.  A
+ * parity bit (the IRA staircase, ldpc.c:277-284), then the information
+ * edges, in layers with nodes sharing bits those of the shared groups
+ * first (slots 1 .. kde - 1, the only ones the hand-over rounds touch);
+ * half h takes the positions p = 2k + h, its "slots" k = 0..KS-1.  A
  * layer of T = degree + 2 positions leaves slot k of half h empty when
  * 2k + h >= T (only slots k >= KA can be, KA a template bound), and node
  * (0, 0) has no previous parity (slot 0 of half 1 in layer 0).  Empty
@@ -235,34 +237,44 @@ LG_PK_HD void sk_restore(SkHalf<KS>& n, const PkPart& p)
 
 /* round 0: the dependent slots with the values phase A read */
 template <int KS>
-LG_PK_HD void sk_fold_dep0(SkHalf<KS>& n, uint32_t dep)
+LG_PK_HD void sk_fold_dep0(SkHalf<KS>& n, uint32_t dep, int kde)
 {
 #pragma unroll
-    for (int k = 0; k < KS; k++)
+    for (int k = 1; k <= SK_KD(KS); k++) {
+        if (k >= kde)
+            continue;
         if ((dep >> k) & 1)
             sk_fold(n, k, n.x[k]);
+    }
 }
 
 /* write the values handed to the successors (slots in succ); n holds the
  * node's combined minimums */
 template <int KS, bool NMS>
-LG_PK_HD void sk_put_hand(const SkHalf<KS>& n, uint16_t* hb, const uint32_t* hw, uint32_t succ, lg_pk ps, lg_pk pb)
+LG_PK_HD void sk_put_hand(const SkHalf<KS>& n, uint16_t* hb, const uint32_t* hw, uint32_t succ, int kde, lg_pk ps,
+                          lg_pk pb)
 {
     const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
 #pragma unroll
-    for (int k = 0; k < KS; k++)
+    for (int k = 1; k <= SK_KD(KS); k++) {
+        if (k >= kde)
+            continue;
         if ((succ >> k) & 1)
             hb[sk_cell(hw, k)] = pk_post_bytes(pk_adds(n.x[k], sk_out(n, m, k)));
+    }
 }
 
 /* the values this half handed over in the previous round (slots in succ,
  * own cells of that round's buffer hb), read before the round writes */
 template <int KS>
-LG_PK_HD void sk_prev_hand(uint32_t* old, const uint16_t* hb, const uint32_t* hw, uint32_t succ)
+LG_PK_HD void sk_prev_hand(uint32_t* old, const uint16_t* hb, const uint32_t* hw, uint32_t succ, int kde)
 {
 #pragma unroll
-    for (int k = 0; k < KS; k++)
+    for (int k = 1; k <= SK_KD(KS); k++) {
+        if (k >= kde)
+            continue;
         old[k] = ((succ >> k) & 1) ? hb[sk_cell(hw, k)] : 0u;
+    }
 }
 
 /* sk_put_hand, and non-zero if a value differs from old (the previous
@@ -270,31 +282,37 @@ LG_PK_HD void sk_prev_hand(uint32_t* old, const uint16_t* hb, const uint32_t* hw
  * round would see the inputs of this one, and the layer is at its fixed
  * point */
 template <int KS, bool NMS>
-LG_PK_HD uint32_t sk_put_hand_chg(const SkHalf<KS>& n, uint16_t* hb, const uint32_t* hw, uint32_t succ,
+LG_PK_HD uint32_t sk_put_hand_chg(const SkHalf<KS>& n, uint16_t* hb, const uint32_t* hw, uint32_t succ, int kde,
                                   const uint32_t* old, lg_pk ps, lg_pk pb)
 {
     const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
     uint32_t chg = 0;
 #pragma unroll
-    for (int k = 0; k < KS; k++)
+    for (int k = 1; k <= SK_KD(KS); k++) {
+        if (k >= kde)
+            continue;
         if ((succ >> k) & 1) {
             const uint16_t v = pk_post_bytes(pk_adds(n.x[k], sk_out(n, m, k)));
             chg |= v ^ old[k];
             hb[sk_cell(hw, k)] = v;
         }
+    }
     return chg;
 }
 
 /* take the values handed over by the predecessors (slots in dep) into the
  * half's minimums */
 template <int KS>
-LG_PK_HD void sk_take_hand(SkHalf<KS>& n, const uint16_t* hb, const uint32_t* hw, uint32_t dep, const uint32_t* mw,
-                           uint32_t sel_even, uint32_t sel_odd)
+LG_PK_HD void sk_take_hand(SkHalf<KS>& n, const uint16_t* hb, const uint32_t* hw, uint32_t dep, int kde,
+                           const uint32_t* mw, uint32_t sel_even, uint32_t sel_odd)
 {
 #pragma unroll
-    for (int k = 0; k < KS; k++)
+    for (int k = 1; k <= SK_KD(KS); k++) {
+        if (k >= kde)
+            continue;
         if ((dep >> k) & 1)
             sk_fold(n, k, pk_in_w(hb[sk_cell(hw, k) - 360], mw, k, sel_even, sel_odd));
+    }
 }
 
 #endif

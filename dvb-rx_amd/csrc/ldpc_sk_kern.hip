@@ -11,9 +11,9 @@
  * slot addresses (SK_WORDS(KS) rows), and for layers with nodes sharing
  * bits the slots' dependency masks (one row: dep in bits 0-15, succ in
  * 16-31) and hand-over cells (SK_WORDS(KS) rows).  Layer descriptor
- * (PkArgs.pkl): x = positions T | level count << 8 | barrier << 16, y =
- * first message row
- * (rows of 720 words in the workgroup's message block), z = first
+ * (PkArgs.pkl): x = positions T | level count << 8 | barrier << 16 |
+ * kde << 24 (slots 1 .. kde - 1 hold the shared edges), y = first message
+ * row (rows of 720 words in the workgroup's message block), z = first
  * address row, w = the mask row (0 when the layer has none).
  */
 #include <hip/hip_runtime.h>
@@ -50,6 +50,7 @@ __device__ __forceinline__ uint4 sl_desc(const uint4* skl, int i)
 __device__ __forceinline__ int sl_T(uint32_t x) { return (int)(x & 0xff); }
 __device__ __forceinline__ int sl_nlev(uint32_t x) { return (int)((x >> 8) & 0xff); }
 __device__ __forceinline__ bool sl_bar(uint32_t x) { return (x >> 16) & 1; }
+__device__ __forceinline__ int sl_kde(uint32_t x) { return (int)(x >> 24); }
 
 /* the partner lane's value (lanes 2j and 2j + 1 swap: quad permutation
  * [1, 0, 3, 2]) */
@@ -127,7 +128,7 @@ __device__ __forceinline__ uint32_t sk_syndrome(const uint16_t* sm, const uint32
  * two hand-over buffers, hstride u16 cells apart. */
 template <bool TAIL>
 __device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uint16_t* sm, int T, int h,
-                                         uint32_t skip0, int nlev, const uint32_t* mw, uint32_t tm,
+                                         uint32_t skip0, int nlev, int kde, const uint32_t* mw, uint32_t tm,
                                          const uint32_t* hw, uint32_t* mrow, uint32_t se, uint32_t so, lg_pk ps,
                                          lg_pk pb, uint16_t* hb, int hstride, unsigned* s_vote, int& nvote)
 {
@@ -135,21 +136,21 @@ __device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uin
     sk_phase_a<KS, KA, TAIL>(n, raw, T, h, skip0, dep, mw, se, so);
     if (TAIL) {
         const PkPart part = sk_part(n);
-        sk_fold_dep0(n, dep);
+        sk_fold_dep0(n, dep, kde);
         sk_pair(n);
-        sk_put_hand<KS, NMS>(n, hb, hw, succ, ps, pb);
+        sk_put_hand<KS, NMS>(n, hb, hw, succ, kde, ps, pb);
         __syncthreads();
 #pragma unroll 1
         for (int r = 1; r < nlev; r++) {
             const uint16_t* prv = hb + ((r - 1) & 1) * hstride;
             uint32_t old[KS];
             sk_restore(n, part);
-            sk_take_hand(n, prv, hw, dep, mw, se, so);
-            sk_prev_hand<KS>(old, prv, hw, succ);
+            sk_take_hand(n, prv, hw, dep, kde, mw, se, so);
+            sk_prev_hand<KS>(old, prv, hw, succ, kde);
             sk_pair(n);
             if (r == nlev - 1)
                 break;
-            const uint32_t chg = sk_put_hand_chg<KS, NMS>(n, hb + (r & 1) * hstride, hw, succ, old, ps, pb);
+            const uint32_t chg = sk_put_hand_chg<KS, NMS>(n, hb + (r & 1) * hstride, hw, succ, kde, old, ps, pb);
             if (wg_and2(chg ? 0u : 3u, s_vote, nvote) & 1)
                 break;
         }
@@ -234,10 +235,10 @@ __device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, 
         }
         (void)N;
         if (nlev > 1)
-            sk_layer<true>(n, raw, sm, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote,
+            sk_layer<true>(n, raw, sm, T, h, skip0, nlev, sl_kde(ld.x), mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote,
                            nvote);
         else
-            sk_layer<false>(n, raw, sm, T, h, skip0, nlev, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote,
+            sk_layer<false>(n, raw, sm, T, h, skip0, nlev, 0, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote,
                             nvote);
         /* the next layer waits for this one where it reads what another
          * wave wrote (its barrier flag; the end of the iteration always) */

@@ -177,7 +177,7 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
                            const PkRows& rows)
 {
     const uint32_t* L = c->sk_lay + 4 * i;
-    const int T = (int)(L[0] & 0xff), nlev = (int)((L[0] >> 8) & 0xff);
+    const int T = (int)(L[0] & 0xff), nlev = (int)((L[0] >> 8) & 0xff), kde = (int)(L[0] >> 24);
     const int WA = SK_WORDS(KS);
     const bool tail = nlev > 1;
     static SkHalf<KS> n[720];
@@ -216,25 +216,25 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
         static uint16_t hb[2][LG_MAX_XFER * 360];
         for (int t = 719; t >= 0; t--) {
             part[t] = sk_part(n[t]);
-            sk_fold_dep0(n[t], tm[t] & 0xffff);
+            sk_fold_dep0(n[t], tm[t] & 0xffff, kde);
         }
         sk_model_pair(n);
         for (int t = 719; t >= 0; t--)
-            sk_put_hand<KS, NMS>(n[t], hb[0], hw[t], tm[t] >> 16, ps[t], pb[t]);
+            sk_put_hand<KS, NMS>(n[t], hb[0], hw[t], tm[t] >> 16, kde, ps[t], pb[t]);
         static uint32_t old[720][KS];
         for (int r = 1; r < nlev; r++) {
             uint32_t chg = 0;
             for (int t = 719; t >= 0; t--) {
                 sk_restore(n[t], part[t]);
-                sk_take_hand(n[t], hb[(r - 1) & 1], hw[t], tm[t] & 0xffff, mw[t], se, so);
-                sk_prev_hand<KS>(old[t], hb[(r - 1) & 1], hw[t], tm[t] >> 16);
+                sk_take_hand(n[t], hb[(r - 1) & 1], hw[t], tm[t] & 0xffff, kde, mw[t], se, so);
+                sk_prev_hand<KS>(old[t], hb[(r - 1) & 1], hw[t], tm[t] >> 16, kde);
             }
             sk_model_pair(n);
             g_pk_rounds++;
             if (r == nlev - 1)
                 break;
             for (int t = 719; t >= 0; t--)
-                chg |= sk_put_hand_chg<KS, NMS>(n[t], hb[r & 1], hw[t], tm[t] >> 16, old[t], ps[t], pb[t]);
+                chg |= sk_put_hand_chg<KS, NMS>(n[t], hb[r & 1], hw[t], tm[t] >> 16, kde, old[t], ps[t], pb[t]);
             if (!chg)
                 break;
         }
