@@ -31,6 +31,9 @@ import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz, GB/s
+# VALU issue: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4
+# cycles (16 lanes) at 2.4 GHz, G wave-instructions/s
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
 
 
 def cpu_flags():
@@ -40,110 +43,180 @@ def cpu_flags():
         return ""
 
 
-def cpu_baseline(batch, seconds=15.0, max_threads=16):
-    """The reference decoder (libdvbrx ldpc.c built from its sources into
-    oracle/_ref) on this host: one thread per core, each with its own
-    handles, decoding batches of LANES frames of one code (the reference's
-    batch semantics).  Falls back to the oracle (a scalar port) where the
-    reference was not built."""
-    flags = cpu_flags()
-    ref_dir = os.path.join(ROOT, "oracle", "_ref")
-    cand = []
-    if "avx512bw" in flags:
-        cand.append(("v4", "AVX-512BW, 64 lanes"))
-    if "avx2" in flags:
-        cand.append(("v3", "AVX2, 32 lanes"))
-    lib = None
-    for name, desc in cand:
-        p = os.path.join(ref_dir, f"libdrxref_{name}.so")
-        if os.path.exists(p):
-            lib = C.CDLL(p)
-            kind, what = "reference", f"libdvbrx ldpc.c built -O3 -march=x86-64-{name} ({desc})"
-            break
-    if lib is None:
-        return None
-    vp = C.c_void_p
-    lib.drx_ldpc_new.restype = vp
-    lib.drx_ldpc_new.argtypes = [vp, C.c_char_p]
-    lib.drx_ldpc_free.argtypes = [vp]
-    lib.drx_ldpc_decode.argtypes = [vp, vp, vp, C.c_int, C.c_int]
-    lanes = lib.drx_ldpc_lanes()
+# CPUs of the GPU box's host that one GPU's job may use (its lease pins this
+# share whatever the affinity set shows)
+CPU_SHARE = 16
+
+
+def host_cpu():
+    """CPU model, logical and physical core counts of this host, the size
+    of the process's affinity set, and one CPU of the affinity set per
+    physical core (sysfs topology), in order."""
+    txt = cpu_flags()
+    model, phys = "unknown", set()
+    for block in txt.split("\n\n"):
+        d = {}
+        for line in block.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                d[k.strip()] = v.strip()
+        if "processor" in d:
+            model = d.get("model name", model)
+            phys.add((d.get("physical id", "0"), d.get("core id", d["processor"])))
     try:
-        cpus = sorted(os.sched_getaffinity(0))
+        aff = sorted(os.sched_getaffinity(0))
     except AttributeError:
-        cpus = list(range(os.cpu_count() or 1))
-    nthreads = max(1, min(max_threads, len(cpus)))
-    # batches of LANES frames of one code, in workload order
-    by_code = {}
-    for f in range(batch.nframes):
-        by_code.setdefault(int(batch.codes[f]), []).append(f)
-    batches = []
-    for code, fl in by_code.items():
+        aff = list(range(os.cpu_count() or 1))
+    seen, per_core = set(), []
+    for c in aff:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            key = (open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip())
+        except OSError:
+            key = (str(c),)
+        if key not in seen:
+            seen.add(key)
+            per_core.append(c)
+    info = {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": len(phys) or None,
+            "affinity_cpus": len(aff), "affinity_physical_cores": len(per_core), "share": CPU_SHARE}
+    return info, per_core
+
+
+def ref_builds():
+    """The reference decoder builds (oracle/_ref) this host can run, widest
+    first: (name, ctypes library, description)."""
+    flags = cpu_flags()
+    out = []
+    for name, need, desc in (("v4", "avx512bw", "AVX-512BW, 64 lanes"), ("v3", "avx2", "AVX2, 32 lanes")):
+        p = os.path.join(ROOT, "oracle", "_ref", f"libdrxref_{name}.so")
+        if need in flags and os.path.exists(p):
+            lib = C.CDLL(p)
+            vp = C.c_void_p
+            lib.drx_ldpc_new.restype = vp
+            lib.drx_ldpc_new.argtypes = [vp, C.c_char_p]
+            lib.drx_ldpc_free.argtypes = [vp]
+            lib.drx_ldpc_decode.argtypes = [vp, vp, vp, C.c_int, C.c_int]
+            out.append((name, lib, f"libdvbrx ldpc.c built -O3 -march=x86-64-{name} ({desc})"))
+    return out
+
+
+def ref_batches(keys, lanes, seed=0):
+    """Frames grouped into the reference's batches: at most `lanes` frames
+    of one code (key), shuffled."""
+    by_key = {}
+    for f, k in enumerate(keys):
+        by_key.setdefault(k, []).append(f)
+    out = []
+    for k, fl in by_key.items():
         for i in range(0, len(fl), lanes):
-            batches.append(fl[i:i + lanes])
-    rng = np.random.default_rng(0)
-    rng.shuffle(batches)
+            out.append(fl[i:i + lanes])
+    np.random.default_rng(seed).shuffle(out)
+    return out
+
+
+def ref_decode_rate(lib, keys, llr_of, kl_of, max_iter, cpus, seconds, max_batches=None):
+    """The reference decoder on len(cpus) threads, each pinned to one CPU
+    with its own handles, decoding batches of LANES frames of one code (the
+    reference's batch semantics: a batch iterates until all its frames
+    hold their checks) until the time is up, cycling over the frames.  The
+    clock runs around the decode calls only (tests/test_ldpc.c:77-95
+    times its loop the same way).  keys[f] = (short_frame, rate) of frame
+    f, llr_of(f) its int8 LLRs, kl_of(f) its kldpc."""
+    lanes = lib.drx_ldpc_lanes()
+    batches = ref_batches(keys, lanes)
+    cap = max_batches if max_batches is not None else 64 * len(batches)
     lock = threading.Lock()
-    state = dict(next=0, frames=0, bits=0, iters=0, nb=0)
+    st = dict(next=0, frames=0, bits=0, iters=0, nb=0, failed=0, busy=0.0)
     deadline = [0.0]
-    rate_of = {}
-    for f in range(batch.nframes):
-        s = batch.specs[f]
-        m = ldpcgpu.pls_lookup(s["pls"])
-        rate_of[f] = (m["short_frame"], m["rate"])
 
     def worker(tid):
         try:
-            os.sched_setaffinity(0, {cpus[tid % len(cpus)]})
+            os.sched_setaffinity(0, {cpus[tid]})
         except (AttributeError, OSError):
             pass
         handles = {}
         out = np.zeros((lanes, 8100), np.uint8)
+        busy = 0.0
         while True:
             with lock:
-                i = state["next"]
-                # the sample cycles over the workload until the time is up
-                if time.perf_counter() > deadline[0] or i >= 64 * len(batches):
+                i = st["next"]
+                if time.perf_counter() > deadline[0] or i >= cap:
                     break
-                state["next"] += 1
+                st["next"] += 1
             fl = batches[i % len(batches)]
-            short, rate = rate_of[fl[0]]
-            key = (short, rate)
+            key = keys[fl[0]]
             if key not in handles:
-                fec = ldpcgpu.fec_lookup(short, rate)
-                p = fec.as_struct()
-                handles[key] = (lib.drx_ldpc_new(C.byref(p), rate.encode()), p)
-            h = handles[key][0]
+                p = ldpcgpu.fec_lookup(*key).as_struct()
+                handles[key] = (lib.drx_ldpc_new(C.byref(p), key[1].encode()), p)
             n = len(fl)
-            ins = (C.c_void_p * n)(*[batch.frame_llr(f).ctypes.data for f in fl])
+            ins = (C.c_void_p * n)(*[llr_of(f).ctypes.data for f in fl])
             outs = (C.c_void_p * n)(*[out[k].ctypes.data for k in range(n)])
-            it = lib.drx_ldpc_decode(h, ins, outs, n, batch.max_iter)
+            t = time.perf_counter()
+            it = lib.drx_ldpc_decode(handles[key][0], ins, outs, n, max_iter)
+            busy += time.perf_counter() - t
             with lock:
-                state["frames"] += n
-                state["bits"] += int(sum(batch.kldpc[f] for f in fl))
-                state["iters"] += it if it >= 0 else batch.max_iter
-                state["nb"] += 1
-        for h, p in handles.values():
+                st["frames"] += n
+                st["bits"] += int(sum(kl_of(f) for f in fl))
+                st["iters"] += it if it >= 0 else max_iter
+                st["failed"] += it < 0
+                st["nb"] += 1
+        with lock:
+            st["busy"] = max(st["busy"], busy)
+        for h, _ in handles.values():
             lib.drx_ldpc_free(h)
 
-    threads = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(len(cpus))]
     t0 = time.perf_counter()
     deadline[0] = t0 + seconds
     for t in threads:
         t.start()
     for t in threads:
         t.join()
-    dt = time.perf_counter() - t0
+    wall = time.perf_counter() - t0
+    return {"threads": len(cpus), "gbit_s": round(st["bits"] / wall / 1e9, 4), "frames_s": round(st["frames"] / wall, 1),
+            "frames": st["frames"], "batches": st["nb"], "batches_failed": st["failed"],
+            "avg_batch_iterations": round(st["iters"] / max(1, st["nb"]), 2), "wall_s": round(wall, 2), "lanes": lanes}
+
+
+def cpu_baseline(keys, llr_of, kl_of, max_iter, seconds=8.0, what="the workload"):
+    """The reference decoder (libdvbrx ldpc.c built from its sources into
+    oracle/_ref) on this host, per build (AVX-512BW and AVX2 where the CPU
+    has them): all cores of the box's share (one thread per physical core,
+    pinned) for `seconds`, and one core for half as long.  value = the
+    best all-core rate."""
+    info, per_core = host_cpu()
+    builds = ref_builds()
+    if not builds:
+        return None
+    allc = per_core[:min(CPU_SHARE, len(per_core))]
+    runs = []
+    for name, lib, desc in builds:
+        for cpus, sec in ((allc, seconds), (allc[:1], seconds / 2)):
+            r = ref_decode_rate(lib, keys, llr_of, kl_of, max_iter, cpus, sec)
+            r.update(build=name, desc=desc)
+            runs.append(r)
+            print(f"# cpu baseline {name} x{len(cpus)}: {r['gbit_s']} Gbit/s", file=sys.stderr, flush=True)
+    best = max((r for r in runs if r["threads"] == len(allc)), key=lambda r: r["gbit_s"])
+    single = max((r for r in runs if r["threads"] == 1), key=lambda r: r["gbit_s"])
     return {
-        "value": state["bits"] / dt / 1e9,
+        "value": best["gbit_s"],
         "unit": "Gbit/s",
-        "cores": nthreads,
-        "kind": kind,
-        "sample": (f"{state['frames']} frames decoded, cycling over the {batch.nframes} of the workload ({state['nb']} batches of <= {lanes} frames of one "
-                   f"code, shuffled), {dt:.1f} s wall, {nthreads} pinned threads, {what}; batch iterations avg "
-                   f"{state['iters'] / max(1, state['nb']):.1f}"),
-        "frames_per_s": state["frames"] / dt,
+        "cores": best["threads"],
+        "kind": "reference",
+        "sample": (f"{best['frames']} frames of {what} decoded in {best['wall_s']} s ({best['batches']} batches of <= "
+                   f"{best['lanes']} frames of one code, shuffled, cycling over the workload), {best['threads']} threads "
+                   f"pinned one per physical core, {best['desc']}; batch iterations avg {best['avg_batch_iterations']}"),
+        "single_core": single["gbit_s"],
+        "single_core_build": single["build"],
+        "host": info,
+        "runs": runs,
     }
+
+
+def sk_name(di):
+    """Kernel name of a split kernel instance from its degree code
+    (2 * slots per lane - 2 + 100 * wide), plain arithmetic."""
+    return f"ldpc_sk_kernel<{(di % 100 + 2) // 2}, {di // 100}, 0>"
 
 
 def copy_calibration(mib):
@@ -162,7 +235,7 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
 
 
-def config1(dec, reps=20):
+def config1(dec, reps=20, cpu=True):
     """The reference's own CPU-runnable case: 64 DVB-S2 normal 1/2 QPSK
     frames at Es/N0 1.5 dB, max 50 iterations.  Timed three ways: the
     device-resident batch API per-frame and as one group (the reference's
@@ -201,10 +274,52 @@ def config1(dec, reps=20):
     drx.close()
     res["dropin_host_buffers"] = {"ms_per_call": round(dt * 1e3, 3), "gbit_s": round(b.info_bits / dt / 1e9, 3),
                                   "result": int(r)}
+    if cpu:
+        res["cpu_baseline"] = config1_cpu(b)
     return res
 
 
-def config3(dec, margin_db=0.2, reps=5):
+def config1_cpu(b, reps=10):
+    """The reference decoder on config 1 as tests/test_ldpc.c:77-95 times
+    it: the 64 frames in batches of LANES on one thread (a batch cannot
+    be split over threads), clock around the decode loop; best of reps
+    after a warm-up, per build."""
+    _, per_core = host_cpu()
+    try:
+        orig = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {per_core[0]})
+    except (AttributeError, OSError, IndexError):
+        orig = None
+    out = []
+    try:
+        for name, lib, desc in ref_builds():
+            lanes = lib.drx_ldpc_lanes()
+            p = ldpcgpu.fec_lookup(0, "1_2").as_struct()
+            h = lib.drx_ldpc_new(C.byref(p), b"1_2")
+            o = np.zeros((b.nframes, 8100), np.uint8)
+            best, its = None, []
+            for r in range(reps + 1):
+                its = []
+                t = time.perf_counter()
+                for f0 in range(0, b.nframes, lanes):
+                    n = min(lanes, b.nframes - f0)
+                    ins = (C.c_void_p * n)(*[b.frame_llr(f).ctypes.data for f in range(f0, f0 + n)])
+                    outs = (C.c_void_p * n)(*[o[f].ctypes.data for f in range(f0, f0 + n)])
+                    its.append(lib.drx_ldpc_decode(h, ins, outs, n, b.max_iter))
+                dt = time.perf_counter() - t
+                if r and (best is None or dt < best):
+                    best = dt
+            lib.drx_ldpc_free(h)
+            out.append({"build": name, "desc": desc, "threads": 1, "ms_per_call": round(best * 1e3, 3),
+                        "gbit_s": round(b.info_bits / best / 1e9, 4), "frames_s": round(b.nframes / best, 1),
+                        "batch_iterations": its})
+    finally:
+        if orig is not None:
+            os.sched_setaffinity(0, orig)
+    return out
+
+
+def config3(dec, margin_db=0.2, reps=5, cpu=False, cpu_seconds=6.0):
     """DVB-S2X high-order batch (16384 normal frames on 64/128/256APSK
     MODCODs, 4096 short on QPSK/8PSK, threshold + margin_db, generated on
     the GPU) decoded by a prepared plan: launch shape chosen by the
@@ -234,6 +349,29 @@ def config3(dec, margin_db=0.2, reps=5):
         E, n = params[c]["links_total"], params[c]["n"]
         hbm += n + (int(b.kldpc[f]) + 7) // 8 + 8
         lds += 4 * E * int(it[f]) + E * (int(it[f]) + 1)
+    cpu_rec = None
+    if cpu:
+        # the reference decoder on the same LLRs (downloaded), all cores of
+        # the share, widest build, for `cpu_seconds`
+        llr = np.zeros(b.llr_bytes, np.int8)
+        dec.download(llr, b.d_llr)
+        keys = []
+        for pls, count in b.groups:
+            m = ldpcgpu.pls_lookup(pls)
+            keys += [(m["short_frame"], m["rate"])] * count
+        offs = b.frames["llr_off"].astype(np.int64)
+        ns = np.array([ldpcgpu.fec_lookup(*k).nldpc for k in keys], np.int64)
+        builds = ref_builds()
+        if builds:
+            info_h, per_core = host_cpu()
+            name, lib, desc = builds[0]
+            r = ref_decode_rate(lib, keys, lambda f: llr[offs[f]:offs[f] + ns[f]], lambda f: int(b.kldpc[f]),
+                                50, per_core[:min(CPU_SHARE, len(per_core))], cpu_seconds)
+            cpu_rec = {"value": r["gbit_s"], "unit": "Gbit/s", "cores": r["threads"], "kind": "reference",
+                       "sample": f"{r['frames']} frames of this batch in {r['wall_s']} s, batches of <= {r['lanes']} "
+                                 f"frames of one MODCOD, {r['threads']} threads pinned one per physical core, {desc}; "
+                                 f"batch iterations avg {r['avg_batch_iterations']}, {r['batches_failed']} of "
+                                 f"{r['batches']} batches not converged"}
     b.free()
     return {
         "workload": f"S2X high order: {b.nframes} frames ({sum(c for p, c in b.groups if p < 0xd8)} normal on "
@@ -244,6 +382,8 @@ def config3(dec, margin_db=0.2, reps=5):
         "roofline_hbm": {"achieved_gbs": round(hbm / sec / 1e9, 2), "frac": round(hbm / sec / 1e9 / HBM_PEAK_GBS, 5)},
         "roofline_lds": {"achieved_gbs": round(lds / sec / 1e9, 1), "frac": round(lds / sec / 1e9 / LDS_PEAK_GBS, 5)},
         "autotuner": info,
+        "cpu_baseline": cpu_rec,
+        "gpu_vs_cpu": round(b.info_bits / sec / 1e9 / cpu_rec["value"], 2) if cpu_rec and cpu_rec["value"] > 0 else None,
     }
 
 
@@ -331,7 +471,7 @@ T2_PLPS = [
 T2_TI_BLOCKS = 8
 
 
-def t2_cpu_job(plp, cells, csi, max_iter, seconds=8.0, max_threads=16):
+def t2_cpu_job(plp, cells, csi, max_iter, seconds=8.0, max_threads=CPU_SHARE):
     """The reference's DVB-T2 FEC job (dvbt2.c decode_job, built from its
     sources into oracle/_ref) on this host: one thread per core, each
     decoding calls of 64 blocks of the same cells until the deadline."""
@@ -345,11 +485,9 @@ def t2_cpu_job(plp, cells, csi, max_iter, seconds=8.0, max_threads=16):
     if lib is None:
         return None
     lib.t2ref_decode.argtypes = [C.c_int] * 7 + [C.c_void_p] * 2 + [C.c_int] + [C.c_void_p] * 5
-    try:
-        cpus = sorted(os.sched_getaffinity(0))
-    except AttributeError:
-        cpus = list(range(os.cpu_count() or 1))
-    nthreads = max(1, min(max_threads, len(cpus)))
+    _, per_core = host_cpu()
+    cpus = per_core[:min(max_threads, len(per_core))]
+    nthreads = len(cpus)
     nb = min(64, cells.shape[0])
     c = np.ascontiguousarray(cells[:nb])
     h = np.ascontiguousarray(csi[:nb])
@@ -359,7 +497,7 @@ def t2_cpu_job(plp, cells, csi, max_iter, seconds=8.0, max_threads=16):
 
     def worker(tid):
         try:
-            os.sched_setaffinity(0, {cpus[tid % len(cpus)]})
+            os.sched_setaffinity(0, {cpus[tid]})
         except (AttributeError, OSError):
             pass
         bb = np.zeros(nb * kbch // 8, np.uint8)
@@ -384,7 +522,7 @@ def t2_cpu_job(plp, cells, csi, max_iter, seconds=8.0, max_threads=16):
     blocks = sum(count)
     return {"value": round(blocks * kbch / sec / 1e9, 4), "unit": "Gbit/s", "cores": nthreads, "kind": "reference",
             "blocks_s": round(blocks / sec, 1),
-            "sample": f"{what}, {nthreads} threads, calls of {nb} blocks, {blocks} blocks in {sec:.1f} s"}
+            "sample": f"{what}, {nthreads} threads pinned one per physical core, calls of {nb} blocks, {blocks} blocks in {sec:.1f} s"}
 
 
 def t2_record(dec, reps=5, cpu=True):
@@ -467,7 +605,7 @@ def t2_record(dec, reps=5, cpu=True):
 MICRO_CODES = [(0, "1_4"), (0, "1_3"), (0, "1_2"), (0, "3_5"), (0, "2_3"), (0, "3_4"), (0, "5_6"), (0, "9_10"),
                (1, "1_2"), (1, "3_4"), (1, "8_9"), (0, "154_180"), (0, "2_3_t2")]
 MICRO_ESN0 = {"1_4": -3.5, "1_3": -2.6, "1_2": -0.5, "3_5": 0.7, "2_3": 1.5, "3_4": 2.5, "5_6": 3.7, "9_10": 4.9,
-              "8_9": 4.7, "154_180": 5.0, "2_3_t2": 1.5}
+              "8_9": 4.7, "154_180": 5.0, "2_3_t2": 1.5, "90_180": -0.5, "135_180": 2.5}
 
 
 def microbench(dec, nframes=2048, iters=10, reps=3, codes=None):
@@ -584,7 +722,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--frames", type=int, default=8192)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+                    help="all-core time of each reference build on the mixed workload (one core: half)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config1", action="store_true",
                     help="skip the side measurements (64-frame rate 1/2 batch, learned vs plain)")
@@ -645,6 +784,17 @@ def main():
         for di, nf, ms in dec.last_launches():
             launches.setdefault((di, nf), []).append(ms)
     dt = time.perf_counter() - t
+    # per-launch durations with the launches one after another, each kernel
+    # alone on the GPU (the rooflines use these), after the timed steps
+    serial_ms = {}
+    dec.set_concurrent(False)
+    for r in range(2):
+        dec.decode_batch(batch.frames, d_llr, d_out, d_it, d_ok)
+        dec.synchronize()
+        if r:
+            serial_ms = {(di, nf): ms for di, nf, ms in dec.last_launches()}
+            serial_decode_ms = dec.last_kernel_ms()
+    dec.set_concurrent(not args.serial)
     out = np.zeros_like(batch.ref)
     it = np.zeros(batch.nframes, np.int32)
     ok = np.zeros(batch.nframes, np.uint8)
@@ -652,9 +802,11 @@ def main():
     dec.download(it, d_it)
     dec.download(ok, d_ok)
     frame_err = 0
+    bad_f = np.zeros(batch.nframes, bool)
     by_modcod = {}  # pls -> [frames, frames in error]
     for f in range(batch.nframes):
         bad = int(not np.array_equal(batch.frame_ref(f), out[batch.layout[f][4]:batch.layout[f][4] + (batch.kldpc[f] + 7) // 8]))
+        bad_f[f] = bool(bad)
         frame_err += bad
         e = by_modcod.setdefault(int(batch.specs[f]["pls"]), [0, 0])
         e[0] += 1
@@ -670,14 +822,24 @@ def main():
     failing = [v for v in modcod_err.values() if 10 * v["in_error"] > v["frames"]]
     fer_rest = (frame_err - sum(v["in_error"] for v in failing)) / max(1, batch.nframes - sum(v["frames"] for v in failing))
     ms_step = dt / args.steps * 1e3
-    value = batch.info_bits / (dt / args.steps) / 1e9
+    sec_step = dt / args.steps
+    # The metric counts decoded information: only frames whose kldpc bits
+    # equal the sent ones count (the time of the others still counts).
+    # Beside it: all frames' bits, and the bits of the MODCODs with no frame
+    # in error in this batch (FER <= 1e-4 cannot be shown on their 60-200
+    # frames each; no error is the batch's best estimate of it).
+    value = float(batch.kldpc[~bad_f].sum()) / sec_step / 1e9
+    value_all = batch.info_bits / sec_step / 1e9
+    clean = np.array([by_modcod[int(batch.specs[f]["pls"])][1] == 0 for f in range(batch.nframes)])
+    value_clean_modcods = float(batch.kldpc[clean].sum()) / sec_step / 1e9
 
     # Roofline of the dominant decode kernel.  One decode call launches the
-    # two-slot kernel once per (degree class, frame size) with frames; the
-    # two degree classes are two kernel instances (ldpc_pk_kernel<0>: layer
-    # degree <= 12, <1>: <= 28).  The dominant instance is the one with the
-    # most launch time; its figures are per launch, averaged over its
-    # launches, as a profiler's per-kernel statistics give them.
+    # split kernel once per (instance, frame size) with frames; an
+    # instance is ldpc_sk_kernel<KS, WIDE, NMS> (KS slots per lane: the
+    # code's largest layer degree + 2, halved, rounded up; WIDE: the code's
+    # layers differ by more than one slot).  The dominant instance is the
+    # one with the most launch time; its figures are per launch, averaged
+    # over its launches, as a profiler's per-kernel statistics give them.
     # Algorithmic bytes per frame:
     #   HBM: N LLR bytes in + kldpc/8 bytes out + 8 (count, flag);
     #   LDS: per iteration read + write of posterior and message per edge
@@ -697,7 +859,8 @@ def main():
         lds_f[f] = 4 * E * int(it[f]) + E * (int(it[f]) + 1)
     launch_ms = {k: float(np.mean(v)) for k, v in launches.items()}
     inst = {}
-    for (di, nf), ms in launch_ms.items():
+    launch_order = []  # the decode's launches in dispatch order
+    for (di, nf), ms in serial_ms.items():
         size = None
         for short in (0, 1):
             m = (cls_of == di) & ((batch.n == 16200) == bool(short))
@@ -706,11 +869,12 @@ def main():
                 sel = m
         if size is None:
             continue
+        launch_order.append(f"{sk_name(di)} {size} x{nf}")
         e = inst.setdefault(di, [])
         e.append(dict(size=size, ms=ms, frames=nf, hbm=int(hbm_f[sel].sum()), lds=int(lds_f[sel].sum())))
     dom_di = max(inst, key=lambda d: sum(x["ms"] for x in inst[d]))
     dl = inst[dom_di]
-    kname = f"ldpc_pk_kernel<{dom_di}>"
+    kname = sk_name(dom_di)
     dom_ms = float(np.mean([x["ms"] for x in dl]))
     hbm_bytes = float(np.mean([x["hbm"] for x in dl]))
     lds_bytes = float(np.mean([x["lds"] for x in dl]))
@@ -718,17 +882,45 @@ def main():
     iters_total = int(it.sum())
     hbm_ach = hbm_bytes / (dom_ms * 1e-3) / 1e9
     lds_ach = lds_bytes / (dom_ms * 1e-3) / 1e9
-    # HBM bytes of the same launches from the committed rocprofv3 counter
-    # summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by what the
-    # counters read for a plain copy of known size)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_mixed_s2_x8192.json")
+    # HBM bytes and VALU instructions of the same launches from the
+    # committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE per
+    # dispatch, scaled by what the counters read for a plain copy of known
+    # size; SQ_INSTS_VALU per dispatch), for the workload's default size
+    traffic, valu = None, None
+    tpath = os.path.join(ROOT, "profiles", "counters_mixed_s2_x8192.json")
     if os.path.exists(tpath) and args.frames == 8192:
         try:
             tl = json.load(open(tpath))["launches"]
-            traffic = round(float(np.mean([tl[f"{kname} {x['size']}"]["bytes"] for x in dl])))
+            traffic = round(float(np.mean([tl[f"{kname} {x['size']}"]["hbm_bytes"] for x in dl])))
+            valu = float(np.mean([tl[f"{kname} {x['size']}"]["valu_insts"] for x in dl]))
         except (OSError, ValueError, KeyError):
-            traffic = None
+            traffic, valu = None, None
+    hbm_rec = {"achieved": round(hbm_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_ach / HBM_PEAK_GBS, 5),
+               "algorithmic_bytes": round(hbm_bytes), "traffic": traffic}
+    roofline = {
+        "bound": "hbm", "achieved": hbm_rec["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_rec["frac"],
+        "traffic": traffic,
+    }
+    if valu:
+        # the binding resource: VALU issue (instructions from the committed
+        # counter profile of these launches, per second of the launch)
+        v_ach = valu / (dom_ms * 1e-3) / 1e9
+        roofline = {"bound": "valu", "achieved": round(v_ach, 2), "peak": VALU_PEAK_GINST, "unit": "Ginst/s",
+                    "frac": round(v_ach / VALU_PEAK_GINST, 4), "traffic": traffic, "hbm": hbm_rec,
+                    "valu_insts_per_launch": round(valu)}
+    roofline.update({
+        "kernel": kname,
+        "kernel_ms": round(dom_ms, 4),
+        "algorithmic_bytes": round(hbm_bytes),
+        "launches": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in x.items() if k != "lds"} for x in dl],
+        "serial_launches_ms": {f"{sk_name(k[0])} x{k[1]}": round(v, 4) for k, v in sorted(serial_ms.items())},
+        "serial_decode_ms": round(serial_decode_ms, 4),
+        "launch_order": launch_order,
+        "concurrent_launches_ms": {f"{sk_name(k[0])} x{k[1]}": round(v, 4) for k, v in sorted(launch_ms.items())},
+        "concurrent_decode_ms": round(kms, 4),
+        "note": "per launch of the kernel alone on the GPU (the decode's launches run one after another in a pass "
+                "after the timed steps), averaged over its launches; the timed steps run them concurrently",
+    })
 
     # the same batch as a prepared plan: launch shape chosen by the
     # autotuner, replayed as one graph (after the timed steps; the plan
@@ -748,9 +940,9 @@ def main():
         pmed = float(np.median(pms))
         plan_rec = {"gbit_s": round(batch.info_bits / (pmed * 1e-3) / 1e9, 3), "ms": round(pmed, 3),
                     "autotuner": pinfo}
-    c1 = None if args.no_config1 else config1(dec)
+    c1 = None if args.no_config1 else config1(dec, cpu=not args.no_cpu)
     c4 = None if args.no_config1 else config4(dec)
-    c3 = None if args.no_config1 else [config3(dec, 0.2), config3(dec, 1.0)]
+    c3 = None if args.no_config1 else [config3(dec, 0.2, cpu=not args.no_cpu), config3(dec, 1.0, cpu=not args.no_cpu)]
     dm = None if args.no_config1 else demap_record(dec)
     bch = None if args.no_config1 else bch_record(dec)
     t2 = None if args.no_config1 else t2_record(dec, cpu=not args.no_cpu)
@@ -771,7 +963,14 @@ def main():
             dec2.close()
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(batch, args.cpu_seconds)
+        mc = {}
+        for sp in batch.specs:
+            if sp["pls"] not in mc:
+                m = ldpcgpu.pls_lookup(sp["pls"])
+                mc[sp["pls"]] = (m["short_frame"], m["rate"])
+        keys = [mc[sp["pls"]] for sp in batch.specs]
+        cpu = cpu_baseline(keys, batch.frame_llr, lambda f: int(batch.kldpc[f]), batch.max_iter, args.cpu_seconds,
+                           "the mixed DVB-S2 workload")
     dec.close()
 
     res = {
@@ -795,27 +994,15 @@ def main():
             "info_bits_per_step": batch.info_bits,
             "avg_iterations": round(float(it.mean()), 3),
             "fer": frame_err / batch.nframes,
+            "gbit_s_all_frames": round(value_all, 3),
+            "gbit_s_modcods_without_frame_errors": round(value_clean_modcods, 3),
+            "value_counts": "information bits of the frames decoded correctly, per second of the whole decode",
             "fer_modcods_in_error": modcod_err,
             "fer_without_failing_modcods": round(fer_rest, 6),
             "frames_unconverged": int((ok == 0).sum()),
             "autotuned_plan": plan_rec,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(hbm_ach, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(hbm_ach / HBM_PEAK_GBS, 5),
-            "traffic": traffic,
-            "kernel": f"{kname} (layer degree <= {dom_di})",
-            "kernel_ms": round(dom_ms, 4),
-            "algorithmic_bytes": round(hbm_bytes),
-            "launches": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in x.items() if k != "lds"} for x in dl],
-            "all_launches_ms": {f"deg<={k[0]} x{k[1]}": round(v, 4) for k, v in sorted(launch_ms.items())},
-            "decode_ms": round(kms, 4),
-            "note": "figures per launch of the kernel, averaged over its launches; the launches of one decode run "
-                    "concurrently on separate streams, so a launch's duration includes time shared with the others",
-        },
+        "roofline": roofline,
         "roofline_lds": {
             "achieved": round(lds_ach, 1),
             "peak": LDS_PEAK_GBS,
@@ -836,6 +1023,7 @@ def main():
     }
     if cpu:
         res["gpu_vs_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None
+        res["gpu_vs_cpu_single_core"] = round(value / cpu["single_core"], 1) if cpu["single_core"] > 0 else None
     print(json.dumps(res), flush=True)
 
 

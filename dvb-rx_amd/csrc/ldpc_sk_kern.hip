@@ -255,8 +255,12 @@ __device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, 
  * continues.  When the segment is exhausted and both slots are empty, the
  * workgroup moves to another segment with frames left, or exits.
  */
+/* the template arguments name the instance (slots per lane, wide,
+ * learned) in profiles; the body uses this file's build constants */
+template <int KS_, int WIDE_, int NMS_>
 __global__ void __launch_bounds__(SK_NT) ldpc_sk_kernel(PkArgs a)
 {
+    static_assert(KS_ == KS && WIDE_ == SK_WIDE && NMS_ == SK_NMS, "instance of this build");
     extern __shared__ __align__(16) uint16_t sm16[];
     __shared__ unsigned s_vote[3];
     __shared__ int s_claim[2], s_seg, s_newseg;
@@ -444,13 +448,13 @@ __global__ void __launch_bounds__(SK_NT) ldpc_sk_kernel(PkArgs a)
 
 hipError_t SK_NAME(sk_launch_, SK_KS, SK_WIDE, SK_NMS)(int grid, size_t lds, hipStream_t st, const PkArgs& a)
 {
-    hipLaunchKernelGGL(ldpc_sk_kernel, dim3(grid), dim3(SK_NT), lds, st, a);
+    hipLaunchKernelGGL((ldpc_sk_kernel<SK_KS, SK_WIDE, SK_NMS>), dim3(grid), dim3(SK_NT), lds, st, a);
     return hipGetLastError();
 }
 
 hipError_t SK_NAME(sk_occupancy_, SK_KS, SK_WIDE, SK_NMS)(size_t lds, int* per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ldpc_sk_kernel, SK_NT, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ldpc_sk_kernel<SK_KS, SK_WIDE, SK_NMS>, SK_NT, lds);
 }
 
 } // namespace lgk

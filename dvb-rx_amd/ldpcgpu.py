@@ -337,11 +337,13 @@ class Decoder:
         return self.L.ldpc_gpu_last_kernel_ms(self.ctx)
 
     def last_launches(self):
-        """[(kernel instance degree bound, frames, ms)] of the last decode's launches."""
-        di = (C.c_int * 16)()
-        nf = (C.c_int * 16)()
-        ms = (C.c_float * 16)()
-        n = self.L.ldpc_gpu_last_launches(self.ctx, di, nf, ms, 16)
+        """[(kernel instance code, frames, ms)] of the last decode's launches
+        (instance code: 2 * slots per lane - 2 + 100 * wide)."""
+        cap = 64  # more than the launches of one decode (instances x frame sizes)
+        di = (C.c_int * cap)()
+        nf = (C.c_int * cap)()
+        ms = (C.c_float * cap)()
+        n = self.L.ldpc_gpu_last_launches(self.ctx, di, nf, ms, cap)
         return [(di[k], nf[k], ms[k]) for k in range(n)]
 
     def decode_host(self, frames, llr, out_bytes):
