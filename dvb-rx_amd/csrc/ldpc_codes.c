@@ -16,9 +16,12 @@ static lg_code g_codes[sizeof(LG_CODES) / sizeof(*LG_CODES)];
 static pthread_once_t g_once = PTHREAD_ONCE_INIT;
 static int g_build_failed;
 
-/* Split-kernel tables (lg_code.sk_*), from the expanded code: slot
- * addresses per lane, and for layers with nodes sharing bits the slot
- * masks and hand-over cells of the two-frame kernel's rounds. */
+/* Split-kernel tables (lg_code.sk_*), from the expanded code: per layer
+ * and lane an address record (the LDS byte offsets of the lane's slot
+ * posteriors), and for layers with nodes sharing bits a tail record (slot
+ * masks and the hand-over cells of the two-frame kernel's rounds).  The
+ * records of one layer are consecutive, lane after lane, so a lane reads
+ * its record with a few wide loads at a fixed offset from the layer's. */
 static int build_split(lg_code* c)
 {
     const int q = c->q, K = c->k;
@@ -30,20 +33,19 @@ static int build_split(lg_code* c)
         if (d < dmin)
             dmin = d;
     }
-    const int ks = (dmax + 3) / 2, sk = (ks + 1) / 2;
+    const int ks = (dmax + 3) / 2;
     if (ks > 16)
         return -1;
+    const int wa = SK_WORDS(ks), ra = SK_RA(ks), rb = SK_RB(ks);
     c->sk_ks = ks;
     c->sk_wide = (dmin + 2) / 2 < ks - 1;
-    int rows = 0, mrows = 0;
-    for (int i = 0; i < q; i++) {
-        rows += sk + (c->nlev[i] > 1 ? 1 + sk : 0);
-        mrows += sk;
-    }
-    c->sk_rows = rows;
-    c->sk_msg_rows = mrows;
+    int words = 0;
+    for (int i = 0; i < q; i++)
+        words += LG_LANES * (ra + (c->nlev[i] > 1 ? rb : 0));
+    c->sk_words = words;
+    c->sk_msg_words = q * LG_LANES * wa;
     c->sk_lay = calloc((size_t)q * 4, sizeof(uint32_t));
-    c->sk_tab = calloc((size_t)rows * 720, sizeof(uint32_t));
+    c->sk_tab = calloc((size_t)words, sizeof(uint32_t));
     if (!c->sk_lay || !c->sk_tab)
         return -1;
     /* barriers: layer i must wait for the workgroup (bit 16) unless it
@@ -54,7 +56,7 @@ static int build_split(lg_code* c)
     uint8_t* used = calloc((size_t)(K / LG_M) + 1, 1);
     if (!used)
         return -1;
-    int row = 0, mrow = 0;
+    int off = 0;
     for (int i = 0; i < q; i++) {
         const int s = c->layer_start[i], d = c->layer_start[i + 1] - s, T = d + 2;
         const int tail = c->nlev[i] > 1;
@@ -82,14 +84,18 @@ static int build_split(lg_code* c)
         for (int e = 0; e < d; e++)
             used[(c->ent[s + e] >> 16) / LG_M] = 1;
         uint32_t* L = c->sk_lay + 4 * i;
+        const int arec = off, trec = off + LG_LANES * ra;
+        off += LG_LANES * (ra + (tail ? rb : 0));
         L[0] = (uint32_t)T | (uint32_t)c->nlev[i] << 8 | (uint32_t)bar << 16 | (uint32_t)kde << 24;
-        L[1] = (uint32_t)mrow;
-        L[2] = (uint32_t)row;
-        L[3] = tail ? (uint32_t)(row + sk) : 0u;
-        for (int t = 0; t < 720; t++) {
+        L[1] = (uint32_t)(i * LG_LANES * wa);
+        L[2] = (uint32_t)arec;
+        L[3] = tail ? (uint32_t)trec : 0u;
+        for (int t = 0; t < LG_LANES; t++) {
             const int j = t >> 1, h = t & 1;
             const uint64_t mt = c->meta[(size_t)i * LG_M + j], ps = c->pos[(size_t)i * LG_M + j];
             uint32_t dep = 0, succ = 0;
+            uint32_t* ar = c->sk_tab + arec + (size_t)t * ra;
+            uint32_t* tr = c->sk_tab + trec + (size_t)t * rb;
             for (int k = 0; k < ks; k++) {
                 const int p = 2 * k + h;
                 uint32_t ad = (uint32_t)c->n, cell = 0;
@@ -112,19 +118,17 @@ static int build_split(lg_code* c)
                         succ |= ((LG_META_SUCC(mt) >> e) & 1u) << k;
                     }
                 }
-                if (ad > 0xffffu || cell > 0xffffu || (cell && (k == 0 || k >= kde))) {
+                if (ad > (uint32_t)c->n || cell > 0xffffu || (cell && (k == 0 || k >= kde))) {
                     free(used);
                     return -1;
                 }
-                c->sk_tab[(size_t)(row + k / 2) * 720 + t] |= ad << (16 * (k & 1));
-                if (tail)
-                    c->sk_tab[(size_t)(row + sk + 1 + k / 2) * 720 + t] |= cell << (16 * (k & 1));
+                ar[k] = SK_POST_OFF + 2 * ad;
+                if (tail && k <= SK_KD(ks))
+                    tr[1 + k / 2] |= cell << (16 * (k & 1));
             }
             if (tail)
-                c->sk_tab[(size_t)(row + sk) * 720 + t] = dep | succ << 16;
+                tr[0] = dep | succ << 16;
         }
-        row += sk + (tail ? 1 + sk : 0);
-        mrow += sk;
     }
     free(used);
     return 0;

@@ -71,32 +71,45 @@ typedef struct lg_code {
                               with k > 1 */
     int max_xfer;          /* largest number of hand-overs of one layer */
     /* Tables of the split kernel (a lane pair per node, ldpc_sk.h): lane
-     * t = 2j + h takes the node's positions p = 2k + h (0 own parity, 1
-     * previous parity, 2 + r the layer's r-th information edge, those of
-     * shared groups first), k < sk_ks.  Rows of 720
-     * words, word t for lane t: per layer SK rows of slot posterior
-     * indices (two u16 a word, slot 2w in the low half; an empty slot
-     * points at index n, a spare cell), and for layers with nodes sharing
-     * bits a row of slot masks (dep in bits 0-15, succ in 16-31) and SK
-     * rows of hand-over cells; SK = (sk_ks + 1) / 2. */
+     * t = 2j + h of LG_LANES takes the node's positions p = 2k + h (0 own
+     * parity, 1 previous parity, 2 + r the layer's r-th information edge,
+     * those of shared groups first), k < sk_ks.  Per layer and lane an
+     * address record of SK_RA(sk_ks) words: word k the LDS byte offset of
+     * slot k's posterior (2 x its index; an empty slot: 2n, a spare cell);
+     * for layers with nodes sharing bits a tail record of SK_RB(sk_ks)
+     * words: the slot masks (dep in bits 0-15, succ in 16-31), then the
+     * hand-over cells of slots 0 .. SK_KD, two u16 a word (slot 2w in the
+     * low half).  A layer's records are consecutive in lane order. */
     int sk_ks;             /* slots per lane: (largest degree + 2) / 2 rounded up */
     int sk_wide;           /* 1 if a layer has fewer slots than sk_ks - 1 */
     uint32_t* sk_lay;      /* q x 4: positions | levels << 8 | barrier << 16 (the
                               layer waits for the workgroup's previous layers)
                               | kde << 24 (slots 1 .. kde - 1 hold the edges
                               of groups shared by nodes of the layer, the
-                              first positions after the parity bits), first
-                              message row, first address row, mask row (0:
-                              none) */
-    uint32_t* sk_tab;      /* sk_rows x 720 */
-    int sk_rows;
-    int sk_msg_rows;       /* message rows (of 720 words) per frame pair */
+                              first positions after the parity bits); first
+                              message word (of a frame pair's block: per layer
+                              and lane SK_WORDS(sk_ks) words); first word of
+                              the address records; first word of the tail
+                              records (0: none) */
+    uint32_t* sk_tab;      /* sk_words words */
+    int sk_words;
+    int sk_msg_words;      /* message words per frame pair */
 } lg_code;
 
 #define LG_MAX_XFER 8      /* hand-overs per layer the kernels provide for */
+#define LG_LANES 720       /* lanes of the split kernel: a pair per check node */
 /* slots per lane that may hold shared edges (1 .. SK_KD) in a code whose
  * split kernel has ks slots per lane: kde - 1 <= SK_KD(ks) */
 #define SK_KD(ks) ((ks) <= 4 ? 1 : (ks) <= 7 ? 3 : (ks) <= 11 ? 5 : 6)
+/* words per lane and layer: messages (two slots of both frames a word),
+ * address record (padded to whole 16-byte loads), tail record (masks and
+ * the cells of slots 0 .. SK_KD, padded likewise) */
+#define SK_WORDS(ks) (((ks) + 1) / 2)
+/* byte offset of the posteriors in the split kernel's LDS (its control
+ * words come first); the address records hold SK_POST_OFF + 2 x index */
+#define SK_POST_OFF 32
+#define SK_RA(ks) ((ks) <= 2 ? 2 : ((ks) + 3) & ~3)
+#define SK_RB(ks) ((1 + SK_KD(ks) / 2 + 1 + 3) & ~3)
 #define LG_XENT_SHIFT(e) ((int)((e) & 0x1ffu))
 #define LG_XENT_KM1(e) ((int)(((e) >> 9) & 3u))
 #define LG_XENT_T0(e) ((int)(((e) >> 11) & 15u))

@@ -209,14 +209,18 @@ static int upload_tables(ldpc_gpu_ctx* c)
         d.dmax = cd->max_layer_deg;
         d.dmin = 255;
         d.msg_words = cd->msg_words;
-        d.pk_msg_words = cd->sk_msg_rows * 720;
+        d.pk_msg_words = cd->sk_msg_words;
         d.code_id = id;
         d.max_xfer = cd->max_xfer;
         d.sk_inst = sk_instance(cd);
         if (d.sk_inst < 0)
             return set_err(c, LDPC_GPU_EINVAL, "code %d: no split kernel for %d slots per lane", id, cd->sk_ks);
-        const uint32_t base_row = (uint32_t)(adt.size() / 720);
-        adt.insert(adt.end(), cd->sk_tab, cd->sk_tab + (size_t)cd->sk_rows * 720);
+        /* the split kernel's descriptors hold byte offsets: of the layer's
+         * messages in a message block, of its records in adt */
+        const size_t base = adt.size();
+        if (4 * (base + (size_t)cd->sk_words) > 0x7fffffffu)
+            return set_err(c, LDPC_GPU_EINVAL, "split kernel tables exceed 2 GiB");
+        adt.insert(adt.end(), cd->sk_tab, cd->sk_tab + cd->sk_words);
         ent.insert(ent.end(), cd->ent, cd->ent + cd->nent);
         for (int i = 0; i < cd->q; i++) {
             const int st = cd->layer_start[i], deg = cd->layer_start[i + 1] - st;
@@ -224,7 +228,8 @@ static int upload_tables(ldpc_gpu_ctx* c)
             lay.push_back(make_uint2((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24,
                                      (uint32_t)cd->msg_off[i]));
             const uint32_t* sl = cd->sk_lay + 4 * i;
-            pkl.push_back(make_uint4(sl[0], sl[1], sl[2] + base_row, sl[3] ? sl[3] + base_row : 0u));
+            pkl.push_back(make_uint4(sl[0], 4 * sl[1], (uint32_t)(4 * (base + sl[2])),
+                                     cd->nlev[i] > 1 ? (uint32_t)(4 * (base + sl[3])) : 0u));
         }
         meta.insert(meta.end(), cd->meta, cd->meta + cd->r);
         /* every layer degree must be one the code's class kernel handles */
@@ -678,9 +683,10 @@ struct PkLayout {
     size_t blocks; /* message blocks: sum of the grids */
 };
 
-/* LDS of a launch: both slots' posteriors (2 bytes per bit of its longest
- * code), then two hand-over buffers of maxx * 360 cells */
-static size_t pk_hoff(const PkLayout& L, int k) { return (2 * ((size_t)L.maxn[k] + 1) + 15) & ~(size_t)15; }
+/* LDS of a launch: the kernel's control words, both slots' posteriors (2
+ * bytes per bit of its longest code, and a spare cell) from SK_POST_OFF,
+ * then two hand-over buffers of maxx * 360 cells */
+static size_t pk_hoff(const PkLayout& L, int k) { return (SK_POST_OFF + 2 * ((size_t)L.maxn[k] + 1) + 15) & ~(size_t)15; }
 static size_t pk_lds(const PkLayout& L, int k) { return pk_hoff(L, k) + 2 * sizeof(uint16_t) * 360 * (size_t)L.maxx[k]; }
 
 static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, float grid_scale, PkLayout& L)

@@ -97,9 +97,9 @@ struct PkArgs {
     const DCode* codes;
     size_t hoff;           /* byte offset of the hand-over buffers in LDS */
     int hstride;           /* u16 cells per hand-over buffer */
-    const uint4* pkl;      /* per layer: positions | levels << 8, first message row, first address
-                              row, mask row (ldpc_sk_kern.hip) */
-    const uint32_t* adt;   /* rows of 720 words: slot addresses, masks, hand-over cells */
+    const uint4* pkl;      /* per layer: positions | levels << 8 | ..., byte offsets of its messages,
+                              address records and tail records (ldpc_sk_kern.hip) */
+    const uint32_t* adt;   /* lane records of every code (lg_code.sk_tab) */
     const DFrame* frames;
     const int* list;       /* frame indices, grouped by code into segments */
     const int2* seg;       /* per segment: first list entry, length */

@@ -83,7 +83,18 @@ LG_PK_HD uint32_t pk_msg_sel(int odd, int first0, int first1)
     return b1 << 24 | 0x0Cu << 16 | b0 << 8 | 0x0Cu;
 }
 
-LG_PK_HD lg_pk pk_abs(lg_pk x) { return pk_max(x, pk_subs(pk_dup(0), x)); }
+/* unsigned saturating subtract of c from both lanes (v_pk_sub_u16 clamp) */
+LG_PK_HD lg_pk pk_subu_sat(lg_pk a, unsigned short c)
+{
+    return pks(__builtin_elementwise_sub_sat(pku(a), lg_pku{ c, c }));
+}
+
+/* the magnitude (units of 256, low byte 0) of a ones' complement
+ * magnitude a = x ^ (x >> 15): (a + 0x80) & 0xff00, saturating */
+LG_PK_HD lg_pk pk_mag1(lg_pk a)
+{
+    return pk_of(pk_bits(pks(__builtin_elementwise_add_sat(pku(a), lg_pku{ 0x80, 0x80 }))) & 0xFF00FF00u);
+}
 
 /* x = sat(L - m) of position p from the two posterior bytes w */
 LG_PK_HD lg_pk pk_in_w(uint32_t w, const uint32_t* mw, int p, uint32_t sel_even, uint32_t sel_odd)
@@ -105,10 +116,10 @@ struct PkOut {
     lg_pk mn1, dmn;
 };
 
-/* Learned magnitude of one minimum, per lane: clamp((min(m, 127) * S - B)
- * >> 4, 0, 127) in units of 256, S (0..63) and B (-1024..1023) in the
- * lanes of ps / pb, so |min * S - B| < 2^15.  S = 16, B = 16 gives
- * pk_mags' values. */
+/* Learned magnitude of one minimum m (units of 256, low byte 0), per
+ * lane: clamp((min(m, 127) * S - B) >> 4, 0, 127) in units of 256, S
+ * (0..63) and B (-1024..1023) in the lanes of ps / pb, so |min * S - B| <
+ * 2^15.  S = 16, B = 16 gives the plain decoder's values. */
 LG_PK_HD lg_pk pk_nms_mag(lg_pk m, lg_pk ps, lg_pk pb)
 {
     const lg_pku v = pku(pk_minu(m, pk_dup(0x7F00))) >> (unsigned short)8;

@@ -25,6 +25,17 @@
  * Messages: slot k in word k / 2 of the half, bytes [k even: 0, 1; odd:
  * 2, 3] (slot 0 of the two frames, then slot 1), as the selectors of
  * pk_msg_sel give them.
+ *
+ * Magnitudes are kept in ones' complement: a = x ^ t with t = x >> 15
+ * (the lane's sign mask), which is |x| for x >= 0 and |x| - 1 for x < 0.
+ * It orders the edges as |x| does, ties broken by sign, so the two
+ * smallest a are edges with the two smallest |x|, and the magnitude of an
+ * a is (a + 0x80) >> 8 (in units of 256); the sign mask comes back as a ^
+ * x where the output needs it.  The node's sign s is the XOR of the masks.
+ *
+ * Slot addresses are LDS byte offsets (the code registry's address
+ * records): the posterior of bit v at SK_POST_OFF + 2v from the start of
+ * the kernel's LDS, the functions below take that start ("lds").
  */
 #ifndef LDPC_SK_H
 #define LDPC_SK_H
@@ -33,24 +44,41 @@
 
 #include "ldpc_pk.h"
 
-#define SK_WORDS(ks) (((ks) + 1) / 2)
+/* the two posterior bytes at byte offset ad of the LDS.  On the device
+ * the kernel's LDS is all dynamic and starts at address 0 (the kernel
+ * checks it), so ad is the LDS address itself. */
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) uint16_t sk_lds_u16;
+LG_PK_HD uint32_t sk_ld(const uint16_t* lds, uint32_t ad)
+{
+    (void)lds;
+    return *(const sk_lds_u16*)(size_t)ad;
+}
+LG_PK_HD void sk_st(uint16_t* lds, uint32_t ad, uint16_t v)
+{
+    (void)lds;
+    *(sk_lds_u16*)(size_t)ad = v;
+}
+#else
+LG_PK_HD uint32_t sk_ld(const uint16_t* lds, uint32_t ad) { return *(const uint16_t*)((const uint8_t*)lds + ad); }
+LG_PK_HD void sk_st(uint16_t* lds, uint32_t ad, uint16_t v) { *(uint16_t*)((uint8_t*)lds + ad) = v; }
+#endif
 
 template <int KS>
 struct SkHalf {
-    int ad[KS];      /* posterior index of each slot */
+    uint32_t ad[KS]; /* posterior byte offset of each slot */
     lg_pk x[KS];     /* extrinsic inputs, both frames */
-    lg_pk a[KS];     /* their magnitudes */
+    lg_pk a[KS];     /* their magnitudes, ones' complement */
     lg_pk min0, min1, s;
 };
 
-/* slot posterior indices from the address words (two u16 a word, slot 2w
- * in the low half) */
+/* slot posterior offsets from the lane's address record */
 template <int KS>
 LG_PK_HD void sk_addrs(SkHalf<KS>& n, const uint32_t* aw)
 {
 #pragma unroll
     for (int k = 0; k < KS; k++)
-        n.ad[k] = (int)((k & 1) ? aw[k >> 1] >> 16 : aw[k >> 1] & 0xffffu);
+        n.ad[k] = aw[k];
 }
 
 /* skip mask of slot k: all ones if it is empty (2k + h >= T, or node
@@ -71,7 +99,7 @@ LG_PK_HD void sk_read(uint32_t* raw, const uint16_t* post, const SkHalf<KS>& n)
 {
 #pragma unroll
     for (int k = 0; k < KS; k++)
-        raw[k] = post[n.ad[k]];
+        raw[k] = sk_ld(post, n.ad[k]);
 }
 
 /* Phase A of a half from its posteriors raw: minimums and sign of the
@@ -90,12 +118,13 @@ LG_PK_HD void sk_phase_a(SkHalf<KS>& n, const uint32_t* raw, int T, int h, uint3
         uint32_t skip = sk_skip<KA>(k, T, h, skip0);
         if (TAIL)
             skip |= 0u - ((dep >> k) & 1);
-        const lg_pk a = pk_of(pk_bits(pk_abs(x)) | skip);
+        const lg_pk t = x >> (short)15;
+        const lg_pk a = pk_of(pk_bits(x ^ t) | skip);
         n.x[k] = x;
         n.a[k] = a;
         n.min1 = pk_minu(n.min1, pk_maxu(n.min0, a));
         n.min0 = pk_minu(n.min0, a);
-        n.s = pk_of(pk_bits(n.s) ^ (pk_bits(x) & ~skip));
+        n.s = pk_of(pk_bits(n.s) ^ (pk_bits(t) & ~skip));
     }
 }
 
@@ -103,12 +132,13 @@ LG_PK_HD void sk_phase_a(SkHalf<KS>& n, const uint32_t* raw, int T, int h, uint3
 template <int KS>
 LG_PK_HD void sk_fold(SkHalf<KS>& n, int k, lg_pk x)
 {
-    const lg_pk a = pk_abs(x);
+    const lg_pk t = x >> (short)15;
+    const lg_pk a = x ^ t;
     n.x[k] = x;
     n.a[k] = a;
     n.min1 = pk_minu(n.min1, pk_maxu(n.min0, a));
     n.min0 = pk_minu(n.min0, a);
-    n.s = n.s ^ x;
+    n.s = n.s ^ t;
 }
 
 /* merge the partner half's minimums and sign (p0, p1, ps) */
@@ -125,36 +155,37 @@ template <int KS, bool NMS>
 LG_PK_HD PkOut sk_mags(const SkHalf<KS>& n, lg_pk ps, lg_pk pb)
 {
     if constexpr (NMS) {
-        const lg_pk mn0 = pk_nms_mag(n.min0, ps, pb), mn1 = pk_nms_mag(n.min1, ps, pb);
+        const lg_pk mn0 = pk_nms_mag(pk_mag1(n.min0), ps, pb), mn1 = pk_nms_mag(pk_mag1(n.min1), ps, pb);
         return PkOut{ mn1, mn0 - mn1 };
     } else {
         (void)ps;
         (void)pb;
-        const lg_pk mn0 = pk_max(pk_minu(n.min0, pk_dup(0x7F00)) - pk_dup(0x0100), pk_dup(0));
-        const lg_pk mn1 = pk_max(pk_minu(n.min1, pk_dup(0x7F00)) - pk_dup(0x0100), pk_dup(0));
+        /* max(min(m, 127) - 1, 0) of m = (a + 0x80) >> 8, in units of 256:
+         * min((a -sat 0x80) & 0xff00, 126 * 256) */
+        const lg_pk mn0 = pk_minu(pk_of(pk_bits(pk_subu_sat(n.min0, 0x80)) & 0xFF00FF00u), pk_dup(0x7E00));
+        const lg_pk mn1 = pk_minu(pk_of(pk_bits(pk_subu_sat(n.min1, 0x80)) & 0xFF00FF00u), pk_dup(0x7E00));
         return PkOut{ mn1, mn0 - mn1 };
     }
 }
 
-/* o_k: |o| = mn1 if |x_k| is the node's smallest input, else mn0; sign =
- * product of the other signs.  a >= min0 always, so a - min0 is 0 exactly
- * for the smallest input.  Where a 0xff low byte makes |x_k| differ from
- * min0 by less than 256, both are 127 and mn0 == mn1 (both minimums >=
- * 127). */
+/* o_k: |o| = mn1 if x_k is the node's smallest input, else mn0; sign =
+ * product of the other signs (the node's sign mask s with x_k's, a_k ^ x_k,
+ * taken out).  a >= min0 always, so a - min0 is 0 exactly for the input
+ * that is the smallest; inputs of equal magnitude give mn0 == mn1, and so
+ * does a 0xff low byte (|x| = 127: both minimums at 127). */
 template <int KS>
 LG_PK_HD lg_pk sk_out(const SkHalf<KS>& n, const PkOut& m, int k)
 {
     const lg_pk mag = pk_sel_mag(n.a[k] - n.min0, m.dmn, m.mn1);
-    const lg_pk t = (n.s ^ n.x[k]) >> 15;
+    const lg_pk t = n.s ^ n.a[k] ^ n.x[k];
     return (mag ^ t) - t;
 }
 
 /* Phase C of a half: final posteriors (TAIL: the slots in succ are written
  * by a later node of the layer) and the new messages, clamped to [-32,
- * 31] (ldpc.c:343), word w to mw_out[stride * w] */
+ * 31] (ldpc.c:343), word w to mo[w] */
 template <int KS, bool TAIL, bool NMS>
-LG_PK_HD void sk_phase_c(const SkHalf<KS>& n, uint16_t* post, uint32_t succ, uint32_t* mw_out, int stride, lg_pk ps,
-                         lg_pk pb)
+LG_PK_HD void sk_phase_c(const SkHalf<KS>& n, uint16_t* post, uint32_t succ, uint32_t* mo, lg_pk ps, lg_pk pb)
 {
     const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
     uint32_t even = 0;
@@ -162,14 +193,14 @@ LG_PK_HD void sk_phase_c(const SkHalf<KS>& n, uint16_t* post, uint32_t succ, uin
     for (int k = 0; k < KS; k++) {
         const lg_pk o = sk_out(n, m, k);
         if (!TAIL || !((succ >> k) & 1))
-            post[n.ad[k]] = pk_post_bytes(pk_adds(n.x[k], o));
+            sk_st(post, n.ad[k], pk_post_bytes(pk_adds(n.x[k], o)));
         const uint32_t c = pk_bits(pk_msg_clamp(o));
         if (!(k & 1)) {
             even = c;
             if (k + 1 == KS)
-                mw_out[stride * (k / 2)] = lg_perm(0, even, PK_PACK_MSG);
+                mo[k / 2] = lg_perm(0, even, PK_PACK_MSG);
         } else {
-            mw_out[stride * (k / 2)] = lg_perm(c, even, PK_PACK_MSG);
+            mo[k / 2] = lg_perm(c, even, PK_PACK_MSG);
         }
     }
 }
@@ -179,13 +210,14 @@ LG_PK_HD void sk_phase_c(const SkHalf<KS>& n, uint16_t* post, uint32_t succ, uin
  * frames), nz bit 7 / 15 stays set while no posterior is zero (a byte b is
  * non-zero iff bit 7 of ((b & 0x7f) + 0x7f) | b is set) */
 template <int KS, int KA>
-LG_PK_HD void sk_check(const uint16_t* post, const int* ad, int T, int h, uint32_t skip0, uint32_t& par, uint32_t& nz)
+LG_PK_HD void sk_check(const uint16_t* post, const uint32_t* ad, int T, int h, uint32_t skip0, uint32_t& par,
+                       uint32_t& nz)
 {
     par = 0;
     nz = 0x8080;
 #pragma unroll
     for (int k = 0; k < KS; k++) {
-        const uint32_t r = post[ad[k]];
+        const uint32_t r = sk_ld(post, ad[k]);
         const uint32_t skip = sk_skip<KA>(k, T, h, skip0);
         par ^= r & ~skip;
         nz &= ((r & 0x7F7F) + 0x7F7F) | r | skip;
@@ -215,7 +247,8 @@ LG_PK_HD uint32_t sk_check_bits(uint32_t par, uint32_t nz)
  * sequential result (the hand-over graph has no cycle, so its fixed point
  * is unique).  The rounds stop at the first of the two.
  *
- * hw: the slots' hand-over cells, two u16 a word as the addresses; a slot
+ * hw: the slots' hand-over cells, two u16 a word (the tail record after
+ * its mask word); a slot
  * at rank r on its bit (its node is the r-th of the layer touching the
  * bit, 0-based) hands over through cell c = (t0 + r) * 360 + m, m the
  * bit's index in its group, and reads its predecessor's at c - 360. */

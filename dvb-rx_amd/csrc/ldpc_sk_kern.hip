@@ -7,14 +7,19 @@
  * differ by more than one slot, every slot may be empty) and SK_NMS (0
  * plain, 1 learned), with its launch entry points.
  *
- * Tables (PkArgs.adt, rows of 720 words, word t for lane t): per layer the
- * slot addresses (SK_WORDS(KS) rows), and for layers with nodes sharing
- * bits the slots' dependency masks (one row: dep in bits 0-15, succ in
- * 16-31) and hand-over cells (SK_WORDS(KS) rows).  Layer descriptor
- * (PkArgs.pkl): x = positions T | level count << 8 | barrier << 16 |
- * kde << 24 (slots 1 .. kde - 1 hold the shared edges), y = first message
- * row (rows of 720 words in the workgroup's message block), z = first
- * address row, w = the mask row (0 when the layer has none).
+ * Tables (PkArgs.adt, lane records as the code registry builds them,
+ * ldpc_codes.h): per layer a record of SK_RA(KS) words per lane (the LDS
+ * byte offsets of its slot posteriors) and, for layers with nodes sharing
+ * bits, one of SK_RB(KS) words (the slots' dependency masks: dep in bits
+ * 0-15, succ in 16-31, then the hand-over cells).  Messages: per layer and
+ * lane SK_WORDS(KS) words in the workgroup's message block.  Layer
+ * descriptor (PkArgs.pkl): x = positions T | level count << 8 | barrier <<
+ * 16 | kde << 24 (slots 1 .. kde - 1 hold the shared edges), y = byte
+ * offset of the layer's messages in the block, z = byte offset of its
+ * address records in adt, w = of its tail records.  A lane reads and
+ * writes its words with buffer instructions: the layer's offset in a
+ * scalar register, the lane's in a fixed vector register, so no address
+ * arithmetic runs per layer on the vector ALUs.
  */
 #include <hip/hip_runtime.h>
 
@@ -34,7 +39,67 @@ namespace {
 constexpr int KS = SK_KS;
 constexpr int KA = SK_WIDE ? 1 : KS - 1; /* slots k >= KA may be empty */
 constexpr bool NMS = SK_NMS != 0;
-constexpr int WA = SK_WORDS(KS);
+constexpr int WA = SK_WORDS(KS);   /* message words per lane and layer */
+constexpr int RA = SK_RA(KS);      /* address record words */
+constexpr int RB = SK_RB(KS);      /* tail record words */
+constexpr int CW = SK_KD(KS) / 2 + 1; /* hand-over cell words (slots 0 .. SK_KD) */
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+/* buffer resource of a range of device memory (offsets up to 2 GiB) */
+__device__ __forceinline__ rsrc_t mk_rsrc(const void* p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7fffffff, 0x00020000);
+}
+
+/* N consecutive words at byte offset voff (lane) + soff (uniform); the
+ * constant steps go with voff, where they fold into the instructions'
+ * offset field */
+template <int N>
+__device__ __forceinline__ void ld_words(uint32_t* w, rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    int k = 0;
+#pragma unroll
+    for (; k + 4 <= N; k += 4) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 4 * k, soff, 0);
+        w[k] = v[0];
+        w[k + 1] = v[1];
+        w[k + 2] = v[2];
+        w[k + 3] = v[3];
+    }
+    if constexpr (N % 4 >= 2) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff + 4 * (N & ~3), soff, 0);
+        w[N & ~3] = v[0];
+        w[(N & ~3) + 1] = v[1];
+    }
+    if constexpr (N % 2)
+        w[N - 1] = __builtin_amdgcn_raw_buffer_load_b32(r, voff + 4 * (N - 1), soff, 0);
+}
+
+template <int N>
+__device__ __forceinline__ void st_words(const uint32_t* w, rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    int k = 0;
+#pragma unroll
+    for (; k + 4 <= N; k += 4) {
+        typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+        const v4u v = { w[k], w[k + 1], w[k + 2], w[k + 3] };
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, voff + 4 * k, soff, 0);
+    }
+    if constexpr (N % 4 >= 2) {
+        typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+        const v2u v = { w[N & ~3], w[(N & ~3) + 1] };
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, voff + 4 * (N & ~3), soff, 0);
+    }
+    if constexpr (N % 2)
+        __builtin_amdgcn_raw_buffer_store_b32(w[N - 1], r, voff + 4 * (N - 1), soff, 0);
+}
+
+/* a lane's fixed offsets: its address record, tail record and message
+ * words within a layer's */
+struct SkLane {
+    uint32_t va, vb, vm;
+};
 
 /* layer descriptors through the scalar cache: their index is uniform over
  * the workgroup, so a constant-address-space view of the table makes them
@@ -81,15 +146,15 @@ __device__ __forceinline__ uint32_t wg_and2(uint32_t v, unsigned* s_vote, int& n
 
 /* syndrome of both frames, chunks of layers with an early exit once every
  * slot that is still decoding has failed a check; bit k: slot k holds */
-__device__ __forceinline__ uint32_t sk_syndrome(const uint16_t* sm, const uint32_t* adt, const uint4* skl, int t, int q,
-                                                int N, uint32_t live, unsigned* s_vote, int& nvote)
+__device__ __forceinline__ uint32_t sk_syndrome(const uint16_t* lds, rsrc_t rt, const uint4* skl, int t, int q,
+                                                uint32_t va, uint32_t live, unsigned* s_vote, int& nvote)
 {
     const int h = t & 1;
     uint32_t good = 3;
     for (int i0 = 0; i0 < q; i0 += SYN_CHUNK) {
         uint32_t ok = 3;
         const int i1 = min(i0 + SYN_CHUNK, q);
-        uint32_t aw[SYN_CHUNK][WA];
+        uint32_t aw[SYN_CHUNK][KS];
         int Ts[SYN_CHUNK];
 #pragma unroll
         for (int c = 0; c < SYN_CHUNK; c++) {
@@ -97,24 +162,18 @@ __device__ __forceinline__ uint32_t sk_syndrome(const uint16_t* sm, const uint32
                 continue;
             const uint4 ld = sl_desc(skl, i0 + c);
             Ts[c] = sl_T(ld.x);
-            const uint32_t* ar = adt + (size_t)720 * ld.z + t;
-#pragma unroll
-            for (int w = 0; w < WA; w++)
-                aw[c][w] = ar[720 * w];
+            ld_words<KS>(aw[c], rt, va, ld.z);
         }
 #pragma unroll
         for (int c = 0; c < SYN_CHUNK; c++) {
             if (i0 + c >= i1)
                 continue;
-            SkHalf<KS> n;
-            sk_addrs(n, aw[c]);
             uint32_t par, nz;
-            sk_check<KS, KA>(sm, n.ad, Ts[c], h, (i0 + c == 0 && t == 1) ? ~0u : 0u, par, nz);
+            sk_check<KS, KA>(lds, aw[c], Ts[c], h, (i0 + c == 0 && t == 1) ? ~0u : 0u, par, nz);
             par ^= lane_swap(par);
             nz &= lane_swap(nz);
             ok &= sk_check_bits(par, nz);
         }
-        (void)N;
         good &= wg_and2(ok, s_vote, nvote);
         if (!(good & live))
             break;
@@ -127,10 +186,11 @@ __device__ __forceinline__ uint32_t sk_syndrome(const uint16_t* sm, const uint32
  * hand over stops changing or the layer's level count is reached; hb: the
  * two hand-over buffers, hstride u16 cells apart. */
 template <bool TAIL>
-__device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uint16_t* sm, int T, int h,
+__device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uint16_t* lds, int T, int h,
                                          uint32_t skip0, int nlev, int kde, const uint32_t* mw, uint32_t tm,
-                                         const uint32_t* hw, uint32_t* mrow, uint32_t se, uint32_t so, lg_pk ps,
-                                         lg_pk pb, uint16_t* hb, int hstride, unsigned* s_vote, int& nvote)
+                                         const uint32_t* hw, rsrc_t rm, uint32_t vm, uint32_t moff, uint32_t se,
+                                         uint32_t so, lg_pk ps, lg_pk pb, uint16_t* hb, int hstride,
+                                         unsigned* s_vote, int& nvote)
 {
     const uint32_t dep = TAIL ? (tm & 0xffffu) : 0, succ = TAIL ? (tm >> 16) : 0;
     sk_phase_a<KS, KA, TAIL>(n, raw, T, h, skip0, dep, mw, se, so);
@@ -157,43 +217,34 @@ __device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uin
     } else {
         sk_pair(n);
     }
-    sk_phase_c<KS, TAIL, NMS>(n, sm, succ, mrow, 720, ps, pb);
+    uint32_t mo[WA];
+    sk_phase_c<KS, TAIL, NMS>(n, lds, succ, mo, ps, pb);
+    st_words<WA>(mo, rm, vm, moff);
 }
 
-/* one iteration (all layers) on both frames; the addresses, masks and
- * messages of layer i+1 are fetched while layer i runs, the descriptor of
- * layer i+2 too */
-__device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, const uint4* skl, uint32_t* msg, int q,
-                                             int N, int first0, int first1, PkRowPtrs rows, uint16_t* hb, int hstride,
+/* one iteration (all layers) on both frames; the address records, masks
+ * and messages of layer i+1 are fetched while layer i runs, the descriptor
+ * of layer i+2 too */
+__device__ __forceinline__ void sk_iteration(uint16_t* lds, rsrc_t rt, const uint4* skl, rsrc_t rm, SkLane ln, int q,
+                                             int first0, int first1, PkRowPtrs rows, uint16_t* hb, int hstride,
                                              unsigned* s_vote, int& nvote)
 {
     const int t = threadIdx.x, h = t & 1, j = t >> 1;
     const uint32_t se = pk_msg_sel(0, first0, first1), so = pk_msg_sel(1, first0, first1);
     const bool load = !(first0 && first1);
-    uint32_t nx[WA], na[WA], nh[WA], ntm = 0;
+    uint32_t nx[WA], na[KS], nt[1 + CW];
 #pragma unroll
-    for (int w = 0; w < WA; w++) {
+    for (int w = 0; w < WA; w++)
         nx[w] = 0;
-        nh[w] = 0;
-    }
+#pragma unroll
+    for (int w = 0; w < 1 + CW; w++)
+        nt[w] = 0;
     auto fetch = [&](const uint4& ld) {
-        const uint32_t* ar = adt + (size_t)720 * ld.z + t;
-#pragma unroll
-        for (int w = 0; w < WA; w++)
-            na[w] = ar[720 * w];
-        if (load) {
-            const uint32_t* row = msg + (size_t)720 * ld.y + t;
-#pragma unroll
-            for (int w = 0; w < WA; w++)
-                nx[w] = row[720 * w];
-        }
-        if (sl_nlev(ld.x) > 1) {
-            const uint32_t* tr = adt + (size_t)720 * ld.w + t;
-            ntm = tr[0];
-#pragma unroll
-            for (int w = 0; w < WA; w++)
-                nh[w] = tr[720 * (w + 1)];
-        }
+        ld_words<KS>(na, rt, ln.va, ld.z);
+        if (load)
+            ld_words<WA>(nx, rm, ln.vm, ld.y);
+        if (sl_nlev(ld.x) > 1)
+            ld_words<1 + CW>(nt, rt, ln.vb, ld.w);
     };
     uint4 ldn = sl_desc(skl, 0);
     fetch(ldn);
@@ -205,25 +256,25 @@ __device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, 
     __builtin_amdgcn_s_waitcnt(0);
     for (int i = 0; i < q; i++) {
         const uint4 ld = ldn;
-        uint32_t mw[WA], aw[WA], hw[WA];
+        uint32_t mw[WA], tw[1 + CW];
 #pragma unroll
-        for (int w = 0; w < WA; w++) {
+        for (int w = 0; w < WA; w++)
             mw[w] = nx[w];
-            aw[w] = na[w];
-            hw[w] = nh[w];
-        }
-        const uint32_t tm = ntm;
+#pragma unroll
+        for (int w = 0; w < 1 + CW; w++)
+            tw[w] = nt[w];
         SkHalf<KS> n;
-        sk_addrs(n, aw);
-        uint32_t raw[KS];
-        sk_read<KS>(raw, sm, n);
+        sk_addrs(n, na);
+        /* the next layer's fetch is issued first: the posterior reads then
+         * sit in the block that uses them */
         if (i + 1 < q) {
             ldn = ldf;
             fetch(ldn);
             ldf = sl_desc(skl, min(i + 2, q - 1));
         }
+        uint32_t raw[KS];
+        sk_read<KS>(raw, lds, n);
         const int T = sl_T(ld.x), nlev = sl_nlev(ld.x);
-        uint32_t* mrow = msg + (size_t)720 * ld.y + t;
         const uint32_t skip0 = (i == 0 && t == 1) ? ~0u : 0u;
         /* learned scale / offset of the layer's check degree, per frame;
          * node (0, 0) has no previous parity, one edge less */
@@ -233,13 +284,12 @@ __device__ __forceinline__ void sk_iteration(uint16_t* sm, const uint32_t* adt, 
             ps = lg_pk{ (short)rows.S0[dc], (short)rows.S1[dc] };
             pb = lg_pk{ (short)rows.B0[dc], (short)rows.B1[dc] };
         }
-        (void)N;
         if (nlev > 1)
-            sk_layer<true>(n, raw, sm, T, h, skip0, nlev, sl_kde(ld.x), mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote,
-                           nvote);
+            sk_layer<true>(n, raw, lds, T, h, skip0, nlev, sl_kde(ld.x), mw, tw[0], tw + 1, rm, ln.vm, ld.y, se, so, ps,
+                           pb, hb, hstride, s_vote, nvote);
         else
-            sk_layer<false>(n, raw, sm, T, h, skip0, nlev, 0, mw, tm, hw, mrow, se, so, ps, pb, hb, hstride, s_vote,
-                            nvote);
+            sk_layer<false>(n, raw, lds, T, h, skip0, nlev, 0, mw, 0, tw + 1, rm, ln.vm, ld.y, se, so, ps, pb, hb,
+                            hstride, s_vote, nvote);
         /* the next layer waits for this one where it reads what another
          * wave wrote (its barrier flag; the end of the iteration always) */
         if (i + 1 == q || sl_bar(ldn.x))
@@ -261,9 +311,21 @@ template <int KS_, int WIDE_, int NMS_>
 __global__ void __launch_bounds__(SK_NT) ldpc_sk_kernel(PkArgs a)
 {
     static_assert(KS_ == KS && WIDE_ == SK_WIDE && NMS_ == SK_NMS, "instance of this build");
-    extern __shared__ __align__(16) uint16_t sm16[];
-    __shared__ unsigned s_vote[3];
-    __shared__ int s_claim[2], s_seg, s_newseg;
+    /* all of the LDS is dynamic, so it starts at address 0 and the slot
+     * offsets of the address records are LDS addresses as they stand:
+     * control words, then the posteriors at SK_POST_OFF, then the
+     * hand-over buffers at a.hoff */
+    extern __shared__ __align__(16) uint16_t lds16[];
+    unsigned* s_vote = (unsigned*)lds16;           /* [3] */
+    int* s_claim = (int*)lds16 + 3;                /* [2] */
+    int& s_seg = ((int*)lds16)[5];
+    int& s_newseg = ((int*)lds16)[6];
+    uint16_t* const sm16 = lds16 + SK_POST_OFF / 2; /* posteriors */
+    /* the slot offsets are LDS addresses only if the LDS starts at 0 (no
+     * static LDS in this kernel); otherwise decode nothing, and the frames'
+     * results stay unwritten */
+    if (__builtin_amdgcn_readfirstlane((uint32_t)(size_t)lds16) != 0u)
+        return;
     const int tid = threadIdx.x;
     if (tid < 3)
         s_vote[tid] = 3;
@@ -272,8 +334,10 @@ __global__ void __launch_bounds__(SK_NT) ldpc_sk_kernel(PkArgs a)
     int frame[2] = { -1, -1 };    /* frame in each slot, -1: empty */
     int it[2] = { 0, 0 }, maxit[2] = { 0, 0 };
     int phase[2] = { 0, 0 }, lit[2] = { 0, 0 }; /* learned mode: 1 = plain fallback; learned iterations */
-    uint32_t* msg = a.msg + a.msg_stride * (size_t)blockIdx.x;
-    uint16_t* hb = (uint16_t*)((uint8_t*)sm16 + a.hoff);
+    const rsrc_t rt = mk_rsrc(a.adt);
+    const rsrc_t rm = mk_rsrc(a.msg + a.msg_stride * (size_t)blockIdx.x);
+    const SkLane ln = { (uint32_t)(4 * RA * tid), (uint32_t)(4 * RB * tid), (uint32_t)(4 * WA * tid) };
+    uint16_t* hb = (uint16_t*)((uint8_t*)lds16 + a.hoff);
     DCode cd;
 
     for (;;) {
@@ -360,7 +424,7 @@ __global__ void __launch_bounds__(SK_NT) ldpc_sk_kernel(PkArgs a)
         /* ---- iterate until a slot is done ---- */
         for (;;) {
             const uint32_t live = (frame[0] >= 0 ? 1u : 0u) | (frame[1] >= 0 ? 2u : 0u);
-            const uint32_t good = sk_syndrome(sm16, a.adt, skl, tid, q, cd.n, live, s_vote, nvote);
+            const uint32_t good = sk_syndrome(lds16, rt, skl, tid, q, ln.va, live, s_vote, nvote);
             uint32_t done = 0;
             for (int s = 0; s < 2; s++) {
                 const int cap = NMS && phase[s] ? a.fallback : maxit[s];
@@ -432,7 +496,7 @@ __global__ void __launch_bounds__(SK_NT) ldpc_sk_kernel(PkArgs a)
             } else {
                 rows = PkRowPtrs{ nullptr, nullptr, nullptr, nullptr };
             }
-            sk_iteration(sm16, a.adt, skl, msg, q, cd.n, frame[0] < 0 || it[0] == 0, frame[1] < 0 || it[1] == 0, rows, hb,
+            sk_iteration(lds16, rt, skl, rm, ln, q, frame[0] < 0 || it[0] == 0, frame[1] < 0 || it[1] == 0, rows, hb,
                          a.hstride, s_vote, nvote);
             for (int s = 0; s < 2; s++)
                 it[s] += frame[s] >= 0;

@@ -184,16 +184,18 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
     static uint32_t mw[720][8], tm[720], hw[720][8];
     static lg_pk ps[720], pb[720];
     static PkPart part[720];
-    uint32_t* m = msg + (size_t)L[1] * 720;
+    uint32_t* m = msg + L[1];
     for (int t = 0; t < 720; t++) {
-        uint32_t aw[8];
-        for (int w = 0; w < WA; w++) {
-            aw[w] = c->sk_tab[(size_t)(L[2] + w) * 720 + t];
-            mw[t][w] = m[w * 720 + t];
-            hw[t][w] = tail ? c->sk_tab[(size_t)(L[3] + 1 + w) * 720 + t] : 0;
-        }
-        tm[t] = tail ? c->sk_tab[(size_t)L[3] * 720 + t] : 0;
-        sk_addrs(n[t], aw);
+        const uint32_t* ar = c->sk_tab + L[2] + (size_t)t * SK_RA(KS);
+        const uint32_t* tr = c->sk_tab + L[3] + (size_t)t * SK_RB(KS);
+        for (int w = 0; w < 8; w++)
+            hw[t][w] = 0;
+        for (int w = 0; w < WA; w++)
+            mw[t][w] = m[(size_t)t * WA + w];
+        for (int w = 0; tail && w <= SK_KD(KS) / 2; w++)
+            hw[t][w] = tr[1 + w];
+        tm[t] = tail ? tr[0] : 0;
+        sk_addrs(n[t], ar);
         const int j = t >> 1;
         ps[t] = lg_pk{ 16, 16 };
         pb[t] = lg_pk{ 16, 16 };
@@ -206,7 +208,7 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
     for (int t = 719; t >= 0; t--) {
         const uint32_t skip0 = (i == 0 && t == 1) ? ~0u : 0u;
         uint32_t raw[KS];
-        sk_read<KS>(raw, post, n[t]);
+        sk_read<KS>(raw, post - SK_POST_OFF / 2, n[t]);
         if (tail)
             sk_phase_a<KS, KA, true>(n[t], raw, T, t & 1, skip0, tm[t] & 0xffff, mw[t], se, so);
         else
@@ -244,9 +246,9 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
     }
     for (int t = 719; t >= 0; t--) {
         if (tail)
-            sk_phase_c<KS, true, NMS>(n[t], post, tm[t] >> 16, m + t, 720, ps[t], pb[t]);
+            sk_phase_c<KS, true, NMS>(n[t], post - SK_POST_OFF / 2, tm[t] >> 16, m + (size_t)t * WA, ps[t], pb[t]);
         else
-            sk_phase_c<KS, false, NMS>(n[t], post, 0, m + t, 720, ps[t], pb[t]);
+            sk_phase_c<KS, false, NMS>(n[t], post - SK_POST_OFF / 2, 0, m + (size_t)t * WA, ps[t], pb[t]);
     }
 }
 
@@ -281,12 +283,9 @@ static uint32_t sk_model_check(const lg_code* c, int i, const uint16_t* post)
         uint32_t par[2], nz[2];
         for (int h = 0; h < 2; h++) {
             const int t = 2 * j + h;
-            uint32_t aw[8];
-            for (int w = 0; w < SK_WORDS(KS); w++)
-                aw[w] = c->sk_tab[(size_t)(L[2] + w) * 720 + t];
             SkHalf<KS> n;
-            sk_addrs(n, aw);
-            sk_check<KS, KA>(post, n.ad, T, h, (i == 0 && t == 1) ? ~0u : 0u, par[h], nz[h]);
+            sk_addrs(n, c->sk_tab + L[2] + (size_t)t * SK_RA(KS));
+            sk_check<KS, KA>(post - SK_POST_OFF / 2, n.ad, T, h, (i == 0 && t == 1) ? ~0u : 0u, par[h], nz[h]);
         }
         good &= sk_check_bits(par[0] ^ par[1], nz[0] & nz[1]);
     }
@@ -319,11 +318,14 @@ static int pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay
     if (!c)
         return -1;
     const int K = c->k, q = c->q, N = c->n;
-    /* posteriors and the spare cell of empty slots */
-    uint16_t* post = (uint16_t*)calloc((size_t)N + 1, sizeof(uint16_t));
-    uint32_t* msg = (uint32_t*)calloc((size_t)c->sk_msg_rows * 720, sizeof(uint32_t));
-    if (!post || !msg) {
-        free(post);
+    /* the kernel's LDS: control words, then the posteriors (and the spare
+     * cell of empty slots) at SK_POST_OFF, which the address records count
+     * from */
+    uint16_t* lds = (uint16_t*)calloc(SK_POST_OFF / 2 + (size_t)N + 1, sizeof(uint16_t));
+    uint16_t* post = lds ? lds + SK_POST_OFF / 2 : nullptr;
+    uint32_t* msg = (uint32_t*)calloc((size_t)c->sk_msg_words, sizeof(uint32_t));
+    if (!lds || !msg) {
+        free(lds);
         free(msg);
         return -1;
     }
@@ -392,14 +394,14 @@ static int pk_pair(int code, const int8_t* llr_a, const int8_t* llr_b, int delay
         if (active[0] || active[1])
             for (int i = 0; i < q; i++)
                 if (pk_layer_any(c, i, post, msg, se, so, S ? &rows : nullptr)) {
-                    free(post);
+                    free(lds);
                     free(msg);
                     return -1;
                 }
         for (int s = 0; s < 2; s++)
             it[s] += active[s];
     }
-    free(post);
+    free(lds);
     free(msg);
     return 0;
 }
