@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "ldpc_codes.h"
 #include "ldpc_gpu.h"
 
 #define SHIM_LANES 64
@@ -111,4 +112,83 @@ int drx_ldpc_decode(drx_ldpc* d, const int8_t* const* llr, uint8_t* const* out, 
         all &= ok[f];
     }
     return all ? it[0] : -1;
+}
+
+/* ---- drx_fec_lookup and drx_bch_* (libdvbrx/include/dvbrx/fec.h) ---------
+ *
+ * For callers built without libdvbrx's fec.h and bch.c.  Weak, so that
+ * libdvbrx's own definitions win where both are linked. */
+
+__attribute__((weak)) int drx_fec_lookup(int short_frame, const char* rate, drx_fec_params* p)
+{
+    int nldpc, kldpc, kbch, t;
+    if (!rate || !p || ldpc_gpu_fec_lookup(short_frame, rate, &nldpc, &kldpc, &kbch, &t))
+        return -1;
+    p->short_frame = short_frame;
+    p->nldpc = nldpc;
+    p->kldpc = kldpc;
+    p->kbch = kbch;
+    p->t = t;
+    return 0;
+}
+
+struct drx_bch {
+    ldpc_gpu_ctx* ctx;
+    drx_fec_params p;
+    char rate[24];
+    uint8_t* d_cw;
+    int32_t* d_res;
+};
+
+__attribute__((weak)) drx_bch* drx_bch_new(const drx_fec_params* p)
+{
+    if (!p)
+        return NULL;
+    drx_bch* b = calloc(1, sizeof(*b));
+    if (!b)
+        return NULL;
+    b->p = *p;
+    const char* dev = getenv("LDPC_GPU_DEVICE");
+    /* the BCH code of a parameter set (drx_bch_new takes no rate): the first
+     * rate with its frame size, kbch, nbch and t */
+    const char* rate = lg_fec_rate(p->short_frame, p->kbch, p->kldpc, p->t);
+    if (!rate || strlen(rate) >= sizeof(b->rate) || ldpc_gpu_create(dev ? atoi(dev) : 0, &b->ctx)) {
+        free(b);
+        return NULL;
+    }
+    strcpy(b->rate, rate);
+    b->d_cw = ldpc_gpu_malloc(b->ctx, (size_t)(p->kldpc + 7) / 8);
+    b->d_res = ldpc_gpu_malloc(b->ctx, sizeof(int32_t));
+    if (!b->d_cw || !b->d_res) {
+        drx_bch_free(b);
+        return NULL;
+    }
+    return b;
+}
+
+__attribute__((weak)) void drx_bch_free(drx_bch* b)
+{
+    if (!b)
+        return;
+    if (b->ctx) {
+        ldpc_gpu_free(b->ctx, b->d_cw);
+        ldpc_gpu_free(b->ctx, b->d_res);
+        ldpc_gpu_destroy(b->ctx);
+    }
+    free(b);
+}
+
+__attribute__((weak)) int drx_bch_decode(drx_bch* b, uint8_t* cw)
+{
+    if (!b || !cw)
+        return -1;
+    const size_t nb = (size_t)(b->p.kldpc + 7) / 8;
+    int32_t res = -1;
+    if (ldpc_gpu_upload(b->ctx, b->d_cw, cw, nb) ||
+        ldpc_gpu_bch_decode(b->ctx, b->p.short_frame, b->rate, 1, b->d_cw, nb, b->d_res) ||
+        ldpc_gpu_download(b->ctx, &res, b->d_res, sizeof(res)))
+        return -1;
+    if (res > 0 && ldpc_gpu_download(b->ctx, cw, b->d_cw, nb))
+        return -1;
+    return res;
 }

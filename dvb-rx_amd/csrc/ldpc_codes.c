@@ -401,6 +401,15 @@ int lg_fec_lookup(int short_frame, const char* rate, int* kbch, int* nbch, int* 
     return -1;
 }
 
+const char* lg_fec_rate(int short_frame, int kbch, int nbch, int t)
+{
+    for (size_t i = 0; i < sizeof(LG_FEC) / sizeof(*LG_FEC); i++)
+        if (LG_FEC[i].short_frame == (short_frame != 0) && LG_FEC[i].kbch == kbch && LG_FEC[i].nbch == nbch &&
+            LG_FEC[i].t == t)
+            return LG_FEC[i].rate;
+    return NULL;
+}
+
 int lg_fec_index(int short_frame, const char* rate)
 {
     int n = 0;

@@ -127,6 +127,9 @@ int lg_code_lookup(int short_frame, const char* rate);
 /* BCH parameters of drx_fec_lookup (libdvbrx/src/bch.c:25-41); the T2
  * codes "2_3_t2" / "3_5_t2" take those of their DVB-S2 rate. */
 int lg_fec_lookup(int short_frame, const char* rate, int* kbch, int* nbch, int* t);
+/* A rate name of the BCH row with these parameters (the first), NULL if
+ * none. */
+const char* lg_fec_rate(int short_frame, int kbch, int nbch, int t);
 /* Position of a rate among the BCH rows of its frame size, -1 if none. */
 int lg_fec_index(int short_frame, const char* rate);
 

@@ -142,6 +142,27 @@ LG_PK_HD lg_pk pk_sel_mag(lg_pk d, lg_pk dmn, lg_pk mn1)
 #endif
 }
 
+/* pk_sel_mag of two inputs, the instructions of the two interleaved so
+ * that no instruction reads the result of the one just before it (a
+ * packed result is read one instruction later at the earliest) */
+LG_PK_HD void pk_sel_mag2(lg_pk d0, lg_pk d1, lg_pk dmn, lg_pk mn1, lg_pk& r0, lg_pk& r1)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t a, b;
+    asm("v_pk_min_u16 %0, %2, %4\n\t"
+        "v_pk_min_u16 %1, %3, %4\n\t"
+        "v_pk_mad_u16 %0, %0, %5, %6\n\t"
+        "v_pk_mad_u16 %1, %1, %5, %6"
+        : "=&v"(a), "=&v"(b)
+        : "v"(pk_bits(d0)), "v"(pk_bits(d1)), "s"(0x00010001u), "v"(pk_bits(dmn)), "v"(pk_bits(mn1)));
+    r0 = pk_of(a);
+    r1 = pk_of(b);
+#else
+    r0 = pk_sel_mag(d0, dmn, mn1);
+    r1 = pk_sel_mag(d1, dmn, mn1);
+#endif
+}
+
 LG_PK_HD uint16_t pk_post_bytes(lg_pk y) { return (uint16_t)lg_perm(0, pk_bits(y), PK_PACK_POST); }
 
 LG_PK_HD lg_pk pk_msg_clamp(lg_pk o) { return pk_min(pk_max(o, pk_dup(-32 * 256)), pk_dup(31 * 256)); }

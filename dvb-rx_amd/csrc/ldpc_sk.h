@@ -173,12 +173,18 @@ LG_PK_HD PkOut sk_mags(const SkHalf<KS>& n, lg_pk ps, lg_pk pb)
  * taken out).  a >= min0 always, so a - min0 is 0 exactly for the input
  * that is the smallest; inputs of equal magnitude give mn0 == mn1, and so
  * does a 0xff low byte (|x| = 127: both minimums at 127). */
+/* the output of slot k from its magnitude: the sign applied */
+template <int KS>
+LG_PK_HD lg_pk sk_sign(const SkHalf<KS>& n, int k, lg_pk mag)
+{
+    const lg_pk t = n.s ^ n.a[k] ^ n.x[k];
+    return (mag ^ t) - t;
+}
+
 template <int KS>
 LG_PK_HD lg_pk sk_out(const SkHalf<KS>& n, const PkOut& m, int k)
 {
-    const lg_pk mag = pk_sel_mag(n.a[k] - n.min0, m.dmn, m.mn1);
-    const lg_pk t = n.s ^ n.a[k] ^ n.x[k];
-    return (mag ^ t) - t;
+    return sk_sign(n, k, pk_sel_mag(n.a[k] - n.min0, m.dmn, m.mn1));
 }
 
 /* Phase C of a half: final posteriors (TAIL: the slots in succ are written
@@ -188,10 +194,17 @@ template <int KS, bool TAIL, bool NMS>
 LG_PK_HD void sk_phase_c(const SkHalf<KS>& n, uint16_t* post, uint32_t succ, uint32_t* mo, lg_pk ps, lg_pk pb)
 {
     const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
+    /* the slots' magnitudes first, two at a time */
+    lg_pk mag[KS];
+#pragma unroll
+    for (int k = 0; k + 1 < KS; k += 2)
+        pk_sel_mag2(n.a[k] - n.min0, n.a[k + 1] - n.min0, m.dmn, m.mn1, mag[k], mag[k + 1]);
+    if (KS & 1)
+        mag[KS - 1] = pk_sel_mag(n.a[KS - 1] - n.min0, m.dmn, m.mn1);
     uint32_t even = 0;
 #pragma unroll
     for (int k = 0; k < KS; k++) {
-        const lg_pk o = sk_out(n, m, k);
+        const lg_pk o = sk_sign(n, k, mag[k]);
         if (!TAIL || !((succ >> k) & 1))
             sk_st(post, n.ad[k], pk_post_bytes(pk_adds(n.x[k], o)));
         const uint32_t c = pk_bits(pk_msg_clamp(o));

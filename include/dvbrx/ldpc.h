@@ -15,9 +15,20 @@
 extern "C" {
 #endif
 
+/* libdvbrx's ldpc.h includes its fec.h (libdvbrx/include/dvbrx/ldpc.h:9),
+ * and its callers reach drx_fec_lookup and drx_bch_* through it.  Where
+ * that header is on the include path it is used as it is; elsewhere the
+ * same type and functions are declared here, and libldpcgpu.so defines
+ * them (weak symbols: linked beside libdvbrx's bch.c, the reference's
+ * win). */
+#if defined(__has_include)
+#if __has_include("dvbrx/fec.h")
+#include "dvbrx/fec.h"
+#endif
+#endif
+
 #ifndef DVBRX_FEC_H
-/* Same layout as libdvbrx/include/dvbrx/fec.h:13-19 (filled by the caller's
- * drx_fec_lookup, or by ldpc_gpu_fec_lookup). */
+/* Same layout as libdvbrx/include/dvbrx/fec.h:13-19. */
 typedef struct {
     int short_frame; /* 0: normal (64800 bits), 1: short (16200 bits) */
     int nldpc;       /* LDPC codeword length */
@@ -25,6 +36,19 @@ typedef struct {
     int kbch;        /* BBFRAME length */
     int t;           /* BCH error-correcting capability */
 } drx_fec_params;
+
+/* libdvbrx/include/dvbrx/fec.h:25: parameters of a rate ("3_4", "9_10",
+ * "2_9_VLSNR", ...); 0 on success. */
+int drx_fec_lookup(int short_frame, const char* rate, drx_fec_params* p);
+
+/* libdvbrx/include/dvbrx/fec.h:27-37: BCH decoding of one codeword in
+ * place (kldpc bits, MSB first); the corrected bit count, or -1 if
+ * uncorrectable (the codeword is left untouched).  Decoded on the GPU
+ * (ldpc_gpu_bch_decode, one codeword a call). */
+typedef struct drx_bch drx_bch;
+drx_bch* drx_bch_new(const drx_fec_params* p);
+void drx_bch_free(drx_bch* b);
+int drx_bch_decode(drx_bch* b, uint8_t* cw);
 #endif
 
 typedef struct drx_ldpc drx_ldpc;

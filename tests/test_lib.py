@@ -32,7 +32,8 @@ def declared(header):
 
 def exported(path):
     out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
-    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+    # T: defined in the text section; W: weak (the drop-in's FEC functions)
+    return {line.split()[-1] for line in out.splitlines() if " T " in line or " W " in line}
 
 
 @pytest.mark.parametrize("lib", sorted(HEADERS))
@@ -135,3 +136,36 @@ def test_workload_composition():
     assert {m["rate"] for m in mods} == set(workloads.S2_NORMAL)
     for s in a[:20]:
         assert abs(s["esn0_db"] - ldpcgpu.qef_db(s["pls"]) - 0.3) < 1e-9
+
+
+FEC_CASES = [(0, "1_2"), (1, "8_9"), (0, "2_9_VLSNR"), (1, "1_5_VLSNR_SF2"), (0, "2_3_t2"), (0, "7_7")]
+
+
+def test_dropin_header_fec_lookup(tmp_path):
+    """include/dvbrx/ldpc.h on its own declares drx_fec_lookup (and
+    drx_bch_*), libldpcgpu.so defines it: a C caller gets the registry's
+    parameters, and -1 for an unknown rate (no GPU call)."""
+    exe = tmp_path / "fec_header"
+    subprocess.run(["gcc", "-O1", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "c", "fec_header.c"), "-L", os.path.join(ROOT, "dvb-rx_amd", "lib"),
+                    "-lldpcgpu", f"-Wl,-rpath,{os.path.join(ROOT, 'dvb-rx_amd', 'lib')}", "-o", str(exe)], check=True)
+    args = [x for c in FEC_CASES for x in (str(c[0]), c[1])]
+    out = subprocess.run([str(exe)] + args, capture_output=True, text=True, check=True).stdout.split("\n")
+    for (short, rate), line in zip(FEC_CASES, out):
+        f = line.split()
+        if rate == "7_7":
+            assert f[2] == "-1"
+            continue
+        want = ldpcgpu.fec_lookup(short, rate)
+        assert [int(x) for x in f[2:]] == [0, want.nldpc, want.kldpc, want.kbch, want.t], line
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/libdvbrx/include/dvbrx/fec.h"),
+                    reason="libdvbrx's headers are not mounted here")
+def test_dropin_header_with_reference_fec_h(tmp_path):
+    """With libdvbrx's include directory on the path the drop-in header
+    includes the reference's fec.h, so including fec.h after it (as rx.c
+    does) compiles: one drx_fec_params, one set of declarations."""
+    subprocess.run(["gcc", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-DWITH_FEC_H", "-I",
+                    os.path.join(ROOT, "include"), "-I", "/root/reference/libdvbrx/include",
+                    os.path.join(ROOT, "tests", "c", "fec_header.c")], check=True)

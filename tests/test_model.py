@@ -49,8 +49,11 @@ def pk_pair(code, a, b, delay, max_iter, nbits):
     return oa, ob, it, ok
 
 
+# codes with nodes sharing bits (in few or many layers) and without, of
+# every slot count class
 PK_CASES = [(0, "1_2", 0), (0, "1_2", 3), (0, "3_5", 5), (0, "9_10", 2), (1, "1_4", 4), (1, "8_9", 1),
-            (0, "2_9_VLSNR", 2), (1, "11_45", 3), (0, "135_180", 1), (0, "2_3_t2", 2), (1, "3_5_t2", 0)]
+            (0, "2_9_VLSNR", 2), (1, "11_45", 3), (0, "135_180", 1), (0, "2_3_t2", 2), (1, "3_5_t2", 0),
+            (0, "1_4", 1), (0, "1_3", 0), (0, "2_5", 2), (0, "13_45", 1)]
 
 
 @pytest.mark.parametrize("short,rate,delay", PK_CASES)
