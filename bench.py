@@ -213,10 +213,11 @@ def cpu_baseline(keys, llr_of, kl_of, max_iter, seconds=8.0, what="the workload"
     }
 
 
-def sk_name(di):
+def sk_name(di, occ=1):
     """Kernel name of a split kernel instance from its degree code
-    (2 * slots per lane - 2 + 100 * wide), plain arithmetic."""
-    return f"ldpc_sk_kernel<{(di % 100 + 2) // 2}, {di // 100}, 0>"
+    (2 * slots per lane - 2 + 100 * wide), plain arithmetic, built for occ
+    workgroups per CU."""
+    return f"ldpc_sk_kernel<{(di % 100 + 2) // 2}, {di // 100}, 0, {occ}>"
 
 
 def copy_calibration(mib):
@@ -786,13 +787,14 @@ def main():
     dt = time.perf_counter() - t
     # per-launch durations with the launches one after another, each kernel
     # alone on the GPU (the rooflines use these), after the timed steps
-    serial_ms = {}
+    serial_ms, serial_occ = {}, {}
     dec.set_concurrent(False)
     for r in range(2):
         dec.decode_batch(batch.frames, d_llr, d_out, d_it, d_ok)
         dec.synchronize()
         if r:
             serial_ms = {(di, nf): ms for di, nf, ms in dec.last_launches()}
+            serial_occ = {(di, nf): sh[0] for (di, nf, _), sh in zip(dec.last_launches(), dec.last_launch_shapes())}
             serial_decode_ms = dec.last_kernel_ms()
     dec.set_concurrent(not args.serial)
     out = np.zeros_like(batch.ref)
@@ -869,12 +871,13 @@ def main():
                 sel = m
         if size is None:
             continue
-        launch_order.append(f"{sk_name(di)} {size} x{nf}")
+        name = sk_name(di, serial_occ[(di, nf)])
+        launch_order.append(f"{name} {size} x{nf}")
         e = inst.setdefault(di, [])
-        e.append(dict(size=size, ms=ms, frames=nf, hbm=int(hbm_f[sel].sum()), lds=int(lds_f[sel].sum())))
+        e.append(dict(kernel=name, size=size, ms=ms, frames=nf, hbm=int(hbm_f[sel].sum()), lds=int(lds_f[sel].sum())))
     dom_di = max(inst, key=lambda d: sum(x["ms"] for x in inst[d]))
     dl = inst[dom_di]
-    kname = sk_name(dom_di)
+    kname = " / ".join(dict.fromkeys(x["kernel"] for x in dl))
     dom_ms = float(np.mean([x["ms"] for x in dl]))
     hbm_bytes = float(np.mean([x["hbm"] for x in dl]))
     lds_bytes = float(np.mean([x["lds"] for x in dl]))
@@ -891,8 +894,8 @@ def main():
     if os.path.exists(tpath) and args.frames == 8192:
         try:
             tl = json.load(open(tpath))["launches"]
-            traffic = round(float(np.mean([tl[f"{kname} {x['size']}"]["hbm_bytes"] for x in dl])))
-            valu = float(np.mean([tl[f"{kname} {x['size']}"]["valu_insts"] for x in dl]))
+            traffic = round(float(np.mean([tl[f"{x['kernel']} {x['size']}"]["hbm_bytes"] for x in dl])))
+            valu = float(np.mean([tl[f"{x['kernel']} {x['size']}"]["valu_insts"] for x in dl]))
         except (OSError, ValueError, KeyError):
             traffic, valu = None, None
     hbm_rec = {"achieved": round(hbm_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_ach / HBM_PEAK_GBS, 5),

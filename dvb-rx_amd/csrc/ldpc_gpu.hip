@@ -84,6 +84,14 @@ DeviceTables g_tables[64];
 
 } // namespace
 
+/* Shape of one persistent launch: workgroups per CU its kernel instance is
+ * built for (1, or 2 where sk_has_occ2) and the share of the resident
+ * workgroups it launches (0 < scale <= 1; 0 reads as 1). */
+struct PkShape {
+    int occ;
+    float scale;
+};
+
 struct ldpc_gpu_ctx {
     int device;
     hipStream_t stream;
@@ -111,9 +119,12 @@ struct ldpc_gpu_ctx {
     hipEvent_t ev_cls[2 * NLAUNCH];
     int cls_frames[NLAUNCH]; /* frames of each launch of the last decode */
     int cls_di[NLAUNCH];     /* its kernel instance's layer degree bound */
+    int cls_occ[NLAUNCH];    /* its workgroups per CU and grid */
+    int cls_grid[NLAUNCH];
     int timed;
     int force_scalar;        /* per-frame batches through the one-frame kernel */
     int serial;              /* launches of a decode one after the other on the context's stream */
+    PkShape shape[NLAUNCH];  /* launch shape per (instance, size); zero: one workgroup per CU, full grid */
     /* learned tables: row 0 plain (S = 16, B = 1), then per code its rows */
     uint8_t* h_nS;
     int16_t* h_nB;
@@ -150,6 +161,10 @@ struct ldpc_gpu_plan {
     int variant;
     float variant_ms[8];
     int launched;
+    PkShape shape[NLAUNCH]; /* the launch shapes the graph was built with */
+    bool concurrent;
+    int nfr[NLAUNCH], grid[NLAUNCH], occ[NLAUNCH];
+    float tuned_ms; /* ms per replay of the tuned shape, < 0 before ldpc_gpu_plan_tune */
 };
 
 static int set_err(ldpc_gpu_ctx* c, int code, const char* fmt, ...)
@@ -295,6 +310,11 @@ int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
     }
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         c->ncu = 256;
+    /* short frames on the instances built for it: two workgroups per CU
+     * (18-23 % less time per frame-iteration once a launch fills the CUs,
+     * the same below that) */
+    for (int k = 0; k < SK_NINST; k++)
+        c->shape[2 * k + 1] = PkShape{ sk_has_occ2(k, false) ? 2 : 1, 1.0f };
     if ((rc = upload_tables(c)) != LDPC_GPU_OK || (rc = static_init(c)) != LDPC_GPU_OK ||
         hipMalloc((void**)&c->ticket, NCOUNTERS * sizeof(unsigned)) != hipSuccess) {
         ldpc_gpu_destroy(c);
@@ -471,6 +491,27 @@ int ldpc_gpu_set_kernel(ldpc_gpu_ctx* c, int kernel)
     return LDPC_GPU_OK;
 }
 
+/* the layer degree bound that names kernel instance k (as
+ * ldpc_gpu_last_launches reports it) */
+static int sk_di(int k) { return 2 * SK_INST[k][0] - 2 + 100 * SK_INST[k][1]; }
+
+int ldpc_gpu_set_launch_shape(ldpc_gpu_ctx* c, int di, int short_frames, int occ, float scale)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    if ((short_frames != 0 && short_frames != 1) || (occ != 1 && occ != 2) || !(scale > 0.0f && scale <= 1.0f))
+        return set_err(c, LDPC_GPU_EINVAL, "launch shape: bad arguments");
+    for (int k = 0; k < SK_NINST; k++) {
+        if (sk_di(k) != di)
+            continue;
+        if (occ == 2 && !sk_has_occ2(k, false))
+            return set_err(c, LDPC_GPU_EINVAL, "launch shape: kernel instance %d has no two-workgroup build", di);
+        c->shape[2 * k + short_frames] = PkShape{ occ, scale };
+        return LDPC_GPU_OK;
+    }
+    return set_err(c, LDPC_GPU_EINVAL, "launch shape: no kernel instance %d", di);
+}
+
 int ldpc_gpu_last_launches(ldpc_gpu_ctx* c, int* di, int* nframes, float* ms, int max)
 {
     if (!c || !c->timed)
@@ -489,6 +530,23 @@ int ldpc_gpu_last_launches(ldpc_gpu_ctx* c, int* di, int* nframes, float* ms, in
             nframes[n] = c->cls_frames[k];
         if (ms)
             ms[n] = t;
+        n++;
+    }
+    return n;
+}
+
+int ldpc_gpu_last_launch_shapes(ldpc_gpu_ctx* c, int* occ, int* grid, int max)
+{
+    if (!c || !c->timed)
+        return 0;
+    int n = 0;
+    for (int k = 0; k < NLAUNCH && n < max; k++) {
+        if (!c->cls_frames[k])
+            continue;
+        if (occ)
+            occ[n] = c->cls_occ[k] ? c->cls_occ[k] : 1;
+        if (grid)
+            grid[n] = c->cls_grid[k];
         n++;
     }
     return n;
@@ -513,19 +571,44 @@ namespace lgk {
 #define SK_CASE_O(KS, W, N)                                                                       \
     if (SK_INST[k][0] == KS && SK_INST[k][1] == W && nms == (N != 0))                                 \
         return sk_occupancy_##KS##_##W##_##N(lds, per_cu);
+#define SK_CASE_L2(KS, W, N)                                                                      \
+    if (SK_INST[k][0] == KS && SK_INST[k][1] == W && nms == (N != 0))                                 \
+        return sk_launch_##KS##_##W##_##N##_o2(grid, lds, st, a);
+#define SK_CASE_O2(KS, W, N)                                                                      \
+    if (SK_INST[k][0] == KS && SK_INST[k][1] == W && nms == (N != 0))                                 \
+        return sk_occupancy_##KS##_##W##_##N##_o2(lds, per_cu);
+#define SK_CASE_H2(KS, W, N)                                                                      \
+    if (SK_INST[k][0] == KS && SK_INST[k][1] == W && nms == (N != 0))                                 \
+        return true;
 
-hipError_t pk_launch(int k, bool nms, int grid, size_t lds, hipStream_t st, const PkArgs& a)
+bool sk_has_occ2(int k, bool nms)
 {
     if (k >= 0 && k < SK_NINST) {
-        SK_ALL(SK_CASE_L)
+        SK_ALL2(SK_CASE_H2)
+    }
+    return false;
+}
+
+hipError_t pk_launch(int k, bool nms, int occ, int grid, size_t lds, hipStream_t st, const PkArgs& a)
+{
+    if (k >= 0 && k < SK_NINST) {
+        if (occ == 2) {
+            SK_ALL2(SK_CASE_L2)
+        } else {
+            SK_ALL(SK_CASE_L)
+        }
     }
     return hipErrorInvalidValue;
 }
 
-hipError_t pk_occupancy(int k, bool nms, size_t lds, int* per_cu)
+hipError_t pk_occupancy(int k, bool nms, int occ, size_t lds, int* per_cu)
 {
     if (k >= 0 && k < SK_NINST) {
-        SK_ALL(SK_CASE_O)
+        if (occ == 2) {
+            SK_ALL2(SK_CASE_O2)
+        } else {
+            SK_ALL(SK_CASE_O)
+        }
     }
     return hipErrorInvalidValue;
 }
@@ -644,6 +727,8 @@ static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFr
     for (int k = 0; k < NCLASS; k++) {
         c->cls_frames[k] = (int)lists[k].size();
         c->cls_di[k] = CLASS_DI[k];
+        c->cls_occ[k] = 1;
+        c->cls_grid[k] = grid[k];
         if (lists[k].empty())
             continue;
         a.list = c->d_list + off[k];
@@ -661,10 +746,10 @@ static int launch(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<DFr
     return LDPC_GPU_OK;
 }
 
-static int pk_grid(ldpc_gpu_ctx* c, int inst, bool nms, size_t lds, int nframes, int* grid)
+static int pk_grid(ldpc_gpu_ctx* c, int inst, bool nms, int occ, size_t lds, int nframes, int* grid)
 {
     int per_cu = 0;
-    HIPCHK(c, pk_occupancy(inst, nms, lds, &per_cu));
+    HIPCHK(c, pk_occupancy(inst, nms, occ, lds, &per_cu));
     if (per_cu < 1)
         return set_err(c, LDPC_GPU_EDEVICE, "two-slot kernel does not fit (LDS %zu)", lds);
     *grid = std::min((nframes + 1) / 2, per_cu * c->ncu);
@@ -672,13 +757,13 @@ static int pk_grid(ldpc_gpu_ctx* c, int inst, bool nms, size_t lds, int nframes,
 }
 
 /* Launch layout of frames with their own early stop (two per workgroup,
- * ldpc_pk_kernel): one launch per (degree class, frame size), frames
+ * ldpc_sk_kernel): one launch per (kernel instance, frame size), frames
  * grouped by code into segments of one list, persistent grid per launch
- * (occupancy x CUs, scaled by grid_scale). */
+ * (occupancy x CUs, scaled by the launch's shape). */
 struct PkLayout {
     std::vector<int> list;
     std::vector<int2> segs;
-    int seg0[NLAUNCH], nseg[NLAUNCH], nfr[NLAUNCH], grid[NLAUNCH], maxn[NLAUNCH], maxx[NLAUNCH];
+    int seg0[NLAUNCH], nseg[NLAUNCH], nfr[NLAUNCH], grid[NLAUNCH], maxn[NLAUNCH], maxx[NLAUNCH], occ[NLAUNCH];
     int maxw;
     size_t blocks; /* message blocks: sum of the grids */
 };
@@ -689,7 +774,12 @@ struct PkLayout {
 static size_t pk_hoff(const PkLayout& L, int k) { return (SK_POST_OFF + 2 * ((size_t)L.maxn[k] + 1) + 15) & ~(size_t)15; }
 static size_t pk_lds(const PkLayout& L, int k) { return pk_hoff(L, k) + 2 * sizeof(uint16_t) * 360 * (size_t)L.maxx[k]; }
 
-static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, float grid_scale, PkLayout& L)
+/* the workgroups per CU launch k runs with under shape sh: 2 only where
+ * that instance is built for it */
+static int pk_occ_of(int k, bool nms, const PkShape& sh) { return sh.occ == 2 && sk_has_occ2(k / 2, nms) ? 2 : 1; }
+static float pk_scale_of(const PkShape& sh) { return sh.scale > 0.0f ? std::min(sh.scale, 1.0f) : 1.0f; }
+
+static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, const PkShape* shape, PkLayout& L)
 {
     const std::vector<DCode>& hc = c->tab->h_codes;
     std::vector<int> gcodes[NLAUNCH];
@@ -719,6 +809,7 @@ static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, f
         L.nseg[k] = (int)gseg[k].size();
         L.nfr[k] = 0;
         L.grid[k] = 0;
+        L.occ[k] = pk_occ_of(k, nms, shape[k]);
         for (auto& sg : gseg[k]) {
             L.segs.push_back(make_int2((int)L.list.size(), (int)sg.size()));
             L.list.insert(L.list.end(), sg.begin(), sg.end());
@@ -727,10 +818,10 @@ static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, f
         if (!L.nfr[k])
             continue;
         const size_t lds = pk_lds(L, k);
-        const int rc = pk_grid(c, k / 2, nms, lds, L.nfr[k], &L.grid[k]);
+        const int rc = pk_grid(c, k / 2, nms, L.occ[k], lds, L.nfr[k], &L.grid[k]);
         if (rc)
             return rc;
-        L.grid[k] = std::max(1, (int)((float)L.grid[k] * grid_scale));
+        L.grid[k] = std::max(1, (int)((float)L.grid[k] * pk_scale_of(shape[k])));
         L.blocks += (size_t)L.grid[k];
     }
     if (L.segs.size() > NCOUNTERS)
@@ -770,7 +861,7 @@ static int pk_enqueue(ldpc_gpu_ctx* c, const PkLayout& L, PkArgs a, const int2* 
             HIPCHK(c, hipStreamWaitEvent(st, fork, 0));
         if (tev)
             HIPCHK(c, hipEventRecord(tev[2 * k], st));
-        HIPCHK(c, pk_launch(k / 2, nms, L.grid[k], lds, st, a));
+        HIPCHK(c, pk_launch(k / 2, nms, L.occ[k], L.grid[k], lds, st, a));
         if (tev)
             HIPCHK(c, hipEventRecord(tev[2 * k + 1], st));
         if (concurrent) {
@@ -815,7 +906,7 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
                      uint8_t* d_fell = nullptr)
 {
     PkLayout L;
-    int rc = pk_layout(c, hf, nms, 1.0f, L);
+    int rc = pk_layout(c, hf, nms, c->shape, L);
     if (rc)
         return rc;
     if ((rc = grow(c, &c->d_list, &c->list_bytes, sizeof(int) * L.list.size())) ||
@@ -829,7 +920,9 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
     for (int k = 0; k < NLAUNCH; k++) {
         join[k] = c->ev_cls[2 * k + 1];
         c->cls_frames[k] = L.nfr[k];
-        c->cls_di[k] = 2 * SK_INST[k / 2][0] - 2 + 100 * SK_INST[k / 2][1];
+        c->cls_di[k] = sk_di(k / 2);
+        c->cls_occ[k] = L.occ[k];
+        c->cls_grid[k] = L.grid[k];
     }
     if ((rc = pk_enqueue(c, L, a, c->d_seg, c->ticket, c->msg, c->stream, c->aux, c->ev0, join, c->ev_cls, !c->serial,
                          nms)))
@@ -1187,36 +1280,51 @@ extern "C" int ldpc_gpu_decode_batch_static(ldpc_gpu_ctx* c, const ldpc_gpu_fram
         L.maxx[k] = maxx[k];
         const size_t lds = pk_lds(L, k);
         int grid = 0;
-        int rc = pk_grid(c, k / 2, false, lds, nfr[k], &grid);
+        const int occ = pk_occ_of(k, false, c->shape[k]);
+        int rc = pk_grid(c, k / 2, false, occ, lds, nfr[k], &grid);
         if (rc)
             return rc;
+        grid = std::max(1, (int)((float)grid * pk_scale_of(c->shape[k])));
         grid = (int)std::min<size_t>((size_t)grid, msg_blocks);
         a.seg = d_seg + seg0[k];
         a.nseg = nsegk[k];
         a.cnt = d_cnt + seg0[k];
         a.hoff = pk_hoff(L, k);
         a.hstride = 360 * maxx[k];
-        HIPCHK(c, pk_launch(k / 2, false, grid, lds, c->stream, a));
+        HIPCHK(c, pk_launch(k / 2, false, occ, grid, lds, c->stream, a));
     }
     return LDPC_GPU_OK;
 }
 
 /* ---- prepared batches: graph capture and the launch-shape autotuner ---- */
 
-/* launch shapes the autotuner chooses from: grid scale (persistent
- * workgroups per launch relative to occupancy x CUs, i.e. how many frames
- * run at once) and whether the (class, size) launches run concurrently */
+/* launch shapes the autotuner starts from, each for all launches of the
+ * plan: grid scale (persistent workgroups per launch relative to occupancy
+ * x CUs, i.e. how many frames run at once) and whether the (instance,
+ * size) launches run concurrently; the context's workgroups per CU */
 static const struct {
     float scale;
     bool concurrent;
 } PLAN_VARIANTS[] = { { 1.0f, true }, { 1.0f, false }, { 0.75f, true }, { 0.5f, true }, { 0.5f, false } };
 #define PLAN_NVARIANTS ((int)(sizeof(PLAN_VARIANTS) / sizeof(PLAN_VARIANTS[0])))
 
+static void plan_variant_shape(ldpc_gpu_plan* p, int variant)
+{
+    for (int k = 0; k < NLAUNCH; k++) {
+        p->shape[k].occ = p->c->shape[k].occ;
+        p->shape[k].scale = PLAN_VARIANTS[variant].scale;
+    }
+    p->concurrent = PLAN_VARIANTS[variant].concurrent;
+}
+
+/* (re)build the plan's graph with its current shape (p->shape,
+ * p->concurrent); variant: the start variant it came from, or -1 once the
+ * per-launch search changed it */
 static int plan_build(ldpc_gpu_plan* p, int variant)
 {
     ldpc_gpu_ctx* c = p->c;
     PkLayout L;
-    int rc = pk_layout(c, p->hf, false, PLAN_VARIANTS[variant].scale, L);
+    int rc = pk_layout(c, p->hf, false, p->shape, L);
     if (rc)
         return rc;
     if (L.list.size() > p->list_cap || L.segs.size() > p->seg_cap || L.blocks * L.maxw > p->msg_cap)
@@ -1234,7 +1342,7 @@ static int plan_build(ldpc_gpu_plan* p, int variant)
     const PkArgs a = pk_args(c, p->d_frames, p->d_list, p->d_llr, p->d_out, p->d_iters, p->d_synd, false, 0, nullptr);
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     rc = pk_enqueue(c, L, a, p->d_seg, p->d_cnt, p->d_msg, c->stream, p->aux, p->fork, p->join, nullptr,
-                    PLAN_VARIANTS[variant].concurrent, false);
+                    p->concurrent, false);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc || e != hipSuccess) {
@@ -1245,6 +1353,11 @@ static int plan_build(ldpc_gpu_plan* p, int variant)
     p->graph = g;
     HIPCHK(c, hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
     p->variant = variant;
+    for (int k = 0; k < NLAUNCH; k++) {
+        p->nfr[k] = L.nfr[k];
+        p->grid[k] = L.grid[k];
+        p->occ[k] = L.occ[k];
+    }
     return LDPC_GPU_OK;
 }
 
@@ -1306,11 +1419,16 @@ extern "C" int ldpc_gpu_plan_create(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frame
     p->d_iters = d_iters;
     p->d_synd = d_synd;
     p->variant = -1;
+    p->tuned_ms = -1.0f;
     for (int v = 0; v < 8; v++)
         p->variant_ms[v] = -1.0f;
-    /* scratch sized for the largest variant (full grid) */
+    /* scratch sized for the largest shape (full grid, two workgroups per
+     * CU where built) */
+    PkShape big[NLAUNCH];
+    for (int k = 0; k < NLAUNCH; k++)
+        big[k] = PkShape{ 2, 1.0f };
     PkLayout L;
-    if ((rc = pk_layout(c, df, false, 1.0f, L))) {
+    if ((rc = pk_layout(c, df, false, big, L))) {
         delete p;
         return rc;
     }
@@ -1338,6 +1456,7 @@ extern "C" int ldpc_gpu_plan_create(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frame
             return fail(LDPC_GPU_EDEVICE);
     if (hipMemcpy(p->d_frames, df.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice) != hipSuccess)
         return fail(LDPC_GPU_EDEVICE);
+    plan_variant_shape(p, 0);
     if ((rc = plan_build(p, 0)))
         return fail(rc);
     *out = p;
@@ -1367,32 +1486,89 @@ extern "C" float ldpc_gpu_plan_last_ms(ldpc_gpu_plan* p)
     return ms;
 }
 
+/* mean ms of reps replays of the plan's graph (after one untimed) */
+static int plan_time(ldpc_gpu_plan* p, int reps, float* ms)
+{
+    ldpc_gpu_ctx* c = p->c;
+    int rc;
+    if ((rc = ldpc_gpu_plan_launch(p)))
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float tot = 0.0f;
+    for (int r = 0; r < reps; r++) {
+        if ((rc = ldpc_gpu_plan_launch(p)))
+            return rc;
+        const float t = ldpc_gpu_plan_last_ms(p);
+        if (t < 0.0f)
+            return set_err(c, LDPC_GPU_EDEVICE, "plan timing failed");
+        tot += t;
+    }
+    *ms = tot / (float)reps;
+    return LDPC_GPU_OK;
+}
+
 extern "C" int ldpc_gpu_plan_tune(ldpc_gpu_plan* p, int reps)
 {
     if (!p || reps < 1)
         return LDPC_GPU_EINVAL;
     ldpc_gpu_ctx* c = p->c;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     int best = 0, rc;
+    /* the shapes common to all launches */
     for (int v = 0; v < PLAN_NVARIANTS; v++) {
-        if ((rc = plan_build(p, v)))
+        plan_variant_shape(p, v);
+        if ((rc = plan_build(p, v)) || (rc = plan_time(p, reps, &p->variant_ms[v])))
             return rc;
-        if ((rc = ldpc_gpu_plan_launch(p)))
-            return rc;
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        float tot = 0.0f;
-        for (int r = 0; r < reps; r++) {
-            if ((rc = ldpc_gpu_plan_launch(p)))
-                return rc;
-            const float ms = ldpc_gpu_plan_last_ms(p);
-            if (ms < 0.0f)
-                return set_err(c, LDPC_GPU_EDEVICE, "plan timing failed");
-            tot += ms;
-        }
-        p->variant_ms[v] = tot / (float)reps;
         if (p->variant_ms[v] < p->variant_ms[best])
             best = v;
     }
-    return plan_build(p, best);
+    plan_variant_shape(p, best);
+    if ((rc = plan_build(p, best)))
+        return rc;
+    /* then one pass of coordinate descent over the launches, longest
+     * first: each launch's workgroups per CU and grid scale with the
+     * others held, kept where the whole plan gets faster by more than
+     * 0.5 % (timing noise) */
+    float cur = p->variant_ms[best];
+    int order[NLAUNCH], nl = 0;
+    for (int k = 0; k < NLAUNCH; k++)
+        if (p->nfr[k])
+            order[nl++] = k;
+    /* bits decoded by the launch: a normal frame is four short ones (odd
+     * launches hold short frames) */
+    auto work = [&](int k) { return (size_t)p->nfr[k] * (k & 1 ? 1 : 4); };
+    std::stable_sort(order, order + nl, [&](int x, int y) { return work(x) > work(y); });
+    static const float SCALES[] = { 1.0f, 0.75f, 0.5f };
+    for (int i = 0; i < nl; i++) {
+        const int k = order[i];
+        const PkShape keep = p->shape[k];
+        PkShape pick = keep;
+        for (int occ = 1; occ <= (sk_has_occ2(k / 2, false) ? 2 : 1); occ++)
+            for (float sc : SCALES) {
+                const PkShape t = PkShape{ occ, sc };
+                if (pk_occ_of(k, false, t) == pk_occ_of(k, false, keep) && pk_scale_of(t) == pk_scale_of(keep))
+                    continue;
+                p->shape[k] = t;
+                float ms = 0.0f;
+                if ((rc = plan_build(p, -1)) || (rc = plan_time(p, reps, &ms)))
+                    return rc;
+                if (ms < cur * 0.995f) {
+                    cur = ms;
+                    pick = t;
+                }
+            }
+        p->shape[k] = pick;
+    }
+    p->tuned_ms = cur;
+    const bool changed = [&] {
+        for (int k = 0; k < NLAUNCH; k++)
+            if (p->nfr[k] && (pk_occ_of(k, false, p->shape[k]) != pk_occ_of(k, false, c->shape[k]) ||
+                              pk_scale_of(p->shape[k]) != PLAN_VARIANTS[best].scale))
+                return true;
+        return false;
+    }();
+    return plan_build(p, changed ? -1 : best);
 }
 
 extern "C" int ldpc_gpu_plan_select(ldpc_gpu_plan* p, int variant)
@@ -1403,6 +1579,7 @@ extern "C" int ldpc_gpu_plan_select(ldpc_gpu_plan* p, int variant)
         return set_err(p->c, LDPC_GPU_EINVAL, "plan variant %d out of range", variant);
     HIPCHK(p->c, hipSetDevice(p->c->device));
     HIPCHK(p->c, hipStreamSynchronize(p->c->stream));
+    plan_variant_shape(p, variant);
     return plan_build(p, variant);
 }
 
@@ -1421,6 +1598,32 @@ extern "C" int ldpc_gpu_plan_info(ldpc_gpu_plan* p, int* variant, float* scale, 
             ms[v] = p->variant_ms[v];
     }
     return PLAN_NVARIANTS;
+}
+
+extern "C" int ldpc_gpu_plan_shape(ldpc_gpu_plan* p, int* di, int* short_frames, int* occ, float* scale, int* grid,
+                                   int* frames, int max)
+{
+    if (!p)
+        return LDPC_GPU_EINVAL;
+    int n = 0;
+    for (int k = 0; k < NLAUNCH && n < max; k++) {
+        if (!p->nfr[k])
+            continue;
+        if (di)
+            di[n] = sk_di(k / 2);
+        if (short_frames)
+            short_frames[n] = k & 1;
+        if (occ)
+            occ[n] = p->occ[k];
+        if (scale)
+            scale[n] = pk_scale_of(p->shape[k]);
+        if (grid)
+            grid[n] = p->grid[k];
+        if (frames)
+            frames[n] = p->nfr[k];
+        n++;
+    }
+    return n;
 }
 
 extern "C" int ldpc_gpu_decode_host(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, const int8_t* llr,

@@ -136,9 +136,12 @@ struct PkRowPtrs {
 hipError_t layered_launch(int cls, int grid, size_t lds, hipStream_t st, const KArgs& a);
 hipError_t layered_occupancy(int cls, size_t lds, int* per_cu);
 hipError_t reset_counters(hipStream_t st, unsigned* cnt);
-/* the split kernel instance k (SK_INSTS[k]), plain or learned */
-hipError_t pk_launch(int k, bool nms, int grid, size_t lds, hipStream_t st, const PkArgs& a);
-hipError_t pk_occupancy(int k, bool nms, size_t lds, int* per_cu);
+/* the split kernel instance k (SK_INSTS[k]), plain or learned, built for
+ * occ (1 or 2) workgroups per CU; occ 2 exists for the plain instances
+ * SK_ALL2 lists (sk_has_occ2) */
+hipError_t pk_launch(int k, bool nms, int occ, int grid, size_t lds, hipStream_t st, const PkArgs& a);
+hipError_t pk_occupancy(int k, bool nms, int occ, size_t lds, int* per_cu);
+bool sk_has_occ2(int k, bool nms);
 #define SK_DECL(KS, W, N)                                                                          \
     hipError_t sk_launch_##KS##_##W##_##N(int grid, size_t lds, hipStream_t st, const PkArgs& a);   \
     hipError_t sk_occupancy_##KS##_##W##_##N(size_t lds, int* per_cu);
@@ -148,6 +151,13 @@ hipError_t pk_occupancy(int k, bool nms, size_t lds, int* per_cu);
     X(5, 0, 1) X(6, 0, 1) X(7, 0, 1) X(7, 1, 1) X(8, 0, 1) X(9, 0, 1) X(10, 0, 1) X(10, 1, 1) X(11, 0, 1)         \
     X(14, 0, 1) X(15, 0, 1)
 SK_ALL(SK_DECL)
+/* the instances also built for two workgroups per CU (short frames; their
+ * registers fit 6 waves per SIMD without spilling) */
+#define SK_DECL2(KS, W, N)                                                                         \
+    hipError_t sk_launch_##KS##_##W##_##N##_o2(int grid, size_t lds, hipStream_t st, const PkArgs& a); \
+    hipError_t sk_occupancy_##KS##_##W##_##N##_o2(size_t lds, int* per_cu);
+#define SK_ALL2(X) X(2, 0, 0) X(3, 0, 0) X(4, 0, 0) X(4, 1, 0)
+SK_ALL2(SK_DECL2)
 
 } // namespace lgk
 

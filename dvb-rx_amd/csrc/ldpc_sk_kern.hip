@@ -29,7 +29,12 @@
 #include "ldpc_sk.h"
 
 #if !defined(SK_KS) || !defined(SK_WIDE) || !defined(SK_NMS)
-#error "build with -DSK_KS=2..15 -DSK_WIDE=0|1 -DSK_NMS=0|1"
+#error "build with -DSK_KS=2..15 -DSK_WIDE=0|1 -DSK_NMS=0|1 [-DSK_OCC=2]"
+#endif
+/* workgroups per CU the instance is built for: 1, or 2 (short frames:
+ * 6 waves per SIMD, registers bounded to fit two workgroups) */
+#ifndef SK_OCC
+#define SK_OCC 1
 #endif
 
 namespace lgk {
@@ -306,11 +311,14 @@ __device__ __forceinline__ void sk_iteration(uint16_t* lds, rsrc_t rt, const uin
  * workgroup moves to another segment with frames left, or exits.
  */
 /* the template arguments name the instance (slots per lane, wide,
- * learned) in profiles; the body uses this file's build constants */
-template <int KS_, int WIDE_, int NMS_>
-__global__ void __launch_bounds__(SK_NT) ldpc_sk_kernel(PkArgs a)
+ * learned, workgroups per CU) in profiles; the body uses this file's build
+ * constants.  The launch bound's second argument is the waves per SIMD the
+ * registers must allow: 720 threads are 12 waves, 3 per SIMD per
+ * workgroup. */
+template <int KS_, int WIDE_, int NMS_, int OCC_>
+__global__ void __launch_bounds__(SK_NT, 3 * SK_OCC) ldpc_sk_kernel(PkArgs a)
 {
-    static_assert(KS_ == KS && WIDE_ == SK_WIDE && NMS_ == SK_NMS, "instance of this build");
+    static_assert(KS_ == KS && WIDE_ == SK_WIDE && NMS_ == SK_NMS && OCC_ == SK_OCC, "instance of this build");
     /* all of the LDS is dynamic, so it starts at address 0 and the slot
      * offsets of the address records are LDS addresses as they stand:
      * control words, then the posteriors at SK_POST_OFF, then the
@@ -507,18 +515,24 @@ __global__ void __launch_bounds__(SK_NT) ldpc_sk_kernel(PkArgs a)
 
 } // namespace
 
-#define SK_CAT(a, b, c, d) a##b##_##c##_##d
-#define SK_NAME(a, b, c, d) SK_CAT(a, b, c, d)
+#if SK_OCC == 1
+#define SK_SUF
+#else
+#define SK_SUF _o2
+#endif
+#define SK_CAT(a, b, c, d, e) a##b##_##c##_##d##e
+#define SK_NAME(a, b, c, d, e) SK_CAT(a, b, c, d, e)
 
-hipError_t SK_NAME(sk_launch_, SK_KS, SK_WIDE, SK_NMS)(int grid, size_t lds, hipStream_t st, const PkArgs& a)
+hipError_t SK_NAME(sk_launch_, SK_KS, SK_WIDE, SK_NMS, SK_SUF)(int grid, size_t lds, hipStream_t st, const PkArgs& a)
 {
-    hipLaunchKernelGGL((ldpc_sk_kernel<SK_KS, SK_WIDE, SK_NMS>), dim3(grid), dim3(SK_NT), lds, st, a);
+    hipLaunchKernelGGL((ldpc_sk_kernel<SK_KS, SK_WIDE, SK_NMS, SK_OCC>), dim3(grid), dim3(SK_NT), lds, st, a);
     return hipGetLastError();
 }
 
-hipError_t SK_NAME(sk_occupancy_, SK_KS, SK_WIDE, SK_NMS)(size_t lds, int* per_cu)
+hipError_t SK_NAME(sk_occupancy_, SK_KS, SK_WIDE, SK_NMS, SK_SUF)(size_t lds, int* per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ldpc_sk_kernel<SK_KS, SK_WIDE, SK_NMS>, SK_NT, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ldpc_sk_kernel<SK_KS, SK_WIDE, SK_NMS, SK_OCC>, SK_NT,
+                                                        lds);
 }
 
 } // namespace lgk

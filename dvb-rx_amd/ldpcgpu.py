@@ -89,6 +89,7 @@ def gpu_lib():
         L.ldpc_gpu_last_kernel_ms.argtypes = [vp]
         L.ldpc_gpu_last_kernel_ms.restype = C.c_float
         L.ldpc_gpu_last_launches.argtypes = [vp, ip, ip, C.POINTER(C.c_float), i32]
+        L.ldpc_gpu_last_launch_shapes.argtypes = [vp, ip, ip, i32]
         L.ldpc_gpu_code_max_degree.argtypes = [i32]
         L.ldpc_gpu_code_kernel_degree.argtypes = [i32]
         L.ldpc_gpu_set_kernel.argtypes = [vp, i32]
@@ -100,6 +101,8 @@ def gpu_lib():
         L.ldpc_gpu_plan_last_ms.restype = C.c_float
         L.ldpc_gpu_plan_info.argtypes = [vp, ip, vp, vp, vp, i32]
         L.ldpc_gpu_plan_select.argtypes = [vp, i32]
+        L.ldpc_gpu_plan_shape.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32]
+        L.ldpc_gpu_set_launch_shape.argtypes = [vp, i32, i32, i32, C.c_float]
         L.ldpc_gpu_plan_destroy.argtypes = [vp]
         L.ldpc_gpu_decode_batch_nms.argtypes = [vp, vp, i32, vp, vp, vp, vp, i32, vp]
         L.drx_ldpc_lanes.argtypes = []
@@ -303,6 +306,12 @@ class Decoder:
         one after the other on the context's stream (False)."""
         self._check(self.L.ldpc_gpu_set_concurrent(self.ctx, int(bool(on))), "set_concurrent")
 
+    def set_launch_shape(self, di, short_frames, occ, scale=1.0):
+        """Workgroups per CU (1, or 2 where built) and grid scale of the
+        launch of kernel instance di over short or normal frames."""
+        self._check(self.L.ldpc_gpu_set_launch_shape(self.ctx, int(di), int(short_frames), int(occ),
+                                                     float(scale)), "set_launch_shape")
+
     def query(self):
         """True when everything enqueued on the context has finished
         (non-blocking)."""
@@ -345,6 +354,14 @@ class Decoder:
         ms = (C.c_float * cap)()
         n = self.L.ldpc_gpu_last_launches(self.ctx, di, nf, ms, cap)
         return [(di[k], nf[k], ms[k]) for k in range(n)]
+
+    def last_launch_shapes(self):
+        """[(workgroups per CU, workgroups launched)] of the same launches."""
+        cap = 64
+        occ = (C.c_int * cap)()
+        grid = (C.c_int * cap)()
+        n = self.L.ldpc_gpu_last_launch_shapes(self.ctx, occ, grid, cap)
+        return [(occ[k], grid[k]) for k in range(n)]
 
     def decode_host(self, frames, llr, out_bytes):
         """Synchronous decode of host buffers: returns (out, iters, synd)."""
@@ -399,7 +416,23 @@ class Plan:
         ms = np.zeros(8, np.float32)
         n = self.L.ldpc_gpu_plan_info(self.p, C.byref(v), sc.ctypes.data, cc.ctypes.data, ms.ctypes.data, 8)
         return {"variant": v.value, "variants": [{"grid_scale": float(sc[k]), "concurrent": bool(cc[k]),
-                                                  "ms": round(float(ms[k]), 4)} for k in range(n)]}
+                                                  "ms": round(float(ms[k]), 4)} for k in range(n)],
+                "launches": self.shape()}
+
+    def shape(self):
+        """Per launch: kernel instance, frame size, workgroups per CU, grid
+        scale, workgroups launched, frames and frames in flight per CU."""
+        cap = 64
+        a = {k: np.zeros(cap, np.int32) for k in ("di", "short", "occ", "grid", "frames")}
+        sc = np.zeros(cap, np.float32)
+        n = self.L.ldpc_gpu_plan_shape(self.p, a["di"].ctypes.data, a["short"].ctypes.data, a["occ"].ctypes.data,
+                                       sc.ctypes.data, a["grid"].ctypes.data, a["frames"].ctypes.data, cap)
+        self.dec._check(min(n, 0), "plan_shape")
+        ncu = 256
+        return [{"di": int(a["di"][k]), "short": bool(a["short"][k]), "occ": int(a["occ"][k]),
+                 "grid_scale": round(float(sc[k]), 3), "grid": int(a["grid"][k]), "frames": int(a["frames"][k]),
+                 "frames_per_cu": round(2 * min(int(a["grid"][k]), (int(a["frames"][k]) + 1) // 2) / ncu, 2)}
+                for k in range(n)]
 
 
 NMS_DEGS = 33

@@ -11,9 +11,13 @@ the same command (bench.py on the mixed workload, launches serial):
   --copy-fetch DIR   bench.py --copy-calibration run with --pmc FETCH_SIZE
   --copy-write DIR   bench.py --copy-calibration run with --pmc WRITE_SIZE
 
-Dispatches of ldpc_sk_kernel<KS, WIDE, NMS> are keyed by instance and
-frame size (LDS block over 80 KB: normal frames) and averaged over the
-dispatches of a key.  FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are
+Dispatches of ldpc_sk_kernel<KS, WIDE, NMS, OCC> are keyed by instance and
+frame size and averaged over the dispatches of a key.  The traces do not
+carry the launch's dynamic LDS, so the frame size comes from the order of
+the decode's launches (--order: the JSON line bench.py printed, whose
+roofline.launch_order lists them in dispatch order; every decode of the
+run has that order): the i-th decoder dispatch of a run is the
+(i mod count)-th launch of the list.  FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are
 scaled by the factor that maps the plain copy kernel's reading to its
 known size.  Derived per launch (durations from the trace, clock 2.4 GHz):
 
@@ -37,7 +41,8 @@ import sys
 
 CLOCK_GHZ = 2.4
 SIMDS = 256 * 4
-KRE = re.compile(r"ldpc_sk_kernel<(\d+), *(\d+), *(\d+)>")
+KRE = re.compile(r"ldpc_sk_kernel<(\d+), *(\d+), *(\d+)(?:, *(\d+))?>")
+ORDER = []  # the decode's launches in dispatch order: "name size"
 
 
 def rows(d, name):
@@ -51,23 +56,36 @@ def rows(d, name):
     return out
 
 
-def key_of(name, lds):
+def sk_name(name):
     m = KRE.search(name)
     if not m:
         return None
-    return f"ldpc_sk_kernel<{m.group(1)}, {m.group(2)}, {m.group(3)}> {'normal' if lds > 80000 else 'short'}"
+    return f"ldpc_sk_kernel<{m.group(1)}, {m.group(2)}, {m.group(3)}, {m.group(4) or 1}>"
+
+
+def label(dispatches):
+    """{dispatch id: key} for (dispatch id, kernel name) pairs of one run:
+    decoder dispatches in id order, matched to the launch order."""
+    sk = sorted((did, sk_name(n)) for did, n in dispatches if sk_name(n))
+    out = {}
+    for i, (did, nm) in enumerate(sk):
+        want = ORDER[i % len(ORDER)]
+        if not want.startswith(nm + " "):
+            sys.exit(f"dispatch {did} ({nm}) does not match launch {i % len(ORDER)} of the order ({want})")
+        out[did] = want
+    return out
 
 
 def counters(d):
-    """{dispatch: (key, {counter: value summed over its dimensions})}"""
+    """{dispatch: (key, kernel name, {counter: value summed over its dimensions})}"""
     acc = {}
     for r in rows(d, "counter_collection"):
-        k = key_of(r["Kernel_Name"], int(r.get("LDS_Block_Size") or 0))
         did = int(r["Dispatch_Id"])
-        e = acc.setdefault(did, (k, r["Kernel_Name"], {}))
+        e = acc.setdefault(did, (None, r["Kernel_Name"], {}))
         c = r["Counter_Name"]
         e[2][c] = e[2].get(c, 0.0) + float(r["Counter_Value"])
-    return acc
+    keys = label([(did, e[1]) for did, e in acc.items()])
+    return {did: (keys.get(did), e[1], e[2]) for did, e in acc.items()}
 
 
 def copy_factor(d, counter, copy_bytes):
@@ -88,6 +106,7 @@ def mean(v):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace", required=True)
+    ap.add_argument("--order", required=True, help="bench.py's JSON output line (file) of the profiled command")
     ap.add_argument("--pmc", nargs="*", default=[])
     ap.add_argument("--fetch")
     ap.add_argument("--write")
@@ -96,10 +115,15 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
+    with open(a.order) as fh:
+        line = [x for x in fh.read().splitlines() if x.startswith("{")][-1]
+    ORDER.extend(re.sub(r" x\d+$", "", x) for x in json.loads(line)["roofline"]["launch_order"])
 
     launches = {}
-    for r in rows(a.trace, "kernel_trace"):
-        k = key_of(r["Kernel_Name"], int(r.get("LDS_Block_Size") or 0))
+    trace = rows(a.trace, "kernel_trace")
+    keys = label([(int(r["Dispatch_Id"]), r["Kernel_Name"]) for r in trace])
+    for r in trace:
+        k = keys.get(int(r["Dispatch_Id"]))
         if k is None:
             continue
         e = launches.setdefault(k, {"ms": [], "grid_wg": int(r["Grid_Size_X"]) // int(r["Workgroup_Size_X"]),

@@ -156,11 +156,19 @@ int ldpc_gpu_decode_batch_nms(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, i
  * The caller refills d_llr between launches; results as
  * ldpc_gpu_decode_batch.
  *
- * ldpc_gpu_plan_tune is the autotuner: it times every launch shape (the
- * persistent grid per launch, i.e. frames in flight, at 100 / 75 / 50 % of
- * what fits, and concurrent or serial (class, size) launches) over reps
- * replays and keeps the fastest.  ldpc_gpu_plan_info reports the chosen
- * shape and the timings (ms per launch, < 0 if not timed).
+ * ldpc_gpu_plan_tune is the autotuner: it times the shapes common to all
+ * launches (the persistent grid per launch, i.e. frames in flight, at
+ * 100 / 75 / 50 % of what fits, and concurrent or serial launches) over
+ * reps replays, keeps the fastest, then tunes each launch (one per kernel
+ * instance and frame size) in turn, longest first: its workgroups per CU
+ * (1, or 2 where built: plain decoding of short frames on instances 2, 4,
+ * 6 and 106) and its grid scale, keeping a change where the whole replay
+ * gets faster by more than 0.5 %.  ldpc_gpu_plan_info reports the start
+ * variant chosen (-1 once a launch changed) and the variants' timings (ms
+ * per replay, < 0 if not timed); ldpc_gpu_plan_shape the shape of each
+ * launch: kernel instance (the bound ldpc_gpu_last_launches reports),
+ * short frames or not, workgroups per CU, grid scale, workgroups launched
+ * and frames; it returns the count.
  */
 typedef struct ldpc_gpu_plan ldpc_gpu_plan;
 int ldpc_gpu_plan_create(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
@@ -169,7 +177,10 @@ int ldpc_gpu_plan_launch(ldpc_gpu_plan* plan);
 int ldpc_gpu_plan_tune(ldpc_gpu_plan* plan, int reps);
 float ldpc_gpu_plan_last_ms(ldpc_gpu_plan* plan);
 int ldpc_gpu_plan_info(ldpc_gpu_plan* plan, int* variant, float* scale, int* concurrent, float* ms, int max);
-/* Use launch shape `variant` (0 .. count returned by ldpc_gpu_plan_info - 1). */
+int ldpc_gpu_plan_shape(ldpc_gpu_plan* plan, int* di, int* short_frames, int* occ, float* scale, int* grid,
+                        int* frames, int max);
+/* Use launch shape `variant` (0 .. count returned by ldpc_gpu_plan_info - 1)
+ * for all launches. */
 int ldpc_gpu_plan_select(ldpc_gpu_plan* plan, int variant);
 void ldpc_gpu_plan_destroy(ldpc_gpu_plan* plan);
 
@@ -285,6 +296,19 @@ int ldpc_gpu_set_kernel(ldpc_gpu_ctx* ctx, int kernel);
  * with batches of one launch kind. */
 int ldpc_gpu_set_concurrent(ldpc_gpu_ctx* ctx, int concurrent);
 
+/* Launch shape of the context's decodes (no counterpart in the reference):
+ * for the launch of kernel instance di (the bound ldpc_gpu_last_launches
+ * reports) over short (short_frames 1) or normal frames, the workgroups per
+ * CU it is built for (1, or 2 for plain decoding on instances 2, 4, 6 and
+ * 106: two frames more in flight per CU; learned decoding keeps 1) and the
+ * share of the resident workgroups it launches (0 < scale <= 1).  Default:
+ * 2 for the short-frame launches of those instances, 1 elsewhere, scale 1.  Used by
+ * ldpc_gpu_decode_batch, ldpc_gpu_decode_batch_static and plans created
+ * afterwards; ldpc_gpu_plan_shape's output can be set here.  Results do not
+ * depend on the shape.  LDPC_GPU_EINVAL for an instance or build that does
+ * not exist. */
+int ldpc_gpu_set_launch_shape(ldpc_gpu_ctx* ctx, int di, int short_frames, int occ, float scale);
+
 /* Timing of the last ldpc_gpu_decode_batch's decode kernel on the stream it
  * ran on (HIP events around the launch), in milliseconds; < 0 if none. */
 float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* ctx);
@@ -296,6 +320,9 @@ float ldpc_gpu_last_kernel_ms(ldpc_gpu_ctx* ctx);
  * with that bound, the frames of the launch and its time in ms; returns
  * the count. */
 int ldpc_gpu_last_launches(ldpc_gpu_ctx* ctx, int* di, int* nframes, float* ms, int max);
+/* The same launches, in the same order: the workgroups per CU each was
+ * built for (ldpc_gpu_set_launch_shape) and the workgroups it launched. */
+int ldpc_gpu_last_launch_shapes(ldpc_gpu_ctx* ctx, int* occ, int* grid, int max);
 
 #ifdef __cplusplus
 }
