@@ -19,7 +19,11 @@ roofline.launch_order lists them in dispatch order; every decode of the
 run has that order): the i-th decoder dispatch of a run is the
 (i mod count)-th launch of the list.  FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are
 scaled by the factor that maps the plain copy kernel's reading to its
-known size.  Derived per launch (durations from the trace, clock 2.4 GHz):
+known size.  Durations: bench.py runs its decodes' launches concurrently,
+then (the last --serial-decodes decodes) one after another, each kernel
+alone on the GPU, as the counter runs do; "ms" is the mean over those
+serial dispatches, "ms_concurrent" over the others.  Derived per launch
+(serial durations, clock 2.4 GHz):
 
   valu_issue_frac  SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x cycles): the
                    share of the SIMDs' VALU issue slots used (one wave64
@@ -113,6 +117,8 @@ def main():
     ap.add_argument("--copy-fetch")
     ap.add_argument("--copy-write")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--serial-decodes", type=int, default=2,
+                    help="the run's last decodes, whose launches ran one after another")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
     with open(a.order) as fh:
@@ -129,7 +135,11 @@ def main():
         e = launches.setdefault(k, {"ms": [], "grid_wg": int(r["Grid_Size_X"]) // int(r["Workgroup_Size_X"]),
                                     "lds": int(r["LDS_Block_Size"]), "vgpr": int(r["VGPR_Count"]),
                                     "sgpr": int(r["SGPR_Count"])})
-        e["ms"].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+        e.setdefault("all", []).append((int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6))
+    for e in launches.values():
+        d = [ms for _, ms in sorted(e.pop("all"))]
+        ns = min(a.serial_decodes, len(d))
+        e["ms"], e["ms_concurrent"] = d[len(d) - ns:], d[:len(d) - ns]
     vals = {}
     for d in a.pmc:
         for k, name, c in counters(d).values():
@@ -159,8 +169,9 @@ def main():
         ms = mean(t["ms"])
         cyc = ms * 1e-3 * CLOCK_GHZ * 1e9
         v = {cn: mean(x) for cn, x in vals.get(k, {}).items()}
-        rec = {"ms": round(ms, 4), "dispatches": len(t["ms"]), "grid_wg": t["grid_wg"], "lds": t["lds"],
-               "vgpr": t["vgpr"], "sgpr": t["sgpr"]}
+        rec = {"ms": round(ms, 4), "ms_concurrent": round(mean(t["ms_concurrent"]), 4) if t["ms_concurrent"] else None,
+               "dispatches": len(t["ms"]) + len(t["ms_concurrent"]), "grid_wg": t["grid_wg"], "vgpr": t["vgpr"],
+               "sgpr": t["sgpr"]}
         rec.update({cn: round(x) for cn, x in v.items()})
         if "fetch_bytes" in v and "write_bytes" in v:
             rec["hbm_bytes"] = round(v["fetch_bytes"] + v["write_bytes"])
