@@ -956,11 +956,11 @@ def main():
         # the 16 receivers on the GPU, two decoder contexts: frames sent as
         # they arrive (dynamic batching), as libdvbrx's receiver batches them
         # (64 frames per receiver, sent when full or after 100 ms), and at
-        # 4x the load
+        # 4x and 8x the load
         dec2 = ldpcgpu.Decoder(0)
         try:
             c5 = []
-            for dur, load, bmax in ((0.5, 1.0, 1), (0.5, 1.0, 64), (0.25, 4.0, 1)):
+            for dur, load, bmax in ((0.5, 1.0, 1), (0.5, 1.0, 64), (0.25, 4.0, 1), (0.25, 8.0, 1)):
                 rec, _, _ = receiver.run([dec, dec2], dur, load, batch_max=bmax)
                 c5.append(rec)
                 print(f"# receivers load {load} batch {bmax}: p50 {rec['latency_ms']['p50']} ms p99 "

@@ -454,6 +454,45 @@ int ldpc_gpu_download(ldpc_gpu_ctx* c, void* dst, const void* d_src, size_t byte
     return LDPC_GPU_OK;
 }
 
+void* ldpc_gpu_host_alloc(ldpc_gpu_ctx* c, size_t bytes)
+{
+    if (!c || !bytes || hipSetDevice(c->device) != hipSuccess)
+        return nullptr;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        set_err(c, LDPC_GPU_ENOMEM, "pinned host allocation of %zu bytes failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+int ldpc_gpu_host_free(ldpc_gpu_ctx* c, void* h)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    if (h)
+        HIPCHK(c, hipHostFree(h));
+    return LDPC_GPU_OK;
+}
+
+int ldpc_gpu_upload_async(ldpc_gpu_ctx* c, void* d_dst, const void* src, size_t bytes)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return LDPC_GPU_OK;
+}
+
+int ldpc_gpu_download_async(ldpc_gpu_ctx* c, void* dst, const void* d_src, size_t bytes)
+{
+    if (!c)
+        return LDPC_GPU_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    return LDPC_GPU_OK;
+}
+
 int ldpc_gpu_synchronize(ldpc_gpu_ctx* c)
 {
     if (!c)

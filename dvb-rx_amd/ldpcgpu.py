@@ -73,6 +73,11 @@ def gpu_lib():
         L.ldpc_gpu_upload.argtypes = [vp, vp, vp, sz]
         L.ldpc_gpu_download.argtypes = [vp, vp, vp, sz]
         L.ldpc_gpu_synchronize.argtypes = [vp]
+        L.ldpc_gpu_host_alloc.argtypes = [vp, sz]
+        L.ldpc_gpu_host_alloc.restype = vp
+        L.ldpc_gpu_host_free.argtypes = [vp, vp]
+        L.ldpc_gpu_upload_async.argtypes = [vp, vp, vp, sz]
+        L.ldpc_gpu_download_async.argtypes = [vp, vp, vp, sz]
         L.ldpc_gpu_query.argtypes = [vp]
         L.ldpc_gpu_set_concurrent.argtypes = [vp, C.c_int]
         L.ldpc_gpu_demap_symbols.argtypes = [C.c_int]
@@ -297,6 +302,28 @@ class Decoder:
 
     def download(self, arr, dptr):
         self._check(self.L.ldpc_gpu_download(self.ctx, arr.ctypes.data, dptr, arr.nbytes), "download")
+
+    def host_array(self, shape, dtype):
+        """A numpy array in page-locked host memory (free with
+        host_free(arr)); its transfers can be enqueued without waiting."""
+        dtype = np.dtype(dtype)
+        nbytes = max(1, int(np.prod(shape)) * dtype.itemsize)
+        ptr = self.L.ldpc_gpu_host_alloc(self.ctx, nbytes)
+        if not ptr:
+            raise MemoryError(f"host_alloc({nbytes}): {self.error()}")
+        buf = (C.c_uint8 * nbytes).from_address(ptr)
+        return np.frombuffer(buf, np.uint8, int(np.prod(shape)) * dtype.itemsize).view(dtype).reshape(shape)
+
+    def host_free(self, arr):
+        self._check(self.L.ldpc_gpu_host_free(self.ctx, arr.ctypes.data), "host_free")
+
+    def upload_async(self, dptr, src_ptr, nbytes):
+        """Enqueue a copy of nbytes from host address src_ptr (pinned for
+        an asynchronous transfer) to dptr on the context's stream."""
+        self._check(self.L.ldpc_gpu_upload_async(self.ctx, dptr, src_ptr, nbytes), "upload_async")
+
+    def download_async(self, dst_ptr, dptr, nbytes):
+        self._check(self.L.ldpc_gpu_download_async(self.ctx, dst_ptr, dptr, nbytes), "download_async")
 
     def synchronize(self):
         self._check(self.L.ldpc_gpu_synchronize(self.ctx), "synchronize")

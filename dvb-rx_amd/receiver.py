@@ -22,11 +22,14 @@ radio input); everything after arrival runs inside the timed loop:
              device memory in use is bounded (as rxmt.c's pool of batch
              buffers bounds its own, :222-228);
   GPU work   per context, on its own stream, one launch of mixed codes and
-             frame sizes: the frames' symbols / cells copied in, the
+             frame sizes, enqueued without waiting: the frames' symbols /
+             cells copied in from the receivers' page-locked input buffers
+             (runs of consecutive frames, no gather on the host), the
              max-log demapper (ldpc_gpu_demap) or the T2 front end
              (ldpc_gpu_t2_demap), one ldpc_gpu_decode_batch of all of them
              (S2 up to 50 iterations, T2 up to 30, the reference's T2 FEC
-             cap), BCH (ldpc_gpu_bch_decode), results copied out;
+             cap), BCH (ldpc_gpu_bch_decode), results copied out to the
+             context's page-locked result buffers;
   delivery   a receiver hands its frames on in arrival order: a frame
              leaves once it and every earlier frame of its receiver are
              decoded (rxmt.c:571-600), to the receiver's sink.
@@ -211,122 +214,163 @@ class Trace:
         kd = self.kinds[self.kind[f]]
         return kd["refs"][kd["rows"][self.pos[f]]]
 
+    def pin(self, dec):
+        """Move the received symbols / cells (the receivers' input buffers)
+        into page-locked host memory, so that their copies to the device
+        are enqueued without waiting and run at the link's rate."""
+        self.pinned = getattr(self, "pinned", [])
+        if self.pinned:
+            return
+        for kd in self.kinds:
+            for key in ("cells", "csi") if kd["t2"] else ("sym", "n0"):
+                pa = dec.host_array(kd[key].shape, kd[key].dtype)
+                pa[...] = kd[key]
+                kd[key] = pa
+                self.pinned.append((kd, key, pa))
+
+    def release(self, dec):
+        """The input back in ordinary memory, the page-locked buffers freed."""
+        for kd, key, pa in getattr(self, "pinned", []):
+            kd[key] = np.array(pa)
+            dec.host_free(pa)
+        self.pinned = []
+
 
 class GpuFec:
     """The GPU work of the loop: per decoder context a fixed device region
-    for `capacity` frames, and submit / poll / results on the context's
-    stream."""
+    for `capacity` frames and page-locked host buffers for its results;
+    submit enqueues everything on the context's stream and returns, poll
+    collects a finished launch's results."""
 
-    def __init__(self, decs, trace, capacity, learned_tables=None):
+    def __init__(self, decs, trace, capacity):
         self.decs = decs
         self.tr = trace
         self.cap = capacity
         self.mx_sym = max([kd["ns"] * 8 for kd in trace.kinds if not kd["t2"]] +
                           [kd["ncells"] * 12 for kd in trace.kinds if kd["t2"]])
-        self.regions = []
+        self.regions, self.host = [], []
         for dk in decs:
             r = dict(sym=dk.malloc(capacity * self.mx_sym), n0=dk.malloc(4 * capacity),
                      llr=dk.malloc(capacity * 64800), out=dk.malloc(capacity * 8100),
                      it=dk.malloc(4 * capacity), ok=dk.malloc(capacity), bch=dk.malloc(4 * capacity),
                      err=dk.malloc(4 * capacity), sig=dk.malloc(4 * capacity))
             self.regions.append(r)
+            self.host.append(dict(out=dk.host_array(capacity * 8100, np.uint8), it=dk.host_array(capacity, np.int32),
+                                  ok=dk.host_array(capacity, np.uint8), bch=dk.host_array(capacity, np.int32)))
         self.batch = [None] * len(decs)
-        self.results = {}
+        self.finished = []  # per completed launch: frames, segments, bits, iterations, flags, BCH results
 
     def close(self):
-        for dk, r in zip(self.decs, self.regions):
+        for dk, r, h in zip(self.decs, self.regions, self.host):
             for p in r.values():
                 dk.free(p)
-        self.regions = []
-
-    def layout(self, frames):
-        """The frames of a launch grouped by stream kind (each kind's frames
-        consecutive, so BCH runs per kind over one stride)."""
-        kinds = self.tr.kind[frames]
-        order = np.argsort(kinds, kind="stable")
-        return frames[order]
+            for a in h.values():
+                dk.host_free(a)
+        self.regions, self.host = [], []
 
     def submit(self, c, frames):
-        tr, dk, r = self.tr, self.decs[c], self.regions[c]
-        frames = self.layout(frames)
+        """Enqueue the launch of `frames` on context c: per stream kind the
+        frames' symbols (or cells and channel state) copied in, by runs of
+        consecutive frames of the trace's pinned buffers, the demapper (or
+        T2 front end), then one decode of all, BCH per kind and the
+        results copied out to the context's pinned buffers."""
+        tr, dk, r, h = self.tr, self.decs[c], self.regions[c], self.host[c]
+        kinds, pos = tr.kind[frames], tr.pos[frames]
+        o = np.lexsort((pos, kinds))
+        frames, kinds, pos = frames[o], kinds[o], pos[o]
         n = len(frames)
         ft = np.zeros(n, ldpcgpu.FRAME_DTYPE)
         segs = []
-        q = 0
-        sym_off = 0
-        out_off = 0
-        for k, kd in enumerate(tr.kinds):
-            sel = frames[tr.kind[frames] == k]
-            if len(sel) == 0:
-                continue
+        q = sym_off = out_off = 0
+        for k in np.unique(kinds):
+            kd = tr.kinds[k]
+            a, b = np.searchsorted(kinds, [k, k + 1])
+            p, m = pos[a:b], int(b - a)
+            cut = np.flatnonzero(np.diff(p) != 1) + 1
+            runs = list(zip(np.r_[0, cut], np.r_[cut, m]))
             N, kb = kd["fec"].nldpc, kd["fec"].kldpc // 8
-            p = tr.pos[sel]
+            d_llr = r["llr"] + q * 64800
             if kd["t2"]:
-                for c0 in range(0, len(sel), 255):
-                    m = min(255, len(sel) - c0)
-                    cells = np.ascontiguousarray(kd["cells"][p[c0:c0 + m]])
-                    csi = np.ascontiguousarray(kd["csi"][p[c0:c0 + m]])
-                    dk.upload(r["sym"] + sym_off, cells)
-                    dk.upload(r["sym"] + sym_off + cells.nbytes, csi)
-                    ldpcgpu.t2_demap(dk, ldpcgpu.T2Plp(kd["cod"], kd["mod"], kd["ft"], 1, m, 0), m, r["sym"] + sym_off,
-                                     r["sym"] + sym_off + cells.nbytes, r["llr"] + q * 64800 + c0 * N,
+                cells, csi = kd["cells"], kd["csi"]
+                cb, hb = cells[0].nbytes, csi[0].nbytes
+                d_cells = r["sym"] + sym_off
+                d_csi = d_cells + m * cb
+                for s0, e0 in runs:
+                    dk.upload_async(d_cells + int(s0) * cb, cells.ctypes.data + int(p[s0]) * cb, int(e0 - s0) * cb)
+                    dk.upload_async(d_csi + int(s0) * hb, csi.ctypes.data + int(p[s0]) * hb, int(e0 - s0) * hb)
+                for c0 in range(0, m, 255):
+                    mm = min(255, m - c0)
+                    ldpcgpu.t2_demap(dk, ldpcgpu.T2Plp(kd["cod"], kd["mod"], kd["ft"], 1, mm, 0), mm,
+                                     d_cells + c0 * cb, d_csi + c0 * hb, d_llr + c0 * N,
                                      r["err"] + 4 * (q + c0), r["sig"] + 4 * (q + c0))
-                    sym_off += cells.nbytes + csi.nbytes
+                sym_off += m * (cb + hb)
             else:
-                sym = np.ascontiguousarray(kd["sym"][p])
-                dk.upload(r["sym"] + sym_off, sym)
-                dk.upload(r["n0"] + 4 * q, np.ascontiguousarray(kd["n0"][p]))
-                ldpcgpu.demap(dk, kd["pls"], len(sel), r["sym"] + sym_off, r["n0"] + 4 * q, r["llr"] + q * 64800)
-                sym_off += sym.nbytes
-            for m in range(len(sel)):
-                ft[q + m] = (kd["code"], kd["max_iter"], kd["fec"].kldpc, -1, q * 64800 + m * N, out_off + m * kb)
-            segs.append((k, q, len(sel), out_off))
-            q += len(sel)
-            out_off += len(sel) * kb
+                sym, n0 = kd["sym"], kd["n0"]
+                sb = sym[0].nbytes
+                for s0, e0 in runs:
+                    dk.upload_async(r["sym"] + sym_off + int(s0) * sb, sym.ctypes.data + int(p[s0]) * sb,
+                                    int(e0 - s0) * sb)
+                    dk.upload_async(r["n0"] + 4 * (q + int(s0)), n0.ctypes.data + 4 * int(p[s0]), 4 * int(e0 - s0))
+                ldpcgpu.demap(dk, kd["pls"], m, r["sym"] + sym_off, r["n0"] + 4 * q, d_llr)
+                sym_off += m * sb
+            blk = ft[q:q + m]
+            blk["code"], blk["max_iter"], blk["nbits"], blk["group"] = kd["code"], kd["max_iter"], kd["fec"].kldpc, -1
+            blk["llr_off"] = q * 64800 + np.arange(m, dtype=np.uint64) * N
+            blk["out_off"] = out_off + np.arange(m, dtype=np.uint64) * kb
+            segs.append((int(k), q, m, out_off))
+            q += m
+            out_off += m * kb
         dk.decode_batch(ft, r["llr"], r["out"], r["it"], r["ok"])
         for k, q0, cnt, o0 in segs:
             kd = tr.kinds[k]
             ldpcgpu.bch_decode(dk, kd["short"], kd["rate"], cnt, r["out"] + o0, kd["fec"].kldpc // 8, r["bch"] + 4 * q0)
+        for name, nb in (("out", out_off), ("it", 4 * n), ("ok", n), ("bch", 4 * n)):
+            dk.download_async(h[name].ctypes.data, r[name], nb)
         self.batch[c] = (frames, segs, out_off)
 
     def poll(self, c):
         if not self.decs[c].query():
             return False
         frames, segs, out_bytes = self.batch[c]
-        dk, r = self.decs[c], self.regions[c]
-        n = len(frames)
-        out = np.zeros(out_bytes, np.uint8)
-        it = np.zeros(n, np.int32)
-        ok = np.zeros(n, np.uint8)
-        bch = np.zeros(n, np.int32)
-        dk.download(out, r["out"])
-        dk.download(it, r["it"])
-        dk.download(ok, r["ok"])
-        dk.download(bch, r["bch"])
-        for k, q0, cnt, o0 in segs:
-            kb = self.tr.kinds[k]["fec"].kldpc // 8
-            for m in range(cnt):
-                f = int(frames[q0 + m])
-                self.results[f] = (out[o0 + m * kb:o0 + (m + 1) * kb].copy(), int(it[q0 + m]), int(ok[q0 + m]),
-                                   int(bch[q0 + m]))
+        h, n = self.host[c], len(frames)
+        self.finished.append((frames, segs, h["out"][:out_bytes].copy(), h["it"][:n].copy(), h["ok"][:n].copy(),
+                              h["bch"][:n].copy()))
         self.batch[c] = None
         return True
 
+    def results(self):
+        """{frame: (decoded kldpc bits packed, iterations, syndrome flag,
+        BCH result)} of the completed launches."""
+        res = {}
+        for frames, segs, out, it, ok, bch in self.finished:
+            for k, q0, cnt, o0 in segs:
+                kb = self.tr.kinds[k]["fec"].kldpc // 8
+                for m in range(cnt):
+                    res[int(frames[q0 + m])] = (out[o0 + m * kb:o0 + (m + 1) * kb], int(it[q0 + m]), int(ok[q0 + m]),
+                                                int(bch[q0 + m]))
+        return res
 
-def run(decs, duration=0.5, load_scale=1.0, seed=5, capacity=4096, batch_max=1, flush_ms=FLUSH_MS, trace=None):
+
+def run(decs, duration=0.5, load_scale=1.0, seed=5, capacity=4096, batch_max=1, flush_ms=FLUSH_MS, trace=None,
+        concurrent=True):
     """The receivers on the GPU for `duration` seconds of traffic; decs:
-    decoder contexts (one batch in flight each).  Returns the latency and
-    throughput record; with trace given (a Trace), its frames."""
+    decoder contexts (one batch in flight each), each with its decode's
+    launches forked over streams (concurrent, the default: a batch mixes
+    four codes, whose launches then overlap) or one after another on its
+    stream.  Returns the latency and throughput record, the trace and the
+    frames' results."""
     tr = trace or Trace(decs[0], duration, load_scale, seed)
-    g = GpuFec(decs, tr, capacity)
     sink = {}
 
     def on_deliver(s, f):
         sink.setdefault(s, []).append(f)
 
+    g = None
     try:
+        tr.pin(decs[0])
+        g = GpuFec(decs, tr, capacity)
         for dk in decs:
-            dk.set_concurrent(False)
+            dk.set_concurrent(concurrent)
         # warm every context with a full region of each kind (kernel code
         # loaded, the contexts' decoder scratch grown: no device allocation
         # inside the loop)
@@ -337,27 +381,30 @@ def run(decs, duration=0.5, load_scale=1.0, seed=5, capacity=4096, batch_max=1, 
                     g.submit(c, sel)
                     while not g.poll(c):
                         time.sleep(1e-4)
-        g.results.clear()
+        g.finished.clear()
         t0 = time.perf_counter()
         done, delivered, sent, launches = rx_loop(tr.arr, tr.sid, g.submit, g.poll, len(decs), capacity, batch_max,
                                                   flush_ms, on_deliver, t0=t0)
     finally:
         for dk in decs:
             dk.set_concurrent(True)
-        g.close()
+        if g:
+            g.close()
+        tr.release(decs[0])
+    results = g.results()
     n = tr.n
     # delivery in arrival order per receiver, every frame once
     for s, lst in sink.items():
         assert lst == sorted(lst) and len(set(lst)) == len(lst)
     assert sum(len(v) for v in sink.values()) == n
-    errors = sum(int(not np.array_equal(g.results[f][0][:len(tr.ref(f))], tr.ref(f))) for f in range(n))
-    bch_fail = sum(int(g.results[f][3] < 0) for f in range(n))
+    errors = sum(int(not np.array_equal(results[f][0][:len(tr.ref(f))], tr.ref(f))) for f in range(n))
+    bch_fail = sum(int(results[f][3] < 0) for f in range(n))
     lat = (delivered - tr.arr) * 1e3
     wait = (sent - tr.arr) * 1e3
     kl = np.array([tr.kinds[k]["fec"].kldpc for k in tr.kind], np.int64)
     busy = sum(x[2] for x in launches)
     span = max(float(delivered.max()), duration)
-    it = np.array([g.results[f][1] for f in range(n)])
+    it = np.array([results[f][1] for f in range(n)])
 
     def pct(a):
         return {"p50": round(float(np.percentile(a, 50)), 3), "p99": round(float(np.percentile(a, 99)), 3),
@@ -365,7 +412,7 @@ def run(decs, duration=0.5, load_scale=1.0, seed=5, capacity=4096, batch_max=1, 
 
     rec = {
         "decoder": "plain", "frames": int(n), "duration_s": duration, "load_scale": load_scale,
-        "contexts": len(decs), "capacity_frames_per_context": capacity,
+        "contexts": len(decs), "capacity_frames_per_context": capacity, "concurrent_launches": bool(concurrent),
         "batching": f"per receiver, sent at {batch_max} frame(s) or after {flush_ms:g} ms",
         "offered_gbit_s": round(float(kl.sum()) / duration / 1e9, 4),
         "decoded_gbit_s": round(float(kl.sum()) / span / 1e9, 4),
@@ -376,9 +423,12 @@ def run(decs, duration=0.5, load_scale=1.0, seed=5, capacity=4096, batch_max=1, 
         "latency_ms_by_stream": {STREAMS[k][0]: pct(lat[tr.kind == k]) for k in range(len(STREAMS))
                                  if (tr.kind == k).any()},
         "frame_errors_after_bch": errors, "bch_failed": bch_fail,
+        "frame_errors_by_stream": {STREAMS[k][0]: int(sum(not np.array_equal(results[f][0][:len(tr.ref(f))], tr.ref(f))
+                                                        for f in np.nonzero(tr.kind == k)[0]))
+                                   for k in range(len(STREAMS)) if (tr.kind == k).any()},
         "includes": "symbol / cell upload, demapper or T2 front end, LDPC, BCH, results download, in-order hand-off",
     }
-    return rec, tr, g.results
+    return rec, tr, results
 
 
 def main():
@@ -389,10 +439,12 @@ def main():
     ap.add_argument("--capacity", type=int, default=4096)
     ap.add_argument("--batch-max", type=int, default=1)
     ap.add_argument("--flush-ms", type=float, default=FLUSH_MS)
+    ap.add_argument("--serial", action="store_true", help="each decode's launches one after another")
     a = ap.parse_args()
     decs = [ldpcgpu.Decoder(0) for _ in range(a.contexts)]
     try:
-        rec, _, _ = run(decs, a.duration, a.load, capacity=a.capacity, batch_max=a.batch_max, flush_ms=a.flush_ms)
+        rec, _, _ = run(decs, a.duration, a.load, capacity=a.capacity, batch_max=a.batch_max, flush_ms=a.flush_ms,
+                        concurrent=not a.serial)
     finally:
         for d in decs:
             d.close()

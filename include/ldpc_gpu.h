@@ -76,6 +76,18 @@ int ldpc_gpu_free(ldpc_gpu_ctx* ctx, void* d_ptr);
 int ldpc_gpu_upload(ldpc_gpu_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 int ldpc_gpu_download(ldpc_gpu_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 int ldpc_gpu_synchronize(ldpc_gpu_ctx* ctx);
+/* Page-locked host memory and copies enqueued on the context's stream
+ * without waiting (no counterpart in the reference, whose receiver hands
+ * its FEC stage host buffers): a receiver keeps its input and result
+ * buffers pinned, so their transfers run at full link rate and overlap
+ * other contexts' decoding.  A buffer given to an async copy must stay as
+ * it is (upload) or unread (download) until the stream has passed the
+ * copy (ldpc_gpu_query returns 1, or ldpc_gpu_synchronize).  The async
+ * copies accept pageable memory too, then with the runtime's staging. */
+void* ldpc_gpu_host_alloc(ldpc_gpu_ctx* ctx, size_t bytes);
+int ldpc_gpu_host_free(ldpc_gpu_ctx* ctx, void* h_ptr);
+int ldpc_gpu_upload_async(ldpc_gpu_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+int ldpc_gpu_download_async(ldpc_gpu_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 /* Non-blocking completion test (no counterpart in the reference, whose
  * decode call returns when the batch is done): 1 if every call enqueued on
  * the context has finished, 0 if work is pending, < 0 on error.  Lets one

@@ -47,3 +47,28 @@ def test_receiver_frames_match_the_oracle(batch_max):
             assert (out == bits[0]).all(), (kd["name"], f)
             want, _ = oracle_bch(kd["short"], kd["rate"], bits[0])
             assert gbch == want, (kd["name"], f)
+
+
+def test_pinned_buffers_and_async_copies_round_trip():
+    """ldpc_gpu_host_alloc / upload_async / download_async: bytes through
+    the device and back, seen complete by ldpc_gpu_query."""
+    dec = ldpcgpu.Decoder(0)
+    try:
+        a = dec.host_array((3, 1000), np.int32)
+        b = dec.host_array((3, 1000), np.int32)
+        d = dec.malloc(a.nbytes)
+        try:
+            a[...] = np.arange(a.size, dtype=np.int32).reshape(a.shape) * 7 - 11
+            b[...] = 0
+            dec.upload_async(d, a.ctypes.data, a.nbytes)
+            dec.download_async(b.ctypes.data + 4000, d + 4000, 8000)
+            dec.download_async(b.ctypes.data, d, 4000)
+            while not dec.query():
+                pass
+            assert (a == b).all()
+        finally:
+            dec.free(d)
+            dec.host_free(a)
+            dec.host_free(b)
+    finally:
+        dec.close()
