@@ -803,6 +803,7 @@ struct PkLayout {
     std::vector<int> list;
     std::vector<int2> segs;
     int seg0[NLAUNCH], nseg[NLAUNCH], nfr[NLAUNCH], grid[NLAUNCH], maxn[NLAUNCH], maxx[NLAUNCH], occ[NLAUNCH];
+    double cost[NLAUNCH]; /* expected work: layers x slots per lane, summed over the frames */
     int maxw;
     size_t blocks; /* message blocks: sum of the grids */
 };
@@ -824,11 +825,14 @@ static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, c
     std::vector<int> gcodes[NLAUNCH];
     std::vector<std::vector<int>> gseg[NLAUNCH];
     L.maxw = 0;
-    for (int k = 0; k < NLAUNCH; k++)
+    for (int k = 0; k < NLAUNCH; k++) {
         L.maxn[k] = L.maxx[k] = 0;
+        L.cost[k] = 0.0;
+    }
     for (int f = 0; f < (int)hf.size(); f++) {
         const DCode& cd = hc[hf[f].code];
         const int k = cd.sk_inst * 2 + (cd.n < 64800);
+        L.cost[k] += (double)cd.q * SK_INST[cd.sk_inst][0];
         auto it = std::find(gcodes[k].begin(), gcodes[k].end(), hf[f].code);
         size_t g = it - gcodes[k].begin();
         if (it == gcodes[k].end()) {
@@ -883,19 +887,39 @@ static int pk_enqueue(ldpc_gpu_ctx* c, const PkLayout& L, PkArgs a, const int2* 
     HIPCHK(c, hipEventRecord(fork, origin));
     HIPCHK(c, reset_counters(origin, d_cnt));
     a.msg_stride = (size_t)L.maxw;
-    size_t moff = 0;
+    /* message blocks and enqueue order by launch index.  Concurrent: each
+     * launch onto one of the forked streams, whose launches run one after
+     * another; taken in order of expected work, the most first, each goes
+     * to the stream with the least so far */
+    size_t moff[NLAUNCH];
+    int order[NLAUNCH], sidx[NLAUNCH], nl = 0;
+    double load[NAUX] = { 0.0 };
+    for (int k = 0, m = 0; k < NLAUNCH; k++) {
+        moff[k] = (size_t)m * L.maxw;
+        m += L.grid[k];
+        if (L.nfr[k])
+            order[nl++] = k;
+    }
+    std::stable_sort(order, order + nl, [&](int x, int y) { return L.cost[x] > L.cost[y]; });
+    for (int i = 0; i < nl; i++) {
+        int b = 0;
+        for (int s2 = 1; s2 < NAUX; s2++)
+            if (load[s2] < load[b])
+                b = s2;
+        sidx[order[i]] = b;
+        load[b] += L.cost[order[i]];
+    }
     for (int k = 0; k < NLAUNCH; k++) {
         if (!L.nfr[k])
             continue;
         a.seg = d_seg + L.seg0[k];
         a.nseg = L.nseg[k];
         a.cnt = d_cnt + L.seg0[k];
-        a.msg = d_msg + moff;
-        moff += (size_t)L.grid[k] * L.maxw;
+        a.msg = d_msg + moff[k];
         a.hoff = pk_hoff(L, k);
         a.hstride = 360 * L.maxx[k];
         const size_t lds = pk_lds(L, k);
-        hipStream_t st = concurrent ? aux[k % NAUX] : origin;
+        hipStream_t st = concurrent ? aux[sidx[k]] : origin;
         if (concurrent)
             HIPCHK(c, hipStreamWaitEvent(st, fork, 0));
         if (tev)

@@ -109,10 +109,13 @@ def test_plan_refill_and_tune(dec):
         dec.upload(b.llr, llr1)
         plan.tune(2)
         info = plan.info()
-        assert 0 <= info["variant"] < len(info["variants"])
+        # the common shape the tuner kept, or -1 once its per-launch search
+        # changed a launch (timing decides, so both occur)
+        assert -1 <= info["variant"] < len(info["variants"])
         assert all(v["ms"] > 0 for v in info["variants"])
         best = min(range(len(info["variants"])), key=lambda k: info["variants"][k]["ms"])
-        assert info["variant"] == best
+        assert info["variant"] in (best, -1)
+        assert sum(x["frames"] for x in info["launches"]) == len(fr)
         for llr in (llr2, llr1, llr2):
             dec.upload(b.llr, llr)
             dec.upload(ref.llr, llr)
