@@ -95,7 +95,7 @@ struct PkShape {
 struct ldpc_gpu_ctx {
     int device;
     hipStream_t stream;
-    hipStream_t aux[NAUX]; /* forked streams of concurrent launches (launch k on aux[k % NAUX]) */
+    hipStream_t aux[NAUX]; /* forked streams of concurrent launches */
     char err[256];
     DeviceTables* tab;
     int ncu;

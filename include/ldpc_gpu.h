@@ -301,11 +301,10 @@ int ldpc_gpu_t2_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, int n
 int ldpc_gpu_set_kernel(ldpc_gpu_ctx* ctx, int kernel);
 
 /* Launch mode of ldpc_gpu_decode_batch (no counterpart in the reference).
- * 1 (the default): the launches of one call - one per degree class and
- * frame size - run concurrently on forked streams.  0: one after the other
- * on the context's stream, so that the context occupies a single hardware
- * queue; for callers that keep several contexts in flight at once, each
- * with batches of one launch kind. */
+ * 1 (the default): the launches of one call - one per kernel instance and
+ * frame size - run concurrently, spread by expected work over three forked
+ * streams.  0: one after the other on the context's stream, so that the
+ * context occupies a single hardware queue. */
 int ldpc_gpu_set_concurrent(ldpc_gpu_ctx* ctx, int concurrent);
 
 /* Launch shape of the context's decodes (no counterpart in the reference):
