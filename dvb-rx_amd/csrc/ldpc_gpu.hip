@@ -123,7 +123,7 @@ struct ldpc_gpu_ctx {
     int cls_grid[NLAUNCH];
     int timed;
     int force_scalar;        /* per-frame batches through the one-frame kernel */
-    int serial;              /* launches of a decode one after the other on the context's stream */
+    int lanes;               /* forked streams a decode's launches use; 0: one after the other on the context's stream */
     PkShape shape[NLAUNCH];  /* launch shape per (instance, size); zero: one workgroup per CU, full grid */
     /* learned tables: row 0 plain (S = 16, B = 1), then per code its rows */
     uint8_t* h_nS;
@@ -162,7 +162,7 @@ struct ldpc_gpu_plan {
     float variant_ms[8];
     int launched;
     PkShape shape[NLAUNCH]; /* the launch shapes the graph was built with */
-    bool concurrent;
+    int lanes; /* forked streams of the launches, 0: one after the other */
     int nfr[NLAUNCH], grid[NLAUNCH], occ[NLAUNCH];
     float tuned_ms; /* ms per replay of the tuned shape, < 0 before ldpc_gpu_plan_tune */
 };
@@ -310,6 +310,7 @@ int ldpc_gpu_create(int device, ldpc_gpu_ctx** out)
     }
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         c->ncu = 256;
+    c->lanes = NAUX_DEFAULT;
     /* short frames on the instances built for it: two workgroups per CU
      * (18-23 % less time per frame-iteration once a launch fills the CUs,
      * the same below that) */
@@ -518,7 +519,9 @@ int ldpc_gpu_set_concurrent(ldpc_gpu_ctx* c, int concurrent)
 {
     if (!c)
         return LDPC_GPU_EINVAL;
-    c->serial = !concurrent;
+    if (concurrent < 0 || concurrent > NAUX)
+        return set_err(c, LDPC_GPU_EINVAL, "set_concurrent: %d (0 .. %d)", concurrent, NAUX);
+    c->lanes = concurrent == 1 ? NAUX_DEFAULT : concurrent;
     return LDPC_GPU_OK;
 }
 
@@ -880,17 +883,18 @@ static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, c
  * NULL): start / end timing events per launch. */
 static int pk_enqueue(ldpc_gpu_ctx* c, const PkLayout& L, PkArgs a, const int2* d_seg, unsigned* d_cnt,
                       uint32_t* d_msg, hipStream_t origin, const hipStream_t* aux, hipEvent_t fork,
-                      const hipEvent_t* join, const hipEvent_t* tev, bool concurrent, bool nms)
+                      const hipEvent_t* join, const hipEvent_t* tev, int lanes, bool nms)
 {
     /* the fork event also starts the decode's timing (launch_pk), so it is
      * recorded in both modes */
     HIPCHK(c, hipEventRecord(fork, origin));
     HIPCHK(c, reset_counters(origin, d_cnt));
     a.msg_stride = (size_t)L.maxw;
-    /* message blocks and enqueue order by launch index.  Concurrent: each
-     * launch onto one of the forked streams, whose launches run one after
-     * another; taken in order of expected work, the most first, each goes
-     * to the stream with the least so far */
+    /* message blocks and enqueue order by launch index.  lanes > 0: each
+     * launch onto one of the first `lanes` forked streams, whose launches
+     * run one after another; taken in order of expected work, the most
+     * first, each goes to the stream with the least so far */
+    const bool concurrent = lanes > 0;
     size_t moff[NLAUNCH];
     int order[NLAUNCH], sidx[NLAUNCH], nl = 0;
     double load[NAUX] = { 0.0 };
@@ -903,7 +907,7 @@ static int pk_enqueue(ldpc_gpu_ctx* c, const PkLayout& L, PkArgs a, const int2* 
     std::stable_sort(order, order + nl, [&](int x, int y) { return L.cost[x] > L.cost[y]; });
     for (int i = 0; i < nl; i++) {
         int b = 0;
-        for (int s2 = 1; s2 < NAUX; s2++)
+        for (int s2 = 1; s2 < lanes; s2++)
             if (load[s2] < load[b])
                 b = s2;
         sidx[order[i]] = b;
@@ -987,7 +991,7 @@ static int launch_pk(ldpc_gpu_ctx* c, const DFrame* d_frames, const std::vector<
         c->cls_occ[k] = L.occ[k];
         c->cls_grid[k] = L.grid[k];
     }
-    if ((rc = pk_enqueue(c, L, a, c->d_seg, c->ticket, c->msg, c->stream, c->aux, c->ev0, join, c->ev_cls, !c->serial,
+    if ((rc = pk_enqueue(c, L, a, c->d_seg, c->ticket, c->msg, c->stream, c->aux, c->ev0, join, c->ev_cls, c->lanes,
                          nms)))
         return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -1377,11 +1381,11 @@ static void plan_variant_shape(ldpc_gpu_plan* p, int variant)
         p->shape[k].occ = p->c->shape[k].occ;
         p->shape[k].scale = PLAN_VARIANTS[variant].scale;
     }
-    p->concurrent = PLAN_VARIANTS[variant].concurrent;
+    p->lanes = PLAN_VARIANTS[variant].concurrent ? (p->c->lanes ? p->c->lanes : NAUX_DEFAULT) : 0;
 }
 
 /* (re)build the plan's graph with its current shape (p->shape,
- * p->concurrent); variant: the start variant it came from, or -1 once the
+ * p->lanes); variant: the start variant it came from, or -1 once the
  * per-launch search changed it */
 static int plan_build(ldpc_gpu_plan* p, int variant)
 {
@@ -1405,7 +1409,7 @@ static int plan_build(ldpc_gpu_plan* p, int variant)
     const PkArgs a = pk_args(c, p->d_frames, p->d_list, p->d_llr, p->d_out, p->d_iters, p->d_synd, false, 0, nullptr);
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     rc = pk_enqueue(c, L, a, p->d_seg, p->d_cnt, p->d_msg, c->stream, p->aux, p->fork, p->join, nullptr,
-                    p->concurrent, false);
+                    p->lanes, false);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc || e != hipSuccess) {

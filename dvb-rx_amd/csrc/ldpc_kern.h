@@ -21,12 +21,14 @@
 #define SK_INSTS { { 2, 0 }, { 3, 0 }, { 4, 0 }, { 4, 1 }, { 5, 0 }, { 6, 0 }, { 7, 0 }, { 7, 1 }, { 8, 0 }, { 9, 0 }, \
                    { 10, 0 }, { 10, 1 }, { 11, 0 }, { 14, 0 }, { 15, 0 } }
 #define NLAUNCH (2 * SK_NINST) /* launches of one decode: kernel instance x frame size */
-/* forked streams the concurrent launches of a decode share: one hardware
- * queue each beside the context's own stream (GPU_MAX_HW_QUEUES 4).  Three
- * measured best on the mixed S2 batch: 23.5-23.8 ms, against 24.5 with
- * four (two then share a queue), 25.6 with the context's stream as a
- * fourth lane and 28.0 with two. */
-#define NAUX 3
+/* forked streams a context holds for the concurrent launches of a decode
+ * (one hardware queue each beside the context's own stream at
+ * GPU_MAX_HW_QUEUES 4), and how many a decode uses by default.  Three
+ * measured best on the mixed S2 batch (18 launches): 23.5-23.8 ms, against
+ * 24.5 with four (two then share a queue), 25.6 with the context's stream
+ * as a fourth lane and 28.0 with two. */
+#define NAUX 4
+#define NAUX_DEFAULT 3
 
 namespace lgk {
 

@@ -329,9 +329,10 @@ class Decoder:
         self._check(self.L.ldpc_gpu_synchronize(self.ctx), "synchronize")
 
     def set_concurrent(self, on):
-        """Launches of one decode on forked streams (True, the default) or
-        one after the other on the context's stream (False)."""
-        self._check(self.L.ldpc_gpu_set_concurrent(self.ctx, int(bool(on))), "set_concurrent")
+        """Launches of one decode on forked streams (True, the default: 3;
+        or an int, 2 .. 4, that many) or one after the other on the
+        context's stream (False / 0)."""
+        self._check(self.L.ldpc_gpu_set_concurrent(self.ctx, int(on)), "set_concurrent")
 
     def set_launch_shape(self, di, short_frames, occ, scale=1.0):
         """Workgroups per CU (1, or 2 where built) and grid scale of the

@@ -54,6 +54,7 @@ import streaming  # noqa: E402
 from streaming import STREAMS, T2_PLP  # noqa: E402
 
 FLUSH_MS = 100.0  # rxmt.c:127, max_latency_ms
+LANES = 4  # forked streams per context: one per launch of a batch of the four stream kinds
 
 
 def rx_loop(arr, sid, submit, poll, ncontexts, capacity, batch_max=1, flush_ms=FLUSH_MS, on_deliver=None,
@@ -352,11 +353,11 @@ class GpuFec:
 
 
 def run(decs, duration=0.5, load_scale=1.0, seed=5, capacity=4096, batch_max=1, flush_ms=FLUSH_MS, trace=None,
-        concurrent=True):
+        concurrent=LANES):
     """The receivers on the GPU for `duration` seconds of traffic; decs:
     decoder contexts (one batch in flight each), each with its decode's
-    launches forked over streams (concurrent, the default: a batch mixes
-    four codes, whose launches then overlap) or one after another on its
+    launches forked over `concurrent` streams (a batch mixes four codes,
+    whose launches then overlap) or, with 0, one after another on its
     stream.  Returns the latency and throughput record, the trace and the
     frames' results."""
     tr = trace or Trace(decs[0], duration, load_scale, seed)
@@ -412,7 +413,7 @@ def run(decs, duration=0.5, load_scale=1.0, seed=5, capacity=4096, batch_max=1, 
 
     rec = {
         "decoder": "plain", "frames": int(n), "duration_s": duration, "load_scale": load_scale,
-        "contexts": len(decs), "capacity_frames_per_context": capacity, "concurrent_launches": bool(concurrent),
+        "contexts": len(decs), "capacity_frames_per_context": capacity, "launch_streams": int(concurrent),
         "batching": f"per receiver, sent at {batch_max} frame(s) or after {flush_ms:g} ms",
         "offered_gbit_s": round(float(kl.sum()) / duration / 1e9, 4),
         "decoded_gbit_s": round(float(kl.sum()) / span / 1e9, 4),
@@ -439,12 +440,12 @@ def main():
     ap.add_argument("--capacity", type=int, default=4096)
     ap.add_argument("--batch-max", type=int, default=1)
     ap.add_argument("--flush-ms", type=float, default=FLUSH_MS)
-    ap.add_argument("--serial", action="store_true", help="each decode's launches one after another")
+    ap.add_argument("--lanes", type=int, default=LANES, help="forked streams per context; 0: launches one after another")
     a = ap.parse_args()
     decs = [ldpcgpu.Decoder(0) for _ in range(a.contexts)]
     try:
         rec, _, _ = run(decs, a.duration, a.load, capacity=a.capacity, batch_max=a.batch_max, flush_ms=a.flush_ms,
-                        concurrent=not a.serial)
+                        concurrent=a.lanes)
     finally:
         for d in decs:
             d.close()

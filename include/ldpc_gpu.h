@@ -303,8 +303,11 @@ int ldpc_gpu_set_kernel(ldpc_gpu_ctx* ctx, int kernel);
 /* Launch mode of ldpc_gpu_decode_batch (no counterpart in the reference).
  * 1 (the default): the launches of one call - one per kernel instance and
  * frame size - run concurrently, spread by expected work over three forked
- * streams.  0: one after the other on the context's stream, so that the
- * context occupies a single hardware queue. */
+ * streams (one hardware queue each at GPU_MAX_HW_QUEUES 4).  2 .. 4: over
+ * that many (4 suits batches of about four launches, as a receiver's
+ * mixed batches are).  0: one after the other on the context's stream, so
+ * that the context occupies a single hardware queue.  Plans created
+ * afterwards use the same count for their concurrent shapes. */
 int ldpc_gpu_set_concurrent(ldpc_gpu_ctx* ctx, int concurrent);
 
 /* Launch shape of the context's decodes (no counterpart in the reference):

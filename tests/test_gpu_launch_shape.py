@@ -159,3 +159,27 @@ def test_plan_per_launch_tuning_matches_eager(dec):
             plan.close()
         b.free()
         ref.free()
+
+
+def test_launch_stream_counts_give_the_same_results(dec):
+    """ldpc_gpu_set_concurrent: serial, 2, 3 (the default) and 4 forked
+    streams decode the batch identically; out-of-range counts rejected."""
+    llr, fr, nout, _ = _batch(1100, 12)
+    b = Bufs(dec, llr, nout, len(fr))
+    try:
+        b.clear()
+        dec.decode_batch(fr, b.llr, b.out, b.it, b.ok)
+        dec.synchronize()
+        want = b.results()
+        for n in (0, 2, 4, 1, 3):
+            dec.set_concurrent(n)
+            b.clear()
+            dec.decode_batch(fr, b.llr, b.out, b.it, b.ok)
+            dec.synchronize()
+            _same(want, b.results(), ("streams", n))
+        for n in (-1, 5):
+            with pytest.raises(RuntimeError):
+                dec.set_concurrent(n)
+    finally:
+        dec.set_concurrent(True)
+        b.free()
