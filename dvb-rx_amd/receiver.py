@@ -230,7 +230,8 @@ class Trace:
                 self.pinned.append((kd, key, pa))
 
     def release(self, dec):
-        """The input back in ordinary memory, the page-locked buffers freed."""
+        """The input back in ordinary memory, the page-locked buffers freed
+        (no copy from them may be in flight)."""
         for kd, key, pa in getattr(self, "pinned", []):
             kd[key] = np.array(pa)
             dec.host_free(pa)
@@ -262,6 +263,13 @@ class GpuFec:
         self.finished = []  # per completed launch: frames, segments, bits, iterations, flags, BCH results
 
     def close(self):
+        # nothing in flight may still read or write the buffers (an error
+        # can end the loop with launches pending)
+        for dk in self.decs:
+            try:
+                dk.synchronize()
+            except RuntimeError:
+                pass
         for dk, r, h in zip(self.decs, self.regions, self.host):
             for p in r.values():
                 dk.free(p)
