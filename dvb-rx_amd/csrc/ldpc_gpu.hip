@@ -237,15 +237,22 @@ static int upload_tables(ldpc_gpu_ctx* c)
             return set_err(c, LDPC_GPU_EINVAL, "split kernel tables exceed 2 GiB");
         adt.insert(adt.end(), cd->sk_tab, cd->sk_tab + cd->sk_words);
         ent.insert(ent.end(), cd->ent, cd->ent + cd->nent);
+        d.pkl_off = (int)pkl.size();
         for (int i = 0; i < cd->q; i++) {
             const int st = cd->layer_start[i], deg = cd->layer_start[i + 1] - st;
             d.dmin = std::min(d.dmin, deg);
             lay.push_back(make_uint2((uint32_t)st | (uint32_t)deg << 16 | (uint32_t)cd->nlev[i] << 24,
                                      (uint32_t)cd->msg_off[i]));
             const uint32_t* sl = cd->sk_lay + 4 * i;
-            pkl.push_back(make_uint4(sl[0], 4 * sl[1], (uint32_t)(4 * (base + sl[2])),
+            /* bit 17: the workgroup waits after this layer (the next one
+             * waits, bit 16 of its descriptor, or this is the last) */
+            const uint32_t after = i + 1 == cd->q || ((cd->sk_lay[4 * (i + 1)] >> 16) & 1);
+            pkl.push_back(make_uint4(sl[0] | after << 17, 4 * sl[1], (uint32_t)(4 * (base + sl[2])),
                                      cd->nlev[i] > 1 ? (uint32_t)(4 * (base + sl[3])) : 0u));
         }
+        /* the kernel reads descriptors two layers ahead without a bound */
+        pkl.push_back(pkl.back());
+        pkl.push_back(pkl.back());
         meta.insert(meta.end(), cd->meta, cd->meta + cd->r);
         /* every layer degree must be one the code's class kernel handles */
         const int cls = d.dmax <= LOW_MAX_DEG ? 0 : 1;

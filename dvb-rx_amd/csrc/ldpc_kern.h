@@ -37,6 +37,7 @@ struct DCode {
     int n, k, r, q;
     int ent_off;   /* into ent: base << 16 | shift */
     int lay_off;   /* into lay (see lay_deg / lay_nlev / lay_start) */
+    int pkl_off;   /* into pkl: the code's q layer descriptors, then two copies of the last */
     int meta_off;  /* into meta: level and dependency masks per node */
     int nent;      /* (base, shift) pairs */
     int dmax;      /* largest information degree of a layer */

@@ -13,8 +13,10 @@
  * bits, one of SK_RB(KS) words (the slots' dependency masks: dep in bits
  * 0-15, succ in 16-31, then the hand-over cells).  Messages: per layer and
  * lane SK_WORDS(KS) words in the workgroup's message block.  Layer
- * descriptor (PkArgs.pkl): x = positions T | level count << 8 | barrier <<
- * 16 | kde << 24 (slots 1 .. kde - 1 hold the shared edges), y = byte
+ * descriptor (PkArgs.pkl, per code from DCode.pkl_off, two copies of the
+ * last after it): x = positions T | level count << 8 | barrier before <<
+ * 16 | barrier after << 17 | kde << 24 (slots 1 .. kde - 1 hold the shared
+ * edges), y = byte
  * offset of the layer's messages in the block, z = byte offset of its
  * address records in adt, w = of its tail records.  A lane reads and
  * writes its words with buffer instructions: the layer's offset in a
@@ -119,7 +121,7 @@ __device__ __forceinline__ uint4 sl_desc(const uint4* skl, int i)
 
 __device__ __forceinline__ int sl_T(uint32_t x) { return (int)(x & 0xff); }
 __device__ __forceinline__ int sl_nlev(uint32_t x) { return (int)((x >> 8) & 0xff); }
-__device__ __forceinline__ bool sl_bar(uint32_t x) { return (x >> 16) & 1; }
+__device__ __forceinline__ bool sl_bar_after(uint32_t x) { return (x >> 17) & 1; }
 __device__ __forceinline__ int sl_kde(uint32_t x) { return (int)(x >> 24); }
 
 /* the partner lane's value (lanes 2j and 2j + 1 swap: quad permutation
@@ -251,9 +253,13 @@ __device__ __forceinline__ void sk_iteration(uint16_t* lds, rsrc_t rt, const uin
         if (sl_nlev(ld.x) > 1)
             ld_words<1 + CW>(nt, rt, ln.vb, ld.w);
     };
+    /* the descriptors: layer i's in ldn at the top of layer i, layer i +
+     * 1's in ldf, then layer i + 2's from dp (the table holds two copies of
+     * the last descriptor after it) */
     uint4 ldn = sl_desc(skl, 0);
     fetch(ldn);
-    uint4 ldf = sl_desc(skl, min(1, q - 1));
+    uint4 ldf = sl_desc(skl, 1);
+    const uint4* dp = skl + 2;
     /* layer 0's words are needed at once; waiting for them here also tells
      * the compiler that nothing is outstanding when the loop starts, so the
      * loop's own waits need not cover the message stores of the layer
@@ -275,7 +281,8 @@ __device__ __forceinline__ void sk_iteration(uint16_t* lds, rsrc_t rt, const uin
         if (i + 1 < q) {
             ldn = ldf;
             fetch(ldn);
-            ldf = sl_desc(skl, min(i + 2, q - 1));
+            ldf = sl_desc(dp, 0);
+            dp++;
         }
         uint32_t raw[KS];
         sk_read<KS>(raw, lds, n);
@@ -297,7 +304,7 @@ __device__ __forceinline__ void sk_iteration(uint16_t* lds, rsrc_t rt, const uin
                             hstride, s_vote, nvote);
         /* the next layer waits for this one where it reads what another
          * wave wrote (its barrier flag; the end of the iteration always) */
-        if (i + 1 == q || sl_bar(ldn.x))
+        if (sl_bar_after(ld.x))
             __syncthreads();
     }
 }
@@ -397,7 +404,7 @@ __global__ void __launch_bounds__(SK_NT, 3 * SK_OCC) ldpc_sk_kernel(PkArgs a)
         if (newseg) {
             cd = a.codes[a.frames[fany].code];
         }
-        const uint4* skl = a.pkl + cd.lay_off;
+        const uint4* skl = a.pkl + cd.pkl_off;
         const int K = cd.k, R = cd.r, q = cd.q;
         /* a frame's LLRs into its slot bytes: information bits in order,
          * parity bit K + q*jj + ii at K + 360*ii + jj (layer order) */
