@@ -236,7 +236,7 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
 
 
-def config1(dec, reps=20, cpu=True):
+def single_code_record(dec, reps=20, cpu=True):
     """The reference's own CPU-runnable case: 64 DVB-S2 normal 1/2 QPSK
     frames at Es/N0 1.5 dB, max 50 iterations.  Timed three ways: the
     device-resident batch API per-frame and as one group (the reference's
@@ -276,13 +276,14 @@ def config1(dec, reps=20, cpu=True):
     res["dropin_host_buffers"] = {"ms_per_call": round(dt * 1e3, 3), "gbit_s": round(b.info_bits / dt / 1e9, 3),
                                   "result": int(r)}
     if cpu:
-        res["cpu_baseline"] = config1_cpu(b)
+        res["cpu_baseline"] = single_code_cpu(b)
     return res
 
 
-def config1_cpu(b, reps=10):
-    """The reference decoder on config 1 as tests/test_ldpc.c:77-95 times
-    it: the 64 frames in batches of LANES on one thread (a batch cannot
+def single_code_cpu(b, reps=10):
+    """The reference decoder on the 64-frame normal 1/2 QPSK batch as
+    tests/test_ldpc.c:77-95 times it: the frames in batches of LANES on one
+    thread (a batch cannot
     be split over threads), clock around the decode loop; best of reps
     after a warm-up, per build."""
     _, per_core = host_cpu()
@@ -320,7 +321,7 @@ def config1_cpu(b, reps=10):
     return out
 
 
-def config3(dec, margin_db=0.2, reps=5, cpu=False, cpu_seconds=6.0):
+def s2x_high_order_record(dec, margin_db=0.2, reps=5, cpu=False, cpu_seconds=6.0):
     """DVB-S2X high-order batch (16384 normal frames on 64/128/256APSK
     MODCODs, 4096 short on QPSK/8PSK, threshold + margin_db, generated on
     the GPU) decoded by a prepared plan: launch shape chosen by the
@@ -652,7 +653,7 @@ def microbench(dec, nframes=2048, iters=10, reps=3, codes=None):
     return out
 
 
-def config4(dec, nf=8192, db=0.2, reps=3):
+def learned_record(dec, nf=8192, db=0.2, reps=3):
     """Learned vs plain decoding on the two trained MODCODs (DVB-S2 normal
     3/5 8PSK and 2/3 16APSK) at Es/N0 = QEF + db: frames from pool
     codewords through the GPU channel, decoded by the plain decoder (up to
@@ -726,7 +727,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="all-core time of each reference build on the mixed workload (one core: half)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-config1", action="store_true",
+    ap.add_argument("--headline-only", action="store_true",
                     help="skip the side measurements (64-frame rate 1/2 batch, learned vs plain)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
@@ -753,7 +754,7 @@ def main():
             codes = [(int(c.split(":")[0]), c.split(":")[1]) for c in args.codes.split(",")]
             rec = microbench(dec, codes=codes)
         else:
-            rec = {"demapper": demap_record, "bch": bch_record, "learned": config4, "t2": t2_record,
+            rec = {"demapper": demap_record, "bch": bch_record, "learned": learned_record, "t2": t2_record,
                    "microbench": microbench}[args.only](dec)
         dec.close()
         print(json.dumps({args.only: rec}), flush=True)
@@ -929,7 +930,7 @@ def main():
     # autotuner, replayed as one graph (after the timed steps; the plan
     # writes the same results into the same buffers)
     plan_rec = None
-    if not args.no_config1:
+    if not args.headline_only:
         plan = ldpcgpu.Plan(dec, batch.frames, d_llr, d_out, d_it, d_ok)
         try:
             plan.tune(2)
@@ -943,14 +944,14 @@ def main():
         pmed = float(np.median(pms))
         plan_rec = {"gbit_s": round(batch.info_bits / (pmed * 1e-3) / 1e9, 3), "ms": round(pmed, 3),
                     "autotuner": pinfo}
-    c1 = None if args.no_config1 else config1(dec, cpu=not args.no_cpu)
-    c4 = None if args.no_config1 else config4(dec)
-    c3 = None if args.no_config1 else [config3(dec, 0.2, cpu=not args.no_cpu), config3(dec, 1.0, cpu=not args.no_cpu)]
-    dm = None if args.no_config1 else demap_record(dec)
-    bch = None if args.no_config1 else bch_record(dec)
-    t2 = None if args.no_config1 else t2_record(dec, cpu=not args.no_cpu)
+    c1 = None if args.headline_only else single_code_record(dec, cpu=not args.no_cpu)
+    c4 = None if args.headline_only else learned_record(dec)
+    c3 = None if args.headline_only else [s2x_high_order_record(dec, 0.2, cpu=not args.no_cpu), s2x_high_order_record(dec, 1.0, cpu=not args.no_cpu)]
+    dm = None if args.headline_only else demap_record(dec)
+    bch = None if args.headline_only else bch_record(dec)
+    t2 = None if args.headline_only else t2_record(dec, cpu=not args.no_cpu)
     c5 = None
-    if not args.no_config1:
+    if not args.headline_only:
         import receiver
 
         # the 16 receivers on the GPU, two decoder contexts: frames sent as
@@ -1023,7 +1024,7 @@ def main():
             "iterations_total": iters_total,
         },
         "cpu_baseline": cpu,
-        "config1": c1,
+        "single_code_64": c1,
         "learned_vs_plain": c4,
         "s2x_high_order_graph": c3,
         "streaming": c5,

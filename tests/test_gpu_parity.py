@@ -28,7 +28,7 @@ def run_frames(dec, fec, code, llr, max_iter, group=-1):
     return out.reshape(n, kb), it, ok
 
 
-def test_config1_per_frame(dec):
+def test_single_code_64_per_frame(dec):
     """64 frames, DVB-S2 normal 1/2, QPSK, Es/N0 1.5 dB, max 50 iterations,
     each frame with its own early stop (the reference called per frame)."""
     fec, llr, ref = gen(0, "1_2", 1.5, 64, seed=2024, pls=0x10)
@@ -41,7 +41,7 @@ def test_config1_per_frame(dec):
     assert o_ok.all() and (o_bits == ref).all()
 
 
-def test_config1_group(dec):
+def test_single_code_64_group(dec):
     """Same 64 frames as one group: the reference's batch call semantics."""
     fec, llr, ref = gen(0, "1_2", 1.5, 64, seed=2024, pls=0x10)
     code = ldpcgpu.code_lookup(0, "1_2")
