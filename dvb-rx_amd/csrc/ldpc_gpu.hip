@@ -1378,9 +1378,14 @@ extern "C" int ldpc_gpu_decode_batch_static(ldpc_gpu_ctx* c, const ldpc_gpu_fram
  * size) launches run concurrently; the context's workgroups per CU */
 static const struct {
     float scale;
-    bool concurrent;
-} PLAN_VARIANTS[] = { { 1.0f, true }, { 1.0f, false }, { 0.75f, true }, { 0.5f, true }, { 0.5f, false } };
+    int lanes; /* forked streams; -1: the context's count (default 3), 0: one after the other */
+} PLAN_VARIANTS[] = { { 1.0f, -1 }, { 1.0f, 0 }, { 0.75f, -1 }, { 0.5f, -1 }, { 0.5f, 0 }, { 1.0f, 2 }, { 1.0f, 4 } };
 #define PLAN_NVARIANTS ((int)(sizeof(PLAN_VARIANTS) / sizeof(PLAN_VARIANTS[0])))
+
+static int plan_lanes(const ldpc_gpu_plan* p, int v)
+{
+    return PLAN_VARIANTS[v].lanes >= 0 ? PLAN_VARIANTS[v].lanes : (p->c->lanes ? p->c->lanes : NAUX_DEFAULT);
+}
 
 static void plan_variant_shape(ldpc_gpu_plan* p, int variant)
 {
@@ -1388,7 +1393,7 @@ static void plan_variant_shape(ldpc_gpu_plan* p, int variant)
         p->shape[k].occ = p->c->shape[k].occ;
         p->shape[k].scale = PLAN_VARIANTS[variant].scale;
     }
-    p->lanes = PLAN_VARIANTS[variant].concurrent ? (p->c->lanes ? p->c->lanes : NAUX_DEFAULT) : 0;
+    p->lanes = plan_lanes(p, variant);
 }
 
 /* (re)build the plan's graph with its current shape (p->shape,
@@ -1667,7 +1672,7 @@ extern "C" int ldpc_gpu_plan_info(ldpc_gpu_plan* p, int* variant, float* scale, 
         if (scale)
             scale[v] = PLAN_VARIANTS[v].scale;
         if (concurrent)
-            concurrent[v] = PLAN_VARIANTS[v].concurrent;
+            concurrent[v] = plan_lanes(p, v);
         if (ms)
             ms[v] = p->variant_ms[v];
     }

@@ -443,7 +443,7 @@ class Plan:
         cc = np.zeros(8, np.int32)
         ms = np.zeros(8, np.float32)
         n = self.L.ldpc_gpu_plan_info(self.p, C.byref(v), sc.ctypes.data, cc.ctypes.data, ms.ctypes.data, 8)
-        return {"variant": v.value, "variants": [{"grid_scale": float(sc[k]), "concurrent": bool(cc[k]),
+        return {"variant": v.value, "variants": [{"grid_scale": float(sc[k]), "streams": int(cc[k]),
                                                   "ms": round(float(ms[k]), 4)} for k in range(n)],
                 "launches": self.shape()}
 

@@ -170,14 +170,15 @@ int ldpc_gpu_decode_batch_nms(ldpc_gpu_ctx* ctx, const ldpc_gpu_frame* frames, i
  *
  * ldpc_gpu_plan_tune is the autotuner: it times the shapes common to all
  * launches (the persistent grid per launch, i.e. frames in flight, at
- * 100 / 75 / 50 % of what fits, and concurrent or serial launches) over
- * reps replays, keeps the fastest, then tunes each launch (one per kernel
+ * 100 / 75 / 50 % of what fits; the launches over 2, 3 or 4 forked streams
+ * or one after another) over reps replays, keeps the fastest, then tunes each launch (one per kernel
  * instance and frame size) in turn, longest first: its workgroups per CU
  * (1, or 2 where built: plain decoding of short frames on instances 2, 4,
  * 6 and 106) and its grid scale, keeping a change where the whole replay
  * gets faster by more than 0.5 %.  ldpc_gpu_plan_info reports the start
- * variant chosen (-1 once a launch changed) and the variants' timings (ms
- * per replay, < 0 if not timed); ldpc_gpu_plan_shape the shape of each
+ * variant chosen (-1 once a launch changed), per variant its grid scale,
+ * its forked stream count (0: one after another; the `concurrent` array)
+ * and its timing (ms per replay, < 0 if not timed); ldpc_gpu_plan_shape the shape of each
  * launch: kernel instance (the bound ldpc_gpu_last_launches reports),
  * short frames or not, workgroups per CU, grid scale, workgroups launched
  * and frames; it returns the count.
