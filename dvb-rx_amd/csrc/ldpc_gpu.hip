@@ -124,6 +124,7 @@ struct ldpc_gpu_ctx {
     int timed;
     int force_scalar;        /* per-frame batches through the one-frame kernel */
     int lanes;               /* forked streams a decode's launches use; 0: one after the other on the context's stream */
+    int group_fallbacks;     /* calls with groups that the two-pass path left to the exact one */
     PkShape shape[NLAUNCH];  /* launch shape per (instance, size); zero: one workgroup per CU, full grid */
     /* learned tables: row 0 plain (S = 16, B = 1), then per code its rows */
     uint8_t* h_nS;
@@ -531,6 +532,8 @@ int ldpc_gpu_set_concurrent(ldpc_gpu_ctx* c, int concurrent)
     c->lanes = concurrent == 1 ? NAUX_DEFAULT : concurrent;
     return LDPC_GPU_OK;
 }
+
+int ldpc_gpu_group_fallbacks(ldpc_gpu_ctx* c) { return c ? c->group_fallbacks : -1; }
 
 int ldpc_gpu_set_kernel(ldpc_gpu_ctx* c, int kernel)
 {
@@ -1066,7 +1069,7 @@ static int check_frames(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nfram
         df[f].code = s.code;
         df[f].max_iter = s.max_iter;
         df[f].nbits = s.nbits ? s.nbits : cd.k;
-        df[f].pad = 0;
+        df[f].min_iter = 0;
         df[f].llr_off = s.llr_off;
         df[f].out_off = s.out_off;
         *maxw = std::max(*maxw, cd.msg_words);
@@ -1111,6 +1114,53 @@ extern "C" int ldpc_gpu_decode_batch_nms(ldpc_gpu_ctx* c, const ldpc_gpu_frame* 
     return launch_pk(c, c->d_frames, df, d_llr, d_out, d_iters, d_synd, true, fallback_max_iter, d_fallback);
 }
 
+/* a decode's iteration counts and syndrome flags, to the host */
+static int group_fetch(ldpc_gpu_ctx* c, int nframes, const int32_t* d_iters, const uint8_t* d_synd,
+                       std::vector<int32_t>& it, std::vector<uint8_t>& ok)
+{
+    HIPCHK(c, hipMemcpyAsync(it.data(), d_iters, sizeof(int32_t) * nframes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ok.data(), d_synd, nframes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LDPC_GPU_OK;
+}
+
+/* the groups of a call from its frames' own counts: gid[f] (-1: no group),
+ * per group its count T (the largest of its frames' counts; max_iter where
+ * a frame never converged) and max_iter (one per group) */
+static int group_targets(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, const std::vector<int32_t>& it,
+                         const std::vector<uint8_t>& ok, std::vector<int>& gid, std::vector<int>& gT,
+                         std::vector<int>& gmax)
+{
+    std::vector<int> keys, gall;
+    gid.assign(nframes, -1);
+    gT.clear();
+    gmax.clear();
+    for (int f = 0; f < nframes; f++) {
+        if (frames[f].group < 0)
+            continue;
+        auto pos = std::find(keys.begin(), keys.end(), frames[f].group);
+        int g;
+        if (pos == keys.end()) {
+            g = (int)keys.size();
+            keys.push_back(frames[f].group);
+            gT.push_back(0);
+            gmax.push_back(frames[f].max_iter);
+            gall.push_back(1);
+        } else {
+            g = (int)(pos - keys.begin());
+        }
+        gid[f] = g;
+        if (frames[f].max_iter != gmax[g])
+            return set_err(c, LDPC_GPU_EINVAL, "frames of group %d differ in max_iter", frames[f].group);
+        gT[g] = std::max(gT[g], (int)it[f]);
+        gall[g] &= ok[f];
+    }
+    for (size_t g = 0; g < gT.size(); g++)
+        if (!gall[g])
+            gT[g] = gmax[g];
+    return LDPC_GPU_OK;
+}
+
 extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* frames, int nframes, const int8_t* d_llr,
                                      uint8_t* d_out, int32_t* d_iters, uint8_t* d_synd)
 {
@@ -1137,55 +1187,59 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
     if (!ngroup)
         return launch(c, c->d_frames, df, every, maxw, d_llr, d_out, d_iters, d_synd, 0, nframes, 0, 0, 1);
 
-    /* Groups (the reference's batch semantics): decode every frame with its
-     * own early stop and keep its state; a group's count T is the largest
-     * of its frames' counts (max_iter if one never converged); the others
-     * continue to T; while some frame fails at T < max_iter, the whole
-     * group goes one more iteration.  Frames are deterministic, so this
-     * equals iterating the group in lock step. */
+    /* Groups (the reference's batch semantics, ldpc.c:614-638): a group
+     * iterates in lock step and stops at the first iteration where all its
+     * frames hold their checks, or at max_iter.  No group can stop before
+     * the largest of its frames' own first such iterations, so every frame
+     * is first decoded with its own early stop; a group's count T is the
+     * largest of its frames' counts (max_iter if one never converged). */
+    std::vector<int32_t> it(nframes), tgt(nframes);
+    std::vector<uint8_t> ok(nframes);
+    std::vector<int> gid, gT, gmax;
+    if (!c->force_scalar) {
+        /* Two-frame kernel: then every frame again from its LLRs for
+         * exactly its target (the syndrome looked at only there): a group's
+         * frames T, a frame of no group its own count.  Where each group's
+         * frames all hold their checks at T, or T is max_iter, that is the
+         * lock-step result; otherwise (a frame that had converged lost a
+         * check by T, rare) the call is decoded again by the exact path. */
+        if ((rc = launch_pk(c, c->d_frames, df, d_llr, d_out, d_iters, d_synd)) ||
+            (rc = group_fetch(c, nframes, d_iters, d_synd, it, ok)) ||
+            (rc = group_targets(c, frames, nframes, it, ok, gid, gT, gmax)))
+            return rc;
+        std::vector<DFrame> df2 = df;
+        for (int f = 0; f < nframes; f++) {
+            const int T = gid[f] >= 0 ? gT[gid[f]] : it[f];
+            df2[f].max_iter = df2[f].min_iter = T;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_frames, df2.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice,
+                                 c->stream));
+        if ((rc = launch_pk(c, c->d_frames, df2, d_llr, d_out, d_iters, d_synd)) ||
+            (rc = group_fetch(c, nframes, d_iters, d_synd, it, ok)))
+            return rc;
+        bool exact = true;
+        for (int f = 0; f < nframes; f++)
+            if (gid[f] >= 0 && gT[gid[f]] < gmax[gid[f]] && !ok[f])
+                exact = false;
+        c->group_fallbacks += !exact;
+        if (exact)
+            return LDPC_GPU_OK;
+        HIPCHK(c, hipMemcpyAsync(c->d_frames, df.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    /* One-frame kernel, exact in every case: the first pass keeps each
+     * frame's state; frames below their group's T continue to T; while some
+     * frame fails at T < max_iter, the whole group goes one more iteration.
+     * Frames are deterministic, so this equals iterating the group in lock
+     * step. */
     if ((rc = grow(c, &c->state, &c->state_bytes, (size_t)nframes * 64800)) ||
         (rc = grow(c, &c->it_state, &c->it_state_bytes, sizeof(int32_t) * nframes)) ||
         (rc = grow(c, &c->target, &c->target_bytes, sizeof(int32_t) * nframes)))
         return rc;
     rc = launch(c, c->d_frames, df, every, maxw, d_llr, d_out, d_iters, d_synd, 1, nframes, 0, 1, 1);
-    if (rc)
+    if (rc || (rc = group_fetch(c, nframes, d_iters, d_synd, it, ok)) ||
+        (rc = group_targets(c, frames, nframes, it, ok, gid, gT, gmax)))
         return rc;
-    std::vector<int32_t> it(nframes), tgt(nframes);
-    std::vector<uint8_t> ok(nframes);
-    HIPCHK(c, hipMemcpyAsync(it.data(), d_iters, sizeof(int32_t) * nframes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(ok.data(), d_synd, nframes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    /* group -> (T, all ok, max_iter) */
-    std::vector<int> gid(nframes);
-    std::vector<int> gT, gmax, gall;
-    {
-        std::vector<int> keys;
-        for (int f = 0; f < nframes; f++) {
-            if (frames[f].group < 0) {
-                gid[f] = -1;
-                continue;
-            }
-            auto pos = std::find(keys.begin(), keys.end(), frames[f].group);
-            int g;
-            if (pos == keys.end()) {
-                g = (int)keys.size();
-                keys.push_back(frames[f].group);
-                gT.push_back(0);
-                gmax.push_back(frames[f].max_iter);
-                gall.push_back(1);
-            } else {
-                g = (int)(pos - keys.begin());
-            }
-            gid[f] = g;
-            if (frames[f].max_iter != gmax[g])
-                return set_err(c, LDPC_GPU_EINVAL, "frames of group %d differ in max_iter", frames[f].group);
-            gT[g] = std::max(gT[g], (int)it[f]);
-            gall[g] &= ok[f];
-        }
-    }
-    for (size_t g = 0; g < gT.size(); g++)
-        if (!gall[g])
-            gT[g] = gmax[g];
     for (;;) {
         std::vector<int> list;
         for (int f = 0; f < nframes; f++) {
@@ -1289,7 +1343,7 @@ extern "C" int ldpc_gpu_decode_batch_static(ldpc_gpu_ctx* c, const ldpc_gpu_fram
         d.code = s.code;
         d.max_iter = s.max_iter;
         d.nbits = s.nbits ? s.nbits : cd.k;
-        d.pad = 0;
+        d.min_iter = 0;
         d.llr_off = s.llr_off;
         d.out_off = s.out_off;
         const int k = cd.sk_inst * 2 + (cd.n < 64800);

@@ -50,7 +50,8 @@ struct DCode {
 };
 
 struct DFrame {
-    int code, max_iter, nbits, pad;
+    int code, max_iter, nbits;
+    int min_iter; /* the two-frame kernel checks the syndrome from this iteration on (groups' second pass) */
     unsigned long long llr_off, out_off;
 };
 

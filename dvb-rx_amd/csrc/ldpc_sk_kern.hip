@@ -348,6 +348,7 @@ __global__ void __launch_bounds__(SK_NT, 3 * SK_OCC) ldpc_sk_kernel(PkArgs a)
     int seg = -1;                 /* current segment (uniform) */
     int frame[2] = { -1, -1 };    /* frame in each slot, -1: empty */
     int it[2] = { 0, 0 }, maxit[2] = { 0, 0 };
+    int minit[2] = { 0, 0 }; /* iterations before the syndrome counts (DFrame.min_iter) */
     int phase[2] = { 0, 0 }, lit[2] = { 0, 0 }; /* learned mode: 1 = plain fallback; learned iterations */
     const rsrc_t rt = mk_rsrc(a.adt);
     const rsrc_t rm = mk_rsrc(a.msg + a.msg_stride * (size_t)blockIdx.x);
@@ -430,6 +431,7 @@ __global__ void __launch_bounds__(SK_NT, 3 * SK_OCC) ldpc_sk_kernel(PkArgs a)
                 continue;
             it[s] = 0;
             maxit[s] = a.frames[f].max_iter;
+            minit[s] = a.frames[f].min_iter;
             phase[s] = 0;
             lit[s] = 0;
             load_slot(s, f);
@@ -438,8 +440,12 @@ __global__ void __launch_bounds__(SK_NT, 3 * SK_OCC) ldpc_sk_kernel(PkArgs a)
 
         /* ---- iterate until a slot is done ---- */
         for (;;) {
-            const uint32_t live = (frame[0] >= 0 ? 1u : 0u) | (frame[1] >= 0 ? 2u : 0u);
-            const uint32_t good = sk_syndrome(lds16, rt, skl, tid, q, ln.va, live, s_vote, nvote);
+            /* the slots whose syndrome counts at this iteration; none: no
+             * syndrome pass at all */
+            uint32_t live = 0;
+            for (int s = 0; s < 2; s++)
+                live |= (frame[s] >= 0 && it[s] >= minit[s]) ? 1u << s : 0u;
+            const uint32_t good = live ? sk_syndrome(lds16, rt, skl, tid, q, ln.va, live, s_vote, nvote) & live : 0u;
             uint32_t done = 0;
             for (int s = 0; s < 2; s++) {
                 const int cap = NMS && phase[s] ? a.fallback : maxit[s];

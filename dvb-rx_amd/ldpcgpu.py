@@ -98,6 +98,7 @@ def gpu_lib():
         L.ldpc_gpu_code_max_degree.argtypes = [i32]
         L.ldpc_gpu_code_kernel_degree.argtypes = [i32]
         L.ldpc_gpu_set_kernel.argtypes = [vp, i32]
+        L.ldpc_gpu_group_fallbacks.argtypes = [vp]
         L.ldpc_gpu_set_nms.argtypes = [vp, i32, i32, vp, vp]
         L.ldpc_gpu_plan_create.argtypes = [vp, vp, i32, vp, vp, vp, vp, C.POINTER(vp)]
         L.ldpc_gpu_plan_launch.argtypes = [vp]
@@ -369,6 +370,11 @@ class Decoder:
         frames = np.ascontiguousarray(frames, FRAME_DTYPE)
         self._check(self.L.ldpc_gpu_decode_batch_nms(self.ctx, frames.ctypes.data, len(frames), d_llr, d_out, d_iters,
                                                      d_synd, fallback, d_fell), "decode_batch_nms")
+
+    def group_fallbacks(self):
+        """Calls whose groups needed the exact one-frame path after the
+        two-pass attempt."""
+        return self.L.ldpc_gpu_group_fallbacks(self.ctx)
 
     def last_kernel_ms(self):
         return self.L.ldpc_gpu_last_kernel_ms(self.ctx)

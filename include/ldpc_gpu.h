@@ -301,6 +301,17 @@ int ldpc_gpu_t2_decode_host(ldpc_gpu_ctx* ctx, const ldpc_gpu_t2_plp* plp, int n
 #define LDPC_GPU_KERNEL_SINGLE 1
 int ldpc_gpu_set_kernel(ldpc_gpu_ctx* ctx, int kernel);
 
+/* Frames in groups (group >= 0, the reference's lock-step batches) are
+ * decoded in two passes of the two-frame kernel: each frame with its own
+ * early stop, then every frame again to its group's count (the largest of
+ * its frames' first passing iterations, which no group can stop before),
+ * checked there.  Where a group's frames do not all hold their checks at
+ * that count (a converged frame lost a check, rare), the call is decoded
+ * again by the exact one-frame path.  The count of such calls on this
+ * context (no counterpart in the reference).  LDPC_GPU_KERNEL_SINGLE takes
+ * the exact path directly. */
+int ldpc_gpu_group_fallbacks(ldpc_gpu_ctx* ctx);
+
 /* Launch mode of ldpc_gpu_decode_batch (no counterpart in the reference).
  * 1 (the default): the launches of one call - one per kernel instance and
  * frame size - run concurrently, spread by expected work over three forked

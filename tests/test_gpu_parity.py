@@ -227,3 +227,37 @@ def test_reference_batch_directly_when_built():
             assert it == r_it and (bits == r_bits).all(), name
     finally:
         d.close()
+
+
+@pytest.mark.parametrize("short,rate,esn0", [(0, "1_2", 1.05), (1, "3_4", 3.95), (0, "3_5", 2.35), (1, "1_4", -1.7)])
+def test_many_groups_two_pass_against_oracle(short, rate, esn0):
+    """40 groups of 6 frames near threshold (stops spread, some groups fail):
+    the two-pass group path equals the oracle's lock-step decode of every
+    group, and the exact one-frame path; how many calls fell back to the
+    exact path is reported."""
+    fec, llr, _ = gen(short, rate, esn0, 240, seed=77)
+    code = ldpcgpu.code_lookup(short, rate)
+    n, N = llr.shape
+    kb = (fec.kldpc + 7) // 8
+    fr = np.zeros(n, ldpcgpu.FRAME_DTYPE)
+    for f in range(n):
+        fr[f] = (code, 25, fec.kldpc, f // 6, f * N, f * kb)
+    res = {}
+    for kernel in ("auto", "single"):
+        d = ldpcgpu.Decoder(0, kernel=kernel)
+        try:
+            res[kernel] = d.decode_host(fr, llr.reshape(-1), n * kb)
+            print(f"# {rate}: {kernel} fallbacks {d.group_fallbacks()}", flush=True)
+        finally:
+            d.close()
+    for a, b in zip(res["auto"], res["single"]):
+        assert (a == b).all()
+    out, it, ok = res["auto"]
+    o = Oracle(short, rate, fec.kldpc)
+    for g in range(0, n // 6, 4):  # every fourth group against the oracle
+        o_it, o_bits = o.group(llr[6 * g:6 * g + 6], 25)
+        for k in range(6):
+            f = 6 * g + k
+            assert it[f] == (o_it if o_it >= 0 else 25), (g, k)
+            assert (out[f * kb:(f + 1) * kb] == o_bits[k]).all(), (g, k)
+    assert len(set(int(x) for x in it[::6])) > 3, "group counts should spread"
