@@ -1031,8 +1031,10 @@ extern "C" int ldpc_gpu_set_nms(ldpc_gpu_ctx* c, int code, int T, const uint8_t*
         for (int k = 0; k < NMS_ROWS * LDPC_NMS_DEGS; k++)
             c->h_nB[k] = 16;
         HIPCHK(c, hipSetDevice(c->device));
-        HIPCHK(c, hipMalloc((void**)&c->d_nS, NMS_ROWS * LDPC_NMS_DEGS));
-        HIPCHK(c, hipMalloc((void**)&c->d_nB, NMS_ROWS * LDPC_NMS_DEGS * sizeof(int16_t)));
+        /* a dword of padding each: the kernel reads entries as aligned
+         * dwords (nms_u8, nms_i16) */
+        HIPCHK(c, hipMalloc((void**)&c->d_nS, NMS_ROWS * LDPC_NMS_DEGS + 4));
+        HIPCHK(c, hipMalloc((void**)&c->d_nB, NMS_ROWS * LDPC_NMS_DEGS * sizeof(int16_t) + 4));
         HIPCHK(c, hipMalloc((void**)&c->d_nrow0, 64 * sizeof(int)));
         HIPCHK(c, hipMalloc((void**)&c->d_nT, 64 * sizeof(int)));
     }

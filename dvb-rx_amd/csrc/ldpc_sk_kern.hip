@@ -120,6 +120,23 @@ __device__ __forceinline__ uint4 sl_desc(const uint4* skl, int i)
 }
 
 __device__ __forceinline__ int sl_T(uint32_t x) { return (int)(x & 0xff); }
+
+/* entry k (uniform) of a learned table row, by a scalar load of the
+ * aligned dword that holds it: S rows are bytes, B rows 16-bit (even
+ * offsets); the device tables carry a dword of padding after their last
+ * row */
+__device__ __forceinline__ int nms_u8(const uint8_t* row, int k)
+{
+    const uintptr_t ad = (uintptr_t)(row + k);
+    const sl_cu32* w = (const sl_cu32*)(ad & ~(uintptr_t)3);
+    return (int)((w[0] >> (8 * (ad & 3))) & 0xffu);
+}
+__device__ __forceinline__ int nms_i16(const int16_t* row, int k)
+{
+    const uintptr_t ad = (uintptr_t)(row + k);
+    const sl_cu32* w = (const sl_cu32*)(ad & ~(uintptr_t)3);
+    return (int)(int16_t)(w[0] >> (8 * (ad & 3)));
+}
 __device__ __forceinline__ int sl_nlev(uint32_t x) { return (int)((x >> 8) & 0xff); }
 __device__ __forceinline__ bool sl_bar_after(uint32_t x) { return (x >> 17) & 1; }
 __device__ __forceinline__ int sl_kde(uint32_t x) { return (int)(x >> 24); }
@@ -292,9 +309,20 @@ __device__ __forceinline__ void sk_iteration(uint16_t* lds, rsrc_t rt, const uin
          * node (0, 0) has no previous parity, one edge less */
         lg_pk ps = pk_dup(16), pb = pk_dup(16);
         if constexpr (NMS) {
-            const int dc = T - (i == 0 && j == 0);
-            ps = lg_pk{ (short)rows.S0[dc], (short)rows.S1[dc] };
-            pb = lg_pk{ (short)rows.B0[dc], (short)rows.B1[dc] };
+            int s0 = nms_u8(rows.S0, T), s1 = nms_u8(rows.S1, T);
+            int b0 = nms_i16(rows.B0, T), b1 = nms_i16(rows.B1, T);
+            if (i == 0) {
+                const int e0 = nms_u8(rows.S0, T - 1), e1 = nms_u8(rows.S1, T - 1);
+                const int f0 = nms_i16(rows.B0, T - 1), f1 = nms_i16(rows.B1, T - 1);
+                if (j == 0) {
+                    s0 = e0;
+                    s1 = e1;
+                    b0 = f0;
+                    b1 = f1;
+                }
+            }
+            ps = lg_pk{ (short)s0, (short)s1 };
+            pb = lg_pk{ (short)b0, (short)b1 };
         }
         if (nlev > 1)
             sk_layer<true>(n, raw, lds, T, h, skip0, nlev, sl_kde(ld.x), mw, tw[0], tw + 1, rm, ln.vm, ld.y, se, so, ps,
