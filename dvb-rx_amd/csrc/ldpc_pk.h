@@ -127,6 +127,19 @@ LG_PK_HD lg_pk pk_nms_mag(lg_pk m, lg_pk ps, lg_pk pb)
     return pk_min(pk_max(p, pk_dup(0)), pk_dup(127)) << (short)8;
 }
 
+/* pk_nms_mag(pk_mag1(a), ps, pb) from a ones' complement magnitude a, with
+ * nb = -pb: min((a +sat 0x80) >> 8, 127) is min(m, 127) (the shift drops
+ * the low byte that pk_mag1 masks), and v * S + nb a multiply-add; no
+ * intermediate leaves int16 (|v * S - B| < 2^15) */
+LG_PK_HD lg_pk pk_nms_mag_a(lg_pk a, lg_pk ps, lg_pk nb)
+{
+    const lg_pku v = __builtin_elementwise_min(__builtin_elementwise_add_sat(pku(a), lg_pku{ 0x80, 0x80 }) >>
+                                                   (unsigned short)8,
+                                               lg_pku{ 127, 127 });
+    const lg_pk p = pks(v * pku(ps) + pku(nb)) >> (short)4;
+    return pk_min(pk_max(p, pk_dup(0)), pk_dup(127)) << (short)8;
+}
+
 /* min(d, 1) * dmn + mn1 per lane as two instructions: written out so that
  * the compiler does not turn them into per-lane compares and selects */
 LG_PK_HD lg_pk pk_sel_mag(lg_pk d, lg_pk dmn, lg_pk mn1)

@@ -151,15 +151,16 @@ LG_PK_HD void sk_combine(SkHalf<KS>& n, lg_pk p0, lg_pk p1, lg_pk ps)
     n.s = n.s ^ ps;
 }
 
+/* NMS: the learned magnitudes with scales ps and negated offsets nb */
 template <int KS, bool NMS>
-LG_PK_HD PkOut sk_mags(const SkHalf<KS>& n, lg_pk ps, lg_pk pb)
+LG_PK_HD PkOut sk_mags(const SkHalf<KS>& n, lg_pk ps, lg_pk nb)
 {
     if constexpr (NMS) {
-        const lg_pk mn0 = pk_nms_mag(pk_mag1(n.min0), ps, pb), mn1 = pk_nms_mag(pk_mag1(n.min1), ps, pb);
+        const lg_pk mn0 = pk_nms_mag_a(n.min0, ps, nb), mn1 = pk_nms_mag_a(n.min1, ps, nb);
         return PkOut{ mn1, mn0 - mn1 };
     } else {
         (void)ps;
-        (void)pb;
+        (void)nb;
         /* max(min(m, 127) - 1, 0) of m = (a + 0x80) >> 8, in units of 256:
          * min((a -sat 0x80) & 0xff00, 126 * 256) */
         const lg_pk mn0 = pk_minu(pk_of(pk_bits(pk_subu_sat(n.min0, 0x80)) & 0xFF00FF00u), pk_dup(0x7E00));
@@ -191,9 +192,9 @@ LG_PK_HD lg_pk sk_out(const SkHalf<KS>& n, const PkOut& m, int k)
  * by a later node of the layer) and the new messages, clamped to [-32,
  * 31] (ldpc.c:343), word w to mo[w] */
 template <int KS, bool TAIL, bool NMS>
-LG_PK_HD void sk_phase_c(const SkHalf<KS>& n, uint16_t* post, uint32_t succ, uint32_t* mo, lg_pk ps, lg_pk pb)
+LG_PK_HD void sk_phase_c(const SkHalf<KS>& n, uint16_t* post, uint32_t succ, uint32_t* mo, lg_pk ps, lg_pk nb)
 {
-    const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
+    const PkOut m = sk_mags<KS, NMS>(n, ps, nb);
     /* the slots' magnitudes first, two at a time */
     lg_pk mag[KS];
 #pragma unroll
@@ -298,9 +299,9 @@ LG_PK_HD void sk_fold_dep0(SkHalf<KS>& n, uint32_t dep, int kde)
  * node's combined minimums */
 template <int KS, bool NMS>
 LG_PK_HD void sk_put_hand(const SkHalf<KS>& n, uint16_t* hb, const uint32_t* hw, uint32_t succ, int kde, lg_pk ps,
-                          lg_pk pb)
+                          lg_pk nb)
 {
-    const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
+    const PkOut m = sk_mags<KS, NMS>(n, ps, nb);
 #pragma unroll
     for (int k = 1; k <= SK_KD(KS); k++) {
         if (k >= kde)
@@ -329,9 +330,9 @@ LG_PK_HD void sk_prev_hand(uint32_t* old, const uint16_t* hb, const uint32_t* hw
  * point */
 template <int KS, bool NMS>
 LG_PK_HD uint32_t sk_put_hand_chg(const SkHalf<KS>& n, uint16_t* hb, const uint32_t* hw, uint32_t succ, int kde,
-                                  const uint32_t* old, lg_pk ps, lg_pk pb)
+                                  const uint32_t* old, lg_pk ps, lg_pk nb)
 {
-    const PkOut m = sk_mags<KS, NMS>(n, ps, pb);
+    const PkOut m = sk_mags<KS, NMS>(n, ps, nb);
     uint32_t chg = 0;
 #pragma unroll
     for (int k = 1; k <= SK_KD(KS); k++) {

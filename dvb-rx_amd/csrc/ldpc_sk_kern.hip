@@ -213,7 +213,7 @@ template <bool TAIL>
 __device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uint16_t* lds, int T, int h,
                                          uint32_t skip0, int nlev, int kde, const uint32_t* mw, uint32_t tm,
                                          const uint32_t* hw, rsrc_t rm, uint32_t vm, uint32_t moff, uint32_t se,
-                                         uint32_t so, lg_pk ps, lg_pk pb, uint16_t* hb, int hstride,
+                                         uint32_t so, lg_pk ps, lg_pk nb, uint16_t* hb, int hstride,
                                          unsigned* s_vote, int& nvote)
 {
     const uint32_t dep = TAIL ? (tm & 0xffffu) : 0, succ = TAIL ? (tm >> 16) : 0;
@@ -222,7 +222,7 @@ __device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uin
         const PkPart part = sk_part(n);
         sk_fold_dep0(n, dep, kde);
         sk_pair(n);
-        sk_put_hand<KS, NMS>(n, hb, hw, succ, kde, ps, pb);
+        sk_put_hand<KS, NMS>(n, hb, hw, succ, kde, ps, nb);
         __syncthreads();
 #pragma unroll 1
         for (int r = 1; r < nlev; r++) {
@@ -234,7 +234,7 @@ __device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uin
             sk_pair(n);
             if (r == nlev - 1)
                 break;
-            const uint32_t chg = sk_put_hand_chg<KS, NMS>(n, hb + (r & 1) * hstride, hw, succ, kde, old, ps, pb);
+            const uint32_t chg = sk_put_hand_chg<KS, NMS>(n, hb + (r & 1) * hstride, hw, succ, kde, old, ps, nb);
             if (wg_and2(chg ? 0u : 3u, s_vote, nvote) & 1)
                 break;
         }
@@ -242,7 +242,7 @@ __device__ __forceinline__ void sk_layer(SkHalf<KS>& n, const uint32_t* raw, uin
         sk_pair(n);
     }
     uint32_t mo[WA];
-    sk_phase_c<KS, TAIL, NMS>(n, lds, succ, mo, ps, pb);
+    sk_phase_c<KS, TAIL, NMS>(n, lds, succ, mo, ps, nb);
     st_words<WA>(mo, rm, vm, moff);
 }
 
@@ -305,9 +305,9 @@ __device__ __forceinline__ void sk_iteration(uint16_t* lds, rsrc_t rt, const uin
         sk_read<KS>(raw, lds, n);
         const int T = sl_T(ld.x), nlev = sl_nlev(ld.x);
         const uint32_t skip0 = (i == 0 && t == 1) ? ~0u : 0u;
-        /* learned scale / offset of the layer's check degree, per frame;
+        /* learned scale / negated offset of the layer's check degree, per frame;
          * node (0, 0) has no previous parity, one edge less */
-        lg_pk ps = pk_dup(16), pb = pk_dup(16);
+        lg_pk ps = pk_dup(16), nb = pk_dup(-16);
         if constexpr (NMS) {
             int s0 = nms_u8(rows.S0, T), s1 = nms_u8(rows.S1, T);
             int b0 = nms_i16(rows.B0, T), b1 = nms_i16(rows.B1, T);
@@ -322,13 +322,13 @@ __device__ __forceinline__ void sk_iteration(uint16_t* lds, rsrc_t rt, const uin
                 }
             }
             ps = lg_pk{ (short)s0, (short)s1 };
-            pb = lg_pk{ (short)b0, (short)b1 };
+            nb = lg_pk{ (short)-b0, (short)-b1 };
         }
         if (nlev > 1)
             sk_layer<true>(n, raw, lds, T, h, skip0, nlev, sl_kde(ld.x), mw, tw[0], tw + 1, rm, ln.vm, ld.y, se, so, ps,
-                           pb, hb, hstride, s_vote, nvote);
+                           nb, hb, hstride, s_vote, nvote);
         else
-            sk_layer<false>(n, raw, lds, T, h, skip0, nlev, 0, mw, 0, tw + 1, rm, ln.vm, ld.y, se, so, ps, pb, hb,
+            sk_layer<false>(n, raw, lds, T, h, skip0, nlev, 0, mw, 0, tw + 1, rm, ln.vm, ld.y, se, so, ps, nb, hb,
                             hstride, s_vote, nvote);
         /* the next layer waits for this one where it reads what another
          * wave wrote (its barrier flag; the end of the iteration always) */

@@ -182,7 +182,7 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
     const bool tail = nlev > 1;
     static SkHalf<KS> n[720];
     static uint32_t mw[720][8], tm[720], hw[720][8];
-    static lg_pk ps[720], pb[720];
+    static lg_pk ps[720], nb[720];
     static PkPart part[720];
     uint32_t* m = msg + L[1];
     for (int t = 0; t < 720; t++) {
@@ -198,11 +198,11 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
         sk_addrs(n[t], ar);
         const int j = t >> 1;
         ps[t] = lg_pk{ 16, 16 };
-        pb[t] = lg_pk{ 16, 16 };
+        nb[t] = lg_pk{ -16, -16 };
         if (NMS) {
             const int dc = T - (i == 0 && j == 0);
             ps[t] = lg_pk{ (short)rows.S[0][dc], (short)rows.S[1][dc] };
-            pb[t] = lg_pk{ (short)rows.B[0][dc], (short)rows.B[1][dc] };
+            nb[t] = lg_pk{ (short)-rows.B[0][dc], (short)-rows.B[1][dc] };
         }
     }
     for (int t = 719; t >= 0; t--) {
@@ -222,7 +222,7 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
         }
         sk_model_pair(n);
         for (int t = 719; t >= 0; t--)
-            sk_put_hand<KS, NMS>(n[t], hb[0], hw[t], tm[t] >> 16, kde, ps[t], pb[t]);
+            sk_put_hand<KS, NMS>(n[t], hb[0], hw[t], tm[t] >> 16, kde, ps[t], nb[t]);
         static uint32_t old[720][KS];
         for (int r = 1; r < nlev; r++) {
             uint32_t chg = 0;
@@ -236,7 +236,7 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
             if (r == nlev - 1)
                 break;
             for (int t = 719; t >= 0; t--)
-                chg |= sk_put_hand_chg<KS, NMS>(n[t], hb[r & 1], hw[t], tm[t] >> 16, kde, old[t], ps[t], pb[t]);
+                chg |= sk_put_hand_chg<KS, NMS>(n[t], hb[r & 1], hw[t], tm[t] >> 16, kde, old[t], ps[t], nb[t]);
             if (!chg)
                 break;
         }
@@ -246,9 +246,9 @@ static void sk_model_layer(const lg_code* c, int i, uint16_t* post, uint32_t* ms
     }
     for (int t = 719; t >= 0; t--) {
         if (tail)
-            sk_phase_c<KS, true, NMS>(n[t], post - SK_POST_OFF / 2, tm[t] >> 16, m + (size_t)t * WA, ps[t], pb[t]);
+            sk_phase_c<KS, true, NMS>(n[t], post - SK_POST_OFF / 2, tm[t] >> 16, m + (size_t)t * WA, ps[t], nb[t]);
         else
-            sk_phase_c<KS, false, NMS>(n[t], post - SK_POST_OFF / 2, 0, m + (size_t)t * WA, ps[t], pb[t]);
+            sk_phase_c<KS, false, NMS>(n[t], post - SK_POST_OFF / 2, 0, m + (size_t)t * WA, ps[t], nb[t]);
     }
 }
 
