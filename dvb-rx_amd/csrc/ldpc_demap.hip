@@ -147,8 +147,14 @@ __device__ __forceinline__ int8_t quantise(float v)
  * inside a block by a fixed tree of minima (pairs, quads, halves), the
  * high bits by the block's minimum; minima are exact, so the grouping does
  * not change them. */
+typedef const __attribute__((address_space(4))) float dm_cf32;
+
+/* the point tables are read at uniform indices: from the global table
+ * through the constant address space they become scalar loads (16 points
+ * a block in three 64-byte loads) whose values feed the multiply-adds
+ * directly */
 template <int NB>
-__device__ __forceinline__ void minima_full(const float* ar, const float* ai, const float* e, float x, float y,
+__device__ __forceinline__ void minima_full(const dm_cf32* ar, const dm_cf32* ai, const dm_cf32* e, float x, float y,
                                             float* m0, float* m1)
 {
 #pragma unroll
@@ -280,7 +286,7 @@ __global__ void __launch_bounds__(DM_BLOCK) demap_kernel(const DemapTab* tab, co
     if (NB == 8 && s.ring)
         minima_ring(s, v.x, v.y, m0, m1);
     else
-        minima_full<NB>(s.ar, s.ai, s.e, v.x, v.y, m0, m1);
+        minima_full<NB>((const dm_cf32*)tab->ar, (const dm_cf32*)tab->ai, (const dm_cf32*)tab->e, v.x, v.y, m0, m1);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const int pos = s.order[b] * nsym + j;
