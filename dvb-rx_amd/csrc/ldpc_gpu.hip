@@ -843,6 +843,8 @@ static int pk_layout(ldpc_gpu_ctx* c, const std::vector<DFrame>& hf, bool nms, c
         L.cost[k] = 0.0;
     }
     for (int f = 0; f < (int)hf.size(); f++) {
+        if (hf[f].max_iter < 0)
+            continue; /* left out of this decode: its outputs stay as they are */
         const DCode& cd = hc[hf[f].code];
         const int k = cd.sk_inst * 2 + (cd.n < 64800);
         L.cost[k] += (double)cd.q * SK_INST[cd.sk_inst][0];
@@ -1209,16 +1211,24 @@ extern "C" int ldpc_gpu_decode_batch(ldpc_gpu_ctx* c, const ldpc_gpu_frame* fram
             (rc = group_fetch(c, nframes, d_iters, d_synd, it, ok)) ||
             (rc = group_targets(c, frames, nframes, it, ok, gid, gT, gmax)))
             return rc;
+        /* a frame whose own count is already its target ran exactly that
+         * many iterations in the first pass: same state, same outputs, so
+         * it sits the second pass out (max_iter -1), as does every frame
+         * of no group */
         std::vector<DFrame> df2 = df;
+        int again = 0;
         for (int f = 0; f < nframes; f++) {
             const int T = gid[f] >= 0 ? gT[gid[f]] : it[f];
-            df2[f].max_iter = df2[f].min_iter = T;
+            df2[f].max_iter = df2[f].min_iter = T == it[f] ? -1 : T;
+            again += T != it[f];
         }
-        HIPCHK(c, hipMemcpyAsync(c->d_frames, df2.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice,
-                                 c->stream));
-        if ((rc = launch_pk(c, c->d_frames, df2, d_llr, d_out, d_iters, d_synd)) ||
-            (rc = group_fetch(c, nframes, d_iters, d_synd, it, ok)))
-            return rc;
+        if (again) {
+            HIPCHK(c, hipMemcpyAsync(c->d_frames, df2.data(), sizeof(DFrame) * nframes, hipMemcpyHostToDevice,
+                                     c->stream));
+            if ((rc = launch_pk(c, c->d_frames, df2, d_llr, d_out, d_iters, d_synd)) ||
+                (rc = group_fetch(c, nframes, d_iters, d_synd, it, ok)))
+                return rc;
+        }
         bool exact = true;
         for (int f = 0; f < nframes; f++)
             if (gid[f] >= 0 && gT[gid[f]] < gmax[gid[f]] && !ok[f])
