@@ -1689,7 +1689,11 @@ extern "C" int ldpc_gpu_plan_tune(ldpc_gpu_plan* p, int reps)
         const int k = order[i];
         const PkShape keep = p->shape[k];
         PkShape pick = keep;
-        for (int occ = 1; occ <= (sk_has_occ2(k / 2, false) ? 2 : 1); occ++)
+        /* two workgroups per CU only where they fit: short-frame launches
+         * (odd k) of the instances built for it; a normal frame pair
+         * holds most of a CU's LDS */
+        const int max_occ = (k & 1) && sk_has_occ2(k / 2, false) ? 2 : 1;
+        for (int occ = 1; occ <= max_occ; occ++)
             for (float sc : SCALES) {
                 const PkShape t = PkShape{ occ, sc };
                 if (pk_occ_of(k, false, t) == pk_occ_of(k, false, keep) && pk_scale_of(t) == pk_scale_of(keep))
